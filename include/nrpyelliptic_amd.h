@@ -1,0 +1,281 @@
+/*
+ * nrpyelliptic_amd.h -- C ABI of libnrpyell_hip.so, the MI355X (gfx950) implementation
+ * of NRPyElliptic's hyperbolic-relaxation solve for binary-puncture initial data.
+ *
+ * Part 1 is the drop-in surface: the structs and functions of the reference driver
+ * (NRPyElliptic_Ccodes/NRPy_basic_defines.h, NRPy_function_prototypes.h) with the same
+ * names, member order, prototypes and argument meaning, so a caller of the reference
+ * links against this library instead.  The one change of contract is WHERE arrays live:
+ *   - gridfunction arrays (MoL_gridfunctions_struct members, auxevol_gfs, in_gfs,
+ *     rhs_gfs, residual_gf, in_gf) and rfm_struct arrays are DEVICE pointers, allocated
+ *     by this library's MoL_malloc_* / rfm_precompute_rfmstruct_malloc;
+ *   - xx[3] coordinate arrays and bc_struct tables stay in HOST memory, exactly as the
+ *     reference lays them out (bcstruct_set_up output is byte-identical).
+ * Every entry point is synchronous on return, like the reference's OpenMP functions.
+ * Fatal errors print to stderr and exit(1), as the reference does
+ * (bcstruct_set_up.c:178-184,272-277); HIP errors are reported the same way.
+ *
+ * Part 2 is this library's own API (nrpyell_*): a resident solver with hipGraph-captured
+ * RK4 steps and on-device residual checks, FP32->FP64 mixed precision, a batched
+ * multi-configuration solver with an autotuner, and the learned initial guess.
+ * None of it has a counterpart in the reference; it returns error codes instead of
+ * exiting.
+ */
+#ifndef NRPYELLIPTIC_AMD_H
+#define NRPYELLIPTIC_AMD_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+#define NRPYELL_RESTRICT /* restrict does not change the ABI; omitted for C++ callers */
+extern "C" {
+#else
+#define NRPYELL_RESTRICT restrict
+#endif
+
+#define REAL double
+#define NGHOSTS 5
+#define NUM_EVOL_GFS 2
+#define UUGF 0
+#define VVGF 1
+#define NUM_AUXEVOL_GFS 3
+#define ADD_TIMES_AUUGF 0
+#define PSI_BACKGROUNDGF 1
+#define WAVESPEEDGF 2
+#define EXTRAPOLATION_OUTER_BCS 0
+#define RADIATION_OUTER_BCS 1
+
+/* ---- Part 1: reference ABI (NRPy_basic_defines.h) ------------------------------------ */
+
+/* NRPy_basic_defines.h:40-120. Member order is the ABI. */
+typedef struct __paramstruct__ {
+  REAL AMAX;
+  REAL Cart_CoM_offsetx, Cart_CoM_offsety, Cart_CoM_offsetz;
+  REAL Cart_originx, Cart_originy, Cart_originz;
+  REAL P0_x, P0_y, P0_z;
+  REAL P1_x, P1_y, P1_z;
+  REAL S0_x, S0_y, S0_z;
+  REAL S1_x, S1_y, S1_z;
+  REAL SINHWAA;
+  REAL bScale;
+  REAL bare_mass_0, bare_mass_1;
+  REAL dxx0, dxx1, dxx2;
+  REAL eta_damping;
+  REAL f0_of_xx0, f0_of_xx0__D0, f0_of_xx0__DD00, f0_of_xx0__DDD000;
+  REAL f1_of_xx1, f1_of_xx1__D1, f1_of_xx1__DD11, f1_of_xx1__DDD111;
+  REAL f2_of_xx0_xx1, f2_of_xx0_xx1__D0, f2_of_xx0_xx1__D1, f2_of_xx0_xx1__DD00, f2_of_xx0_xx1__DD11;
+  REAL f2_of_xx1, f2_of_xx1__D1, f2_of_xx1__DD11, f2_of_xx1__DDD111;
+  REAL f3_of_xx0, f3_of_xx0__D0, f3_of_xx0__DD00;
+  REAL f4_of_xx2, f4_of_xx2__D2, f4_of_xx2__DD22;
+  REAL invdx0, invdx1, invdx2;
+  REAL puncture_0_x, puncture_0_y, puncture_0_z;
+  REAL puncture_1_x, puncture_1_y, puncture_1_z;
+  REAL time;
+  REAL uu_wavespeed_at_OB, vv_wavespeed_at_OB;
+  REAL xx0, xx1, xx2;
+  REAL xxmax0, xxmax1, xxmax2;
+  REAL xxmin0, xxmin1, xxmin2;
+  int Nxx0, Nxx1, Nxx2;
+  int Nxx_plus_2NGHOSTS0, Nxx_plus_2NGHOSTS1, Nxx_plus_2NGHOSTS2;
+  int has_outer_boundary;
+  int outer_bc_type;
+} paramstruct;
+
+/* NRPy_basic_defines.h:135-151 (device arrays here). */
+typedef struct __rfmstruct__ {
+  REAL *NRPYELL_RESTRICT f0_of_xx0;
+  REAL *NRPYELL_RESTRICT f0_of_xx0__D0;
+  REAL *NRPYELL_RESTRICT f0_of_xx0__DD00;
+  REAL *NRPYELL_RESTRICT f0_of_xx0__DDD000;
+  REAL *NRPYELL_RESTRICT f1_of_xx1;
+  REAL *NRPYELL_RESTRICT f1_of_xx1__D1;
+  REAL *NRPYELL_RESTRICT f1_of_xx1__DD11;
+  REAL *NRPYELL_RESTRICT f2_of_xx0_xx1;
+  REAL *NRPYELL_RESTRICT f2_of_xx0_xx1__D0;
+  REAL *NRPYELL_RESTRICT f2_of_xx0_xx1__D1;
+  REAL *NRPYELL_RESTRICT f2_of_xx0_xx1__DD00;
+  REAL *NRPYELL_RESTRICT f2_of_xx0_xx1__DD11;
+  REAL *NRPYELL_RESTRICT f3_of_xx0;
+  REAL *NRPYELL_RESTRICT f3_of_xx0__D0;
+  REAL *NRPYELL_RESTRICT f3_of_xx0__DD00;
+} rfm_struct;
+
+/* NRPy_basic_defines.h:164-195 (host tables). */
+typedef struct __innerpt_bc_struct__ {
+  int dstpt;
+  int srcpt;
+  int8_t parity[10];
+} innerpt_bc_struct;
+
+typedef struct __outerpt_bc_struct__ {
+  short i0, i1, i2;
+  int8_t FACEX0, FACEX1, FACEX2;
+} outerpt_bc_struct;
+
+typedef struct __bc_info_struct__ {
+  int num_inner_boundary_points;
+  int num_pure_outer_boundary_points[NGHOSTS][3];
+  int bc_loop_bounds[NGHOSTS][6][6];
+} bc_info_struct;
+
+typedef struct __bc_struct__ {
+  innerpt_bc_struct *NRPYELL_RESTRICT inner_bc_array;
+  outerpt_bc_struct *NRPYELL_RESTRICT pure_outer_bc_array[NGHOSTS * 3];
+  bc_info_struct bc_info;
+} bc_struct;
+
+/* NRPy_basic_defines.h:206-213 (device arrays here). */
+typedef struct __MoL_gridfunctions_struct__ {
+  REAL *NRPYELL_RESTRICT y_n_gfs;
+  REAL *NRPYELL_RESTRICT y_nplus1_running_total_gfs;
+  REAL *NRPYELL_RESTRICT k_odd_gfs;
+  REAL *NRPYELL_RESTRICT k_even_gfs;
+  REAL *NRPYELL_RESTRICT auxevol_gfs;
+  REAL *NRPYELL_RESTRICT diagnostic_output_gfs;
+} MoL_gridfunctions_struct;
+
+/* NRPy_basic_defines.h:263-269 */
+typedef struct __griddata__ {
+  paramstruct params;
+  REAL *NRPYELL_RESTRICT xx[3];
+  bc_struct bcstruct;
+  rfm_struct rfmstruct;
+  MoL_gridfunctions_struct gridfuncs;
+} griddata_struct;
+
+/* ---- Part 1: reference functions (NRPy_function_prototypes.h) ------------------------- */
+
+/* NRPy_function_prototypes.h:20 / set_Cparameters_to_default.c:5 */
+void set_Cparameters_to_default(paramstruct *NRPYELL_RESTRICT params);
+/* :11 / set_Nxx_dxx_invdx_params__and__xx.c:5. xx[] are host arrays (malloc). */
+void set_Nxx_dxx_invdx_params__and__xx(const int EigenCoord, const int Nxx[3], paramstruct *NRPYELL_RESTRICT params,
+                                       REAL *NRPYELL_RESTRICT xx[3]);
+/* :6 / bcstruct_set_up.c:350. Host tables, byte-identical to the reference's. */
+void bcstruct_set_up(const paramstruct *NRPYELL_RESTRICT params, REAL *NRPYELL_RESTRICT xx[3],
+                     bc_struct *NRPYELL_RESTRICT bcstruct);
+/* :9 / find_timestep.c:5 */
+REAL find_timestep(const paramstruct *NRPYELL_RESTRICT params, REAL *NRPYELL_RESTRICT xx[3], const REAL CFL_FACTOR);
+/* :1-4 / MoL_malloc_y_n_gfs.c, MoL_malloc_non_y_n_gfs.c, MoL_free_memory_*.c: device memory */
+void MoL_malloc_y_n_gfs(const paramstruct *NRPYELL_RESTRICT params, MoL_gridfunctions_struct *NRPYELL_RESTRICT gridfuncs);
+void MoL_free_memory_y_n_gfs(const paramstruct *NRPYELL_RESTRICT params,
+                             MoL_gridfunctions_struct *NRPYELL_RESTRICT gridfuncs);
+void MoL_malloc_non_y_n_gfs(const paramstruct *NRPYELL_RESTRICT params,
+                            MoL_gridfunctions_struct *NRPYELL_RESTRICT gridfuncs);
+void MoL_free_memory_non_y_n_gfs(const paramstruct *NRPYELL_RESTRICT params,
+                                 MoL_gridfunctions_struct *NRPYELL_RESTRICT gridfuncs);
+/* :17-19 / rfm_precompute_rfmstruct_{malloc,define,freemem}.c: device arrays */
+void rfm_precompute_rfmstruct_malloc(const paramstruct *NRPYELL_RESTRICT params, rfm_struct *NRPYELL_RESTRICT rfmstruct);
+void rfm_precompute_rfmstruct_define(const paramstruct *NRPYELL_RESTRICT params, REAL *NRPYELL_RESTRICT xx[3],
+                                     rfm_struct *NRPYELL_RESTRICT rfmstruct);
+void rfm_precompute_rfmstruct_freemem(const paramstruct *NRPYELL_RESTRICT params, rfm_struct *NRPYELL_RESTRICT rfmstruct);
+/* :24 / initial_guess_all_points.c:7: u = v = 0 at all points (device) */
+void initial_guess_all_points(const paramstruct *NRPYELL_RESTRICT params, REAL *NRPYELL_RESTRICT xx[3],
+                              REAL *NRPYELL_RESTRICT in_gfs);
+/* :27 / auxevol_gfs_all_points.c:6: psi_bg and AA^2 at all points (device) */
+void auxevol_gfs_all_points(const paramstruct *NRPYELL_RESTRICT params, REAL *NRPYELL_RESTRICT xx[3],
+                            REAL *NRPYELL_RESTRICT auxevol_gfs);
+/* :28 / wavespeed_gf_all_points.c:6 (device) */
+void wavespeed_gf_all_points(const paramstruct *NRPYELL_RESTRICT params, const REAL CFL_FACTOR, const REAL dt,
+                             REAL *NRPYELL_RESTRICT xx[3], REAL *NRPYELL_RESTRICT auxevol_gfs);
+/* :34 / compute_wavespeed_at_OB.c:7 */
+REAL compute_wavespeed_at_OB(const paramstruct *NRPYELL_RESTRICT params, REAL *NRPYELL_RESTRICT auxevol_gfs);
+/* :26 / rhs_eval.c:7. HIP kernel; interior points of rhs_gfs only. */
+void rhs_eval(const paramstruct *NRPYELL_RESTRICT params, const rfm_struct *NRPYELL_RESTRICT rfmstruct,
+              const REAL *NRPYELL_RESTRICT auxevol_gfs, const REAL *NRPYELL_RESTRICT in_gfs,
+              REAL *NRPYELL_RESTRICT rhs_gfs);
+/* :7-10 / apply_bcs_outerradiation_and_inner.c:410. HIP kernels. */
+void apply_bcs_outerradiation_and_inner(const paramstruct *NRPYELL_RESTRICT params,
+                                        const bc_struct *NRPYELL_RESTRICT bcstruct, REAL *NRPYELL_RESTRICT xx[3],
+                                        const REAL custom_wavespeed[NUM_EVOL_GFS],
+                                        const REAL custom_f_infinity[NUM_EVOL_GFS], REAL *NRPYELL_RESTRICT gfs,
+                                        REAL *NRPYELL_RESTRICT rhs_gfs);
+/* :8 / apply_bcs_inner_only.c:11. HIP gather kernel. */
+void apply_bcs_inner_only(const paramstruct *NRPYELL_RESTRICT params, const bc_struct *NRPYELL_RESTRICT bcstruct,
+                          REAL *NRPYELL_RESTRICT gfs);
+/* :5 / MoL_step_forward_in_time.c:7. One RK4 step as 4 fused HIP kernels.
+ * Precondition (always true for data produced by this path or the reference's zero
+ * initial guess): y_n ghost values equal their parity images. After the call y_n is
+ * exact at every point; k_odd/k_even/y_nplus1_running_total are scratch. */
+void MoL_step_forward_in_time(griddata_struct *NRPYELL_RESTRICT griddata, const REAL dt);
+/* :29 / residual_all_points.c:7. HIP kernel; interior points of residual_gf only. */
+void residual_all_points(const paramstruct *NRPYELL_RESTRICT params, const rfm_struct *NRPYELL_RESTRICT rfmstruct,
+                         const REAL *NRPYELL_RESTRICT auxevol_gfs, const REAL *NRPYELL_RESTRICT in_gfs,
+                         REAL *NRPYELL_RESTRICT residual_gf);
+/* :30 / L2_norm_of_gf.c:6. Deterministic two-pass HIP reduction; returns log10(1e-16+L2). */
+REAL L2_norm_of_gf(const paramstruct *NRPYELL_RESTRICT params, const int gf_index, const REAL integration_radius,
+                   REAL *NRPYELL_RESTRICT xx[3], const REAL *NRPYELL_RESTRICT in_gf);
+/* :31-32 / gridfunction_z_axis.c:6, gridfunction_xz_plane.c:6: D2H copy, same text format. */
+void gridfunction_z_axis(const paramstruct *NRPYELL_RESTRICT params, const int gf_index, REAL *NRPYELL_RESTRICT xx[3],
+                         const REAL *NRPYELL_RESTRICT in_gf, FILE *NRPYELL_RESTRICT outfile);
+void gridfunction_xz_plane(const paramstruct *NRPYELL_RESTRICT params, const int gf_index, REAL *NRPYELL_RESTRICT xx[3],
+                           const REAL *NRPYELL_RESTRICT in_gf, FILE *NRPYELL_RESTRICT outfile);
+
+/* ---- Part 2: this library's API (no reference counterpart) ----------------------------- */
+
+#define NRPYELL_OK 0
+#define NRPYELL_EINVAL -1
+#define NRPYELL_EHIP -2
+#define NRPYELL_EGHOSTMAP -3
+#define NRPYELL_ENOTCONVERGED -4
+#define NRPYELL_ENAN -5
+
+/* Library/device plumbing used by the Python host layer and tests. */
+int nrpyell_device_synchronize(void);
+int nrpyell_malloc_device(void **ptr, size_t bytes);
+int nrpyell_free_device(void *ptr);
+int nrpyell_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int nrpyell_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int nrpyell_memset_device(void *ptr, int value, size_t bytes);
+const char *nrpyell_last_error(void);
+/* Same tables as bcstruct_set_up but returns NRPYELL_EGHOSTMAP instead of exiting. */
+int nrpyell_bcstruct_set_up_checked(const paramstruct *params, REAL *xx[3], bc_struct *bcstruct);
+void nrpyell_bcstruct_free(bc_struct *bcstruct);
+/* Checks that the closed-form SinhSymTP reflection map the fused kernels use to write ghost
+ * zones reproduces this bcstruct table exactly (host only). NRPYELL_OK or NRPYELL_EGHOSTMAP. */
+int nrpyell_ghostmap_check(const paramstruct *params, const bc_struct *bcstruct);
+
+/* Resident single-configuration solver. */
+typedef struct nrpyell_solver nrpyell_solver;
+
+typedef struct {
+  int precision;          /* 64: FP64 throughout; 32: FP32 bulk then FP64 finish          */
+  int check_every;        /* residual + L2 every this many RK4 steps (>=1)                  */
+  int steps_per_graph;    /* RK4 steps captured in one hipGraph launch (>=1)                */
+  double log10_target;    /* stop when log10 L2 <= this (e.g. -10)                           */
+  int max_steps;          /* give up after this many steps (NRPYELL_ENOTCONVERGED)          */
+  double fp32_switch_log10;   /* precision 32: switch to FP64 at/below this log10 L2         */
+  double fp32_plateau_ratio;  /* precision 32: or when L2 shrinks by less than this factor  */
+  double integration_radius;  /* L2 mask radius, 100 in the reference driver (main.c:150) */
+  int tile_i0, tile_i1;   /* thread-block tile of the fused substep kernel (0 = default)    */
+} nrpyell_solve_options;
+
+typedef struct {
+  int steps;              /* RK4 steps taken                                               */
+  int fp32_steps;         /* of which in FP32                                              */
+  double log10_l2;        /* last measured log10 L2 of the residual                         */
+  double log10_l2_initial;/* log10 L2 of the residual of the initial guess                  */
+  double seconds;         /* wall time of the solve (device-synchronised)                   */
+  int converged;
+} nrpyell_solve_result;
+
+void nrpyell_default_options(nrpyell_solve_options *opt);
+/* Build device tables for griddata (params, xx, bcstruct, rfmstruct, gridfuncs must be set
+ * up as in the reference driver). The solver does not own griddata's arrays. */
+int nrpyell_solver_create(griddata_struct *griddata, REAL dt, const nrpyell_solve_options *opt,
+                          nrpyell_solver **out);
+int nrpyell_solver_destroy(nrpyell_solver *s);
+/* Relax y_n in place until log10 L2 <= target (or max_steps). */
+int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res);
+/* Exactly n RK4 steps (graph replay when opt->steps_per_graph > 1), no residual checks. */
+int nrpyell_solver_steps(nrpyell_solver *s, int n);
+/* Fused residual + L2 of the current y_n; returns log10 L2 in *log10_l2. */
+int nrpyell_solver_residual_l2(nrpyell_solver *s, double *log10_l2);
+/* Time n launches of the fused substep kernel for stage `stage` (1..4) with HIP events
+ * on the solver's stream; returns the mean duration per launch in microseconds. */
+int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

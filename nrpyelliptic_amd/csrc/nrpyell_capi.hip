@@ -1,0 +1,816 @@
+// nrpyell_capi.hip -- the C ABI of libnrpyell_hip.so: the reference driver's function surface
+// (NRPy_function_prototypes.h) over HIP, and the resident solver (nrpyell_solver_*).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "nrpyell_internal.h"
+
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_mu;
+}  // namespace
+
+void set_last_error(const std::string &s) { g_last_error = s; }
+
+[[noreturn]] void fatal(const char *what, const std::string &msg) {
+  fprintf(stderr, "libnrpyell_hip: %s failed: %s\n", what, msg.c_str());
+  exit(1);
+}
+
+extern "C" const char *nrpyell_last_error(void) { return g_last_error.c_str(); }
+
+static hipStream_t default_stream() { return 0; }
+
+// ---------------------------------------------------------------------------------------
+// plumbing
+// ---------------------------------------------------------------------------------------
+extern "C" int nrpyell_device_synchronize(void) {
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
+extern "C" int nrpyell_malloc_device(void **ptr, size_t bytes) {
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
+extern "C" int nrpyell_free_device(void *ptr) {
+  hipError_t e = hipFree(ptr);
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
+extern "C" int nrpyell_memcpy_h2d(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
+extern "C" int nrpyell_memcpy_d2h(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
+extern "C" int nrpyell_memset_device(void *ptr, int value, size_t bytes) {
+  hipError_t e = hipMemset(ptr, value, bytes);
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
+
+template <typename T>
+static T *dalloc(size_t n) {
+  void *p = nullptr;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  return (T *)p;
+}
+template <typename T>
+static T *upload(const T *h, size_t n) {
+  T *d = dalloc<T>(n);
+  HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+// ---------------------------------------------------------------------------------------
+// Reference surface: memory and setup
+// ---------------------------------------------------------------------------------------
+extern "C" void MoL_malloc_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
+  const long nt = dims_from_params(params).ntot;
+  gf->y_n_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+  gf->diagnostic_output_gfs = gf->y_nplus1_running_total_gfs;  // MoL_malloc_y_n_gfs.c:14
+}
+extern "C" void MoL_free_memory_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
+  (void)params;
+  HIPCHK(hipFree(gf->y_n_gfs));
+  gf->y_n_gfs = nullptr;
+}
+extern "C" void MoL_malloc_non_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
+  const long nt = dims_from_params(params).ntot;
+  gf->y_nplus1_running_total_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+  gf->k_odd_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+  gf->k_even_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+  gf->auxevol_gfs = dalloc<double>(NUM_AUXEVOL_GFS * nt);
+  // every array starts defined (the reference's malloc'd buffers are fully written before use)
+  HIPCHK(hipMemset(gf->y_nplus1_running_total_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+  HIPCHK(hipMemset(gf->k_odd_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+  HIPCHK(hipMemset(gf->k_even_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+  HIPCHK(hipMemset(gf->auxevol_gfs, 0, NUM_AUXEVOL_GFS * nt * sizeof(double)));
+  gf->diagnostic_output_gfs = gf->y_nplus1_running_total_gfs;  // MoL_malloc_non_y_n_gfs.c:17
+}
+extern "C" void MoL_free_memory_non_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
+  (void)params;
+  HIPCHK(hipFree(gf->y_nplus1_running_total_gfs));
+  HIPCHK(hipFree(gf->k_odd_gfs));
+  HIPCHK(hipFree(gf->k_even_gfs));
+  HIPCHK(hipFree(gf->auxevol_gfs));
+  gf->y_nplus1_running_total_gfs = gf->k_odd_gfs = gf->k_even_gfs = gf->auxevol_gfs = nullptr;
+  gf->diagnostic_output_gfs = nullptr;
+}
+
+extern "C" void rfm_precompute_rfmstruct_malloc(const paramstruct *params, rfm_struct *r) {
+  const long G0 = params->Nxx_plus_2NGHOSTS0, G1 = params->Nxx_plus_2NGHOSTS1;
+  r->f0_of_xx0 = dalloc<double>(G0); r->f0_of_xx0__D0 = dalloc<double>(G0);
+  r->f0_of_xx0__DD00 = dalloc<double>(G0); r->f0_of_xx0__DDD000 = dalloc<double>(G0);
+  r->f1_of_xx1 = dalloc<double>(G1); r->f1_of_xx1__D1 = dalloc<double>(G1); r->f1_of_xx1__DD11 = dalloc<double>(G1);
+  r->f2_of_xx0_xx1 = dalloc<double>(G0 * G1); r->f2_of_xx0_xx1__D0 = dalloc<double>(G0 * G1);
+  r->f2_of_xx0_xx1__D1 = dalloc<double>(G0 * G1); r->f2_of_xx0_xx1__DD00 = dalloc<double>(G0 * G1);
+  r->f2_of_xx0_xx1__DD11 = dalloc<double>(G0 * G1);
+  r->f3_of_xx0 = dalloc<double>(G0); r->f3_of_xx0__D0 = dalloc<double>(G0); r->f3_of_xx0__DD00 = dalloc<double>(G0);
+}
+extern "C" void rfm_precompute_rfmstruct_define(const paramstruct *params, REAL *xx[3], rfm_struct *r) {
+  const GridDims g = dims_from_params(params);
+  double *x0 = upload(xx[0], g.G0), *x1 = upload(xx[1], g.G1);
+  launch_rfm(g, params->AMAX, params->SINHWAA, params->bScale, x0, x1, *r, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipFree(x0));
+  HIPCHK(hipFree(x1));
+}
+extern "C" void rfm_precompute_rfmstruct_freemem(const paramstruct *params, rfm_struct *r) {
+  (void)params;
+  double **all[] = {&r->f0_of_xx0, &r->f0_of_xx0__D0, &r->f0_of_xx0__DD00, &r->f0_of_xx0__DDD000, &r->f1_of_xx1,
+                    &r->f1_of_xx1__D1, &r->f1_of_xx1__DD11, &r->f2_of_xx0_xx1, &r->f2_of_xx0_xx1__D0,
+                    &r->f2_of_xx0_xx1__D1, &r->f2_of_xx0_xx1__DD00, &r->f2_of_xx0_xx1__DD11, &r->f3_of_xx0,
+                    &r->f3_of_xx0__D0, &r->f3_of_xx0__DD00};
+  for (double **p : all) {
+    HIPCHK(hipFree(*p));
+    *p = nullptr;
+  }
+}
+
+extern "C" void initial_guess_all_points(const paramstruct *params, REAL *xx[3], REAL *in_gfs) {
+  // initial_guess_single_point.c:14-15: u = v = 0 at every point, ghosts included
+  (void)xx;
+  const long nt = dims_from_params(params).ntot;
+  HIPCHK(hipMemset(in_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+  HIPCHK(hipDeviceSynchronize());
+}
+
+static Punctures punctures_of(const paramstruct *p) {
+  Punctures pu;
+  pu.m[0] = p->bare_mass_0; pu.m[1] = p->bare_mass_1;
+  const double pos[2][3] = {{p->puncture_0_x, p->puncture_0_y, p->puncture_0_z},
+                            {p->puncture_1_x, p->puncture_1_y, p->puncture_1_z}};
+  const double P[2][3] = {{p->P0_x, p->P0_y, p->P0_z}, {p->P1_x, p->P1_y, p->P1_z}};
+  const double S[2][3] = {{p->S0_x, p->S0_y, p->S0_z}, {p->S1_x, p->S1_y, p->S1_z}};
+  memcpy(pu.pos, pos, sizeof(pos));
+  memcpy(pu.P, P, sizeof(P));
+  memcpy(pu.S, S, sizeof(S));
+  return pu;
+}
+
+extern "C" void auxevol_gfs_all_points(const paramstruct *params, REAL *xx[3], REAL *auxevol_gfs) {
+  const GridDims g = dims_from_params(params);
+  double *x0 = upload(xx[0], g.G0), *x1 = upload(xx[1], g.G1), *x2 = upload(xx[2], g.G2);
+  launch_auxevol(g, params->AMAX, params->SINHWAA, params->bScale, punctures_of(params), x0, x1, x2, auxevol_gfs,
+                 default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipFree(x0)); HIPCHK(hipFree(x1)); HIPCHK(hipFree(x2));
+}
+
+extern "C" void wavespeed_gf_all_points(const paramstruct *params, const REAL CFL_FACTOR, const REAL dt, REAL *xx[3],
+                                        REAL *auxevol_gfs) {
+  const GridDims g = dims_from_params(params);
+  std::vector<double> ds((size_t)g.G0 * g.G1, 0.0);
+  host_min_ds_2d(params, xx, ds.data());
+  // c = CFL*dsmin/dt (wavespeed_gf_all_points.c:40-41), rounded on the host per (i0,i1)
+  for (double &v : ds) v = CFL_FACTOR * v / dt;
+  double *d = upload(ds.data(), ds.size());
+  launch_wavespeed(g, d, 1.0, dt, auxevol_gfs, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  HIPCHK(hipFree(d));
+}
+
+extern "C" REAL compute_wavespeed_at_OB(const paramstruct *params, REAL *auxevol_gfs) {
+  // compute_wavespeed_at_OB.c:10
+  const GridDims g = dims_from_params(params);
+  double c = 0.0;
+  const long p = idx3(g, g.G0 - NRPYELL_NG - 1, NRPYELL_NG, g.G2 / 2);
+  HIPCHK(hipMemcpy(&c, auxevol_gfs + WAVESPEEDGF * g.ntot + p, sizeof(double), hipMemcpyDeviceToHost));
+  return c;
+}
+
+extern "C" int nrpyell_bcstruct_set_up_checked(const paramstruct *params, REAL *xx[3], bc_struct *bcstruct) {
+  std::string err;
+  const int rc = host_bcstruct(params, xx, bcstruct, &err);
+  if (rc != NRPYELL_OK) set_last_error(err);
+  return rc;
+}
+
+extern "C" void bcstruct_set_up(const paramstruct *params, REAL *xx[3], bc_struct *bcstruct) {
+  std::string err;
+  if (host_bcstruct(params, xx, bcstruct, &err) != NRPYELL_OK) {
+    fprintf(stderr, "%s\n", err.c_str());  // bcstruct_set_up.c:178-184
+    exit(1);
+  }
+}
+
+extern "C" int nrpyell_ghostmap_check(const paramstruct *params, const bc_struct *bcstruct) {
+  std::string err;
+  const int rc = ghostmap_verify(dims_from_params(params), bcstruct, &err);
+  if (rc != NRPYELL_OK) set_last_error(err);
+  return rc;
+}
+
+extern "C" void nrpyell_bcstruct_free(bc_struct *bc) {
+  free(bc->inner_bc_array);
+  bc->inner_bc_array = nullptr;
+  for (int s = 0; s < NGHOSTS * 3; s++) {
+    free(bc->pure_outer_bc_array[s]);
+    bc->pure_outer_bc_array[s] = nullptr;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Device copies of the host bcstruct tables, cached per table (validated on every use).
+// ---------------------------------------------------------------------------------------
+struct DeviceBc {
+  const innerpt_bc_struct *key;
+  int num_inner;
+  long ntot;
+  std::vector<innerpt_bc_struct> probe;  // sampled host records used to validate the cache
+  int *dst = nullptr, *src = nullptr;
+  int8_t *par0 = nullptr;
+  // radiation lists per slot (dirn + 3*gz), built against one xx grid
+  const double *xx_key0 = nullptr, *xx_key1 = nullptr;
+  RadPoint *rad[NGHOSTS * 3] = {};
+  int nrad[NGHOSTS * 3] = {};
+  ~DeviceBc() {
+    hipFree(dst); hipFree(src); hipFree(par0);
+    for (auto *p : rad) hipFree(p);
+  }
+};
+
+static std::map<const void *, std::unique_ptr<DeviceBc>> g_bc_cache;
+
+static bool bc_matches(const DeviceBc &d, const bc_struct *bc, long ntot) {
+  const int n = bc->bc_info.num_inner_boundary_points;
+  if (d.key != bc->inner_bc_array || d.num_inner != n || d.ntot != ntot) return false;
+  for (size_t k = 0; k < d.probe.size(); k++) {
+    const long j = (long)k * (n - 1) / (long)(d.probe.size() > 1 ? d.probe.size() - 1 : 1);
+    if (memcmp(&d.probe[k], &bc->inner_bc_array[j], sizeof(innerpt_bc_struct)) != 0) return false;
+  }
+  return true;
+}
+
+static DeviceBc &device_bc(const paramstruct *params, const bc_struct *bc) {
+  const GridDims g = dims_from_params(params);
+  auto it = g_bc_cache.find(bc->inner_bc_array);
+  if (it != g_bc_cache.end() && bc_matches(*it->second, bc, g.ntot)) return *it->second;
+  auto d = std::make_unique<DeviceBc>();
+  const int n = bc->bc_info.num_inner_boundary_points;
+  d->key = bc->inner_bc_array;
+  d->num_inner = n;
+  d->ntot = g.ntot;
+  for (int k = 0; k < 64 && n > 0; k++) d->probe.push_back(bc->inner_bc_array[(long)k * (n - 1) / 63]);
+  std::vector<int> hd(n), hs(n);
+  std::vector<int8_t> hp(n);
+  for (int k = 0; k < n; k++) {
+    hd[k] = bc->inner_bc_array[k].dstpt;
+    hs[k] = bc->inner_bc_array[k].srcpt;
+    hp[k] = bc->inner_bc_array[k].parity[0];  // evol_gf_parity = {0,0} (NRPy_basic_defines.h:199)
+  }
+  if (n > 0) {
+    d->dst = upload(hd.data(), n);
+    d->src = upload(hs.data(), n);
+    d->par0 = upload(hp.data(), n);
+  }
+  DeviceBc &ref = *d;
+  g_bc_cache[bc->inner_bc_array] = std::move(d);
+  return ref;
+}
+
+// Clamped stencil offset of compute_partial_r_f (apply_bcs_outerradiation_and_inner.c:348-360).
+static int stencil_offset(int idx, int face, int G) {
+  int off = face;
+  if (idx < 5) off = 5 - idx;
+  else if (idx > G - 5 - 1) off = (G - 5 - 1) - idx;
+  return off;
+}
+
+static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const bc_struct *bc, REAL *xx[3]) {
+  if (d.xx_key0 == xx[0] && d.xx_key1 == xx[1]) return;
+  const GridDims g = dims_from_params(params);
+  for (int s = 0; s < NGHOSTS * 3; s++) {
+    hipFree(d.rad[s]);
+    d.rad[s] = nullptr;
+    const int gz = s / 3, dirn = s % 3;
+    const int n = bc->bc_info.num_pure_outer_boundary_points[gz][dirn];
+    d.nrad[s] = n;
+    if (n == 0) continue;
+    std::vector<RadPoint> pts(n);
+    for (int k = 0; k < n; k++) {
+      const outerpt_bc_struct &o = bc->pure_outer_bc_array[s][k];
+      const int f[3] = {o.FACEX0, o.FACEX1, o.FACEX2};
+      const int P[2][3] = {{o.i0, o.i1, o.i2}, {o.i0 + f[0], o.i1 + f[1], o.i2 + f[2]}};
+      RadPoint &r = pts[k];
+      for (int w = 0; w < 2; w++) {
+        r.off[w][0] = stencil_offset(P[w][0], f[0], g.G0);
+        r.off[w][1] = stencil_offset(P[w][1], f[1], g.G1);
+        r.off[w][2] = 0;
+        host_radial_geometry(params, xx[0][P[w][0]], xx[1][P[w][1]], r.geo[w]);
+      }
+      r.p = idx3(g, P[0][0], P[0][1], P[0][2]);
+      r.q = idx3(g, P[1][0], P[1][1], P[1][2]);
+    }
+    d.rad[s] = upload(pts.data(), n);
+  }
+  d.xx_key0 = xx[0];
+  d.xx_key1 = xx[1];
+}
+
+// ---------------------------------------------------------------------------------------
+// Reference surface: operators
+// ---------------------------------------------------------------------------------------
+extern "C" void rhs_eval(const paramstruct *params, const rfm_struct *rfmstruct, const REAL *auxevol_gfs,
+                         const REAL *in_gfs, REAL *rhs_gfs) {
+  const GridDims g = dims_from_params(params);
+  launch_rhs_exact(g, rfmstruct, auxevol_gfs, in_gfs, rhs_gfs, params->eta_damping, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+}
+
+extern "C" void residual_all_points(const paramstruct *params, const rfm_struct *rfmstruct, const REAL *auxevol_gfs,
+                                    const REAL *in_gfs, REAL *residual_gf) {
+  const GridDims g = dims_from_params(params);
+  launch_residual_exact(g, rfmstruct, auxevol_gfs, in_gfs, residual_gf, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+}
+
+extern "C" void apply_bcs_inner_only(const paramstruct *params, const bc_struct *bcstruct, REAL *gfs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceBc &d = device_bc(params, bcstruct);
+  launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, gfs, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+}
+
+extern "C" void apply_bcs_outerradiation_and_inner(const paramstruct *params, const bc_struct *bcstruct, REAL *xx[3],
+                                                   const REAL custom_wavespeed[NUM_EVOL_GFS],
+                                                   const REAL custom_f_infinity[NUM_EVOL_GFS], REAL *gfs,
+                                                   REAL *rhs_gfs) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const GridDims g = dims_from_params(params);
+  DeviceBc &d = device_bc(params, bcstruct);
+  ensure_radiation_lists(d, params, bcstruct, xx);
+  // innermost layer first; within a layer x0, x1, x2 faces (apply_bcs_outerradiation_and_inner.c:431-458)
+  for (int s = 0; s < NGHOSTS * 3; s++)
+    launch_radiation_list(g, d.nrad[s], d.rad[s], gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
+                          custom_f_infinity[0], custom_f_infinity[1], default_stream());
+  launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+}
+
+// L2 weight tables cached per (grid, radius, coordinates).
+struct L2Weights {
+  std::vector<double> key;
+  double *w2d = nullptr;
+  double *partial = nullptr;
+  double *result = nullptr;
+  double volume = 0.0;
+  ~L2Weights() { hipFree(w2d); hipFree(partial); hipFree(result); }
+};
+static std::vector<std::unique_ptr<L2Weights>> g_l2_cache;
+
+static std::vector<double> l2_key(const paramstruct *p, REAL *const xx[3], double radius) {
+  std::vector<double> k = {radius, p->AMAX, p->SINHWAA, p->bScale, (double)p->Nxx0, (double)p->Nxx1, (double)p->Nxx2,
+                           p->dxx0, p->dxx1, p->dxx2};
+  k.insert(k.end(), xx[0], xx[0] + p->Nxx_plus_2NGHOSTS0);
+  k.insert(k.end(), xx[1], xx[1] + p->Nxx_plus_2NGHOSTS1);
+  return k;
+}
+
+static L2Weights &l2_weights(const paramstruct *p, REAL *const xx[3], double radius) {
+  std::vector<double> key = l2_key(p, xx, radius);
+  for (auto &w : g_l2_cache)
+    if (w->key.size() == key.size() && memcmp(w->key.data(), key.data(), key.size() * sizeof(double)) == 0) return *w;
+  auto w = std::make_unique<L2Weights>();
+  w->key = key;
+  const GridDims g = dims_from_params(p);
+  std::vector<double> h((size_t)g.G0 * g.G1);
+  host_l2_weights(p, xx, radius, h.data(), &w->volume);
+  w->w2d = upload(h.data(), h.size());
+  const long nblk = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
+  w->partial = dalloc<double>(nblk);
+  w->result = dalloc<double>(1);
+  if (g_l2_cache.size() > 32) g_l2_cache.erase(g_l2_cache.begin());
+  g_l2_cache.push_back(std::move(w));
+  return *g_l2_cache.back();
+}
+
+extern "C" REAL L2_norm_of_gf(const paramstruct *params, const int gf_index, const REAL integration_radius,
+                              REAL *xx[3], const REAL *in_gf) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const GridDims g = dims_from_params(params);
+  L2Weights &w = l2_weights(params, xx, integration_radius);
+  const long npart = launch_l2_partial(g, in_gf + (long)gf_index * g.ntot, w.w2d, w.partial, default_stream());
+  launch_l2_finalize(w.partial, npart, w.volume, nullptr, 0, 0.0, w.result, default_stream());
+  HIPCHK(hipGetLastError());
+  double out = 0.0;
+  HIPCHK(hipMemcpy(&out, w.result, sizeof(double), hipMemcpyDeviceToHost));
+  return out;
+}
+
+// gridfunction_z_axis.c:6-52: z and gf along the +z axis (i1 = NG), -z axis (i1 = G1-NG-1)
+// and the i0 = NG line, at i2 = G2/2, "%.17e %.17e" per line.
+extern "C" void gridfunction_z_axis(const paramstruct *params, const int gf_index, REAL *xx[3], const REAL *in_gf,
+                                    FILE *outfile) {
+  const GridDims g = dims_from_params(params);
+  std::vector<double> h(g.ntot);
+  HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
+  const double w = params->SINHWAA, A = params->AMAX, b = params->bScale;
+  const double D = exp(1.0 / w) - exp(-1.0 / w);
+  auto zc = [&](double x0, double x1) {
+    const double s = exp(x0 / w) - exp(-x0 / w);
+    return sqrt(A * A * (s * s) / (D * D) + b * b) * cos(x1);
+  };
+  const int i2 = g.G2 / 2;
+  for (int pass = 0; pass < 2; pass++) {
+    const int i1 = pass == 0 ? NRPYELL_NG : g.G1 - NRPYELL_NG - 1;
+    for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++)
+      fprintf(outfile, "%.17e %.17e\n", zc(xx[0][i0], xx[1][i1]), h[idx3(g, i0, i1, i2)]);
+  }
+  for (int i1 = NRPYELL_NG + 1; i1 < g.G1 - NRPYELL_NG - 1; i1++)
+    fprintf(outfile, "%.17e %.17e\n", zc(xx[0][NRPYELL_NG], xx[1][i1]), h[idx3(g, NRPYELL_NG, i1, i2)]);
+}
+
+// gridfunction_xz_plane.c:6-49: x, z, gf on the xz plane (phi = 0 and phi = pi half planes).
+extern "C" void gridfunction_xz_plane(const paramstruct *params, const int gf_index, REAL *xx[3], const REAL *in_gf,
+                                      FILE *outfile) {
+  const GridDims g = dims_from_params(params);
+  std::vector<double> h(g.ntot);
+  HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
+  const double w = params->SINHWAA, A = params->AMAX, b = params->bScale;
+  const double D = exp(1.0 / w) - exp(-1.0 / w);
+  const int i2s[2] = {NRPYELL_NG + g.N2 / 4, NRPYELL_NG + (3 * g.N2) / 4};
+  for (int k = 0; k < 2; k++) {
+    const int i2 = i2s[k];
+    for (int i1 = NRPYELL_NG; i1 < g.G1 - NRPYELL_NG; i1++)
+      for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++) {
+        const double s = exp(xx[0][i0] / w) - exp(-xx[0][i0] / w);
+        const double f0 = A * s / D;
+        const double x = f0 * sin(xx[1][i1]) * cos(xx[2][i2]);
+        const double z = sqrt(f0 * f0 + b * b) * cos(xx[1][i1]);
+        fprintf(outfile, "%.17e %.17e %.17e\n", x, z, h[idx3(g, i0, i1, i2)]);
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Resident solver
+// ---------------------------------------------------------------------------------------
+struct nrpyell_solver {
+  griddata_struct *gd = nullptr;
+  GridDims g{};
+  double dt = 0.0;
+  nrpyell_solve_options opt{};
+  hipStream_t stream = nullptr;
+  int tx = 32, ntile0 = 0, ntile1 = 0;
+  double *coef64 = nullptr;
+  float *coef32 = nullptr;
+  double *geo64 = nullptr;
+  float *geo32 = nullptr;
+  double *w2d = nullptr;
+  double volume = 0.0;
+  double *partial = nullptr;
+  double *l2_out = nullptr;
+  SolverDeviceState *dstate = nullptr;
+  SolverDeviceState *hstate = nullptr;  // pinned
+  // FP32 shadow state
+  float *y32 = nullptr, *ko32 = nullptr, *ke32 = nullptr, *rt32 = nullptr, *aux32 = nullptr;
+  hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+  int graph64_steps = 0, graph32_steps = 0;
+  double graph32_target = 0.0;
+  double *cached_aux = nullptr;
+  ~nrpyell_solver() {
+    if (graph64) hipGraphExecDestroy(graph64);
+    if (graph32) hipGraphExecDestroy(graph32);
+    hipFree(coef64); hipFree(coef32); hipFree(geo64); hipFree(geo32); hipFree(w2d); hipFree(partial);
+    hipFree(l2_out); hipFree(dstate);
+    if (hstate) hipHostFree(hstate);
+    hipFree(y32); hipFree(ko32); hipFree(ke32); hipFree(rt32); hipFree(aux32);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  template <typename T>
+  SubstepArgs<T> args(int stage, T *yn, T *ko, T *ke, T *rt, const T *aux, const T *coef, const T *geo) const {
+    SubstepArgs<T> a;
+    a.g = g;
+    const MoL_gridfunctions_struct &gf = gd->gridfuncs;
+    (void)gf;
+    switch (stage) {
+      case 1: a.in = yn; a.out = ko; break;
+      case 2: a.in = ko; a.out = ke; break;
+      case 3: a.in = ke; a.out = ko; break;
+      default: a.in = ko; a.out = yn; break;
+    }
+    a.yn = yn;
+    a.rt = rt;
+    a.aux = aux;
+    a.coef = coef;
+    a.geo = geo;
+    a.cOB_u = (T)gd->params.uu_wavespeed_at_OB;
+    a.cOB_v = (T)gd->params.vv_wavespeed_at_OB;
+    a.eta = (T)gd->params.eta_damping;
+    a.dt = (T)dt;
+    a.done = &dstate->done;
+    a.tx = tx;
+    a.ntile0 = ntile0;
+    a.ntile1 = ntile1;
+    return a;
+  }
+
+  void step64(hipStream_t st) {
+    MoL_gridfunctions_struct &f = gd->gridfuncs;
+    for (int s = 1; s <= 4; s++)
+      launch_substep<double>(args<double>(s, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs,
+                                          f.auxevol_gfs, coef64, geo64),
+                             s, st);
+  }
+  void step32(hipStream_t st) {
+    for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, aux32, coef32, geo32), s, st);
+  }
+  // residual of the current y_n (FP64 or the FP32 shadow) and a convergence-state update
+  void check(hipStream_t st, bool from32, int steps_per_check, double target) {
+    const void *u = from32 ? (const void *)y32 : (const void *)gd->gridfuncs.y_n_gfs;
+    const long npart = launch_residual_l2(g, u, from32, gd->gridfuncs.auxevol_gfs, coef64, w2d, partial,
+                                          &dstate->done, nullptr, st);
+    launch_l2_finalize(partial, npart, volume, dstate, steps_per_check, target, l2_out, st);
+  }
+};
+
+static void reset_state(nrpyell_solver *s) {
+  memset(s->hstate, 0, sizeof(SolverDeviceState));
+  HIPCHK(hipMemcpyAsync(s->dstate, s->hstate, sizeof(SolverDeviceState), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+static void read_state(nrpyell_solver *s) {
+  HIPCHK(hipMemcpyAsync(s->hstate, s->dstate, sizeof(SolverDeviceState), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
+  o->precision = 64;
+  o->check_every = 10;
+  o->steps_per_graph = 50;
+  o->log10_target = -10.0;
+  o->max_steps = 200000;
+  o->fp32_switch_log10 = -6.0;
+  o->fp32_plateau_ratio = 0.7;
+  o->integration_radius = 100.0;  // main.c:150
+  o->tile_i0 = 0;
+  o->tile_i1 = 0;
+}
+
+extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell_solve_options *opt,
+                                     nrpyell_solver **out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto s = std::make_unique<nrpyell_solver>();
+  s->gd = gd;
+  s->g = dims_from_params(&gd->params);
+  s->dt = dt;
+  if (opt) s->opt = *opt; else nrpyell_default_options(&s->opt);
+  if (s->opt.check_every < 1) s->opt.check_every = 1;
+  if (s->opt.steps_per_graph < s->opt.check_every) s->opt.steps_per_graph = s->opt.check_every;
+  s->opt.steps_per_graph -= s->opt.steps_per_graph % s->opt.check_every;
+  const GridDims &g = s->g;
+  if (g.N0 < 2 * NRPYELL_NG || g.N1 < 2 * NRPYELL_NG || g.N2 < 2 || (g.N2 & 1)) {
+    set_last_error("grid too small for the fused solver (need Nxx0,Nxx1 >= 10 and even Nxx2)");
+    return NRPYELL_EINVAL;
+  }
+  std::string err;
+  if (ghostmap_verify(g, &gd->bcstruct, &err) != NRPYELL_OK) {
+    set_last_error(err);
+    return NRPYELL_EGHOSTMAP;
+  }
+  int tx = s->opt.tile_i0 > 0 ? s->opt.tile_i0 : 32;
+  if (tx != 16 && tx != 32 && tx != 64 && tx != 128 && tx != 256) {
+    set_last_error("tile_i0 must be 16, 32, 64, 128 or 256");
+    return NRPYELL_EINVAL;
+  }
+  s->tx = tx;
+  const int ty = 256 / tx;
+  s->ntile0 = (g.N0 + tx - 1) / tx;
+  s->ntile1 = (g.N1 + ty - 1) / ty;
+  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  const long n2 = (long)g.G0 * g.G1;
+  s->coef64 = dalloc<double>(5 * n2);
+  s->coef32 = dalloc<float>(5 * n2);
+  launch_coef(g, &gd->rfmstruct, s->coef64, s->coef32, s->stream);
+  std::vector<double> geo((size_t)6 * g.G1 * 3);
+  host_outer_geometry(&gd->params, gd->xx, geo.data());
+  s->geo64 = upload(geo.data(), geo.size());
+  std::vector<float> geof(geo.begin(), geo.end());
+  s->geo32 = upload(geof.data(), geof.size());
+  std::vector<double> w(n2);
+  host_l2_weights(&gd->params, gd->xx, s->opt.integration_radius, w.data(), &s->volume);
+  s->w2d = upload(w.data(), w.size());
+  const long nblk = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
+  s->partial = dalloc<double>(nblk);
+  s->l2_out = dalloc<double>(1);
+  s->dstate = dalloc<SolverDeviceState>(1);
+  HIPCHK(hipHostMalloc((void **)&s->hstate, sizeof(SolverDeviceState), hipHostMallocDefault));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  HIPCHK(hipGetLastError());
+  reset_state(s.get());
+  *out = s.release();
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_solver_destroy(nrpyell_solver *s) {
+  if (!s) return NRPYELL_OK;
+  hipStreamSynchronize(s->stream);
+  delete s;
+  return NRPYELL_OK;
+}
+
+static void ensure_fp32(nrpyell_solver *s) {
+  if (s->y32) return;
+  const long nt = s->g.ntot;
+  s->y32 = dalloc<float>(2 * nt);
+  s->ko32 = dalloc<float>(2 * nt);
+  s->ke32 = dalloc<float>(2 * nt);
+  s->rt32 = dalloc<float>(2 * nt);
+  s->aux32 = dalloc<float>(3 * nt);
+}
+
+// Capture `nsteps` RK4 steps with a residual check every check_every steps as one graph.
+static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double target) {
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 1; k <= nsteps; k++) {
+    if (fp32) s->step32(s->stream); else s->step64(s->stream);
+    if (k % s->opt.check_every == 0) s->check(s->stream, fp32, s->opt.check_every, target);
+  }
+  HIPCHK(hipStreamEndCapture(s->stream, &graph));
+  hipGraphExec_t exec;
+  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  HIPCHK(hipGraphDestroy(graph));
+  return exec;
+}
+
+extern "C" int nrpyell_solver_steps(nrpyell_solver *s, int n) {
+  reset_state(s);
+  for (int k = 0; k < n; k++) s->step64(s->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_solver_residual_l2(nrpyell_solver *s, double *log10_l2) {
+  const long npart = launch_residual_l2(s->g, s->gd->gridfuncs.y_n_gfs, false, s->gd->gridfuncs.auxevol_gfs,
+                                        s->coef64, s->w2d, s->partial, nullptr, nullptr, s->stream);
+  launch_l2_finalize(s->partial, npart, s->volume, nullptr, 0, 0.0, s->l2_out, s->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(log10_l2, s->l2_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res) {
+  memset(res, 0, sizeof(*res));
+  const auto t0 = std::chrono::steady_clock::now();
+  const long nt = s->g.ntot;
+  MoL_gridfunctions_struct &f = s->gd->gridfuncs;
+  reset_state(s);
+  // residual of the initial guess (counts no steps)
+  s->check(s->stream, false, 0, s->opt.log10_target);
+  read_state(s);
+  res->log10_l2_initial = s->hstate->last_log10;
+  int steps = 0;
+  if (s->hstate->done == 0 && s->opt.precision == 32) {
+    ensure_fp32(s);
+    launch_convert_d2f(2 * nt, f.y_n_gfs, s->y32, s->stream);
+    launch_convert_d2f(3 * nt, f.auxevol_gfs, s->aux32, s->stream);
+    if (!s->graph32 || s->graph32_target != s->opt.fp32_switch_log10) {
+      if (s->graph32) hipGraphExecDestroy(s->graph32);
+      s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
+      s->graph32_target = s->opt.fp32_switch_log10;
+    }
+    reset_state(s);
+    double prev = res->log10_l2_initial;
+    const double plateau = log10(s->opt.fp32_plateau_ratio);
+    while (true) {
+      HIPCHK(hipGraphLaunch(s->graph32, s->stream));
+      read_state(s);
+      const SolverDeviceState &h = *s->hstate;
+      if (h.done || h.steps >= s->opt.max_steps) break;
+      // plateau: the last check improved by less than the ratio
+      if (h.nhist >= 2 && h.hist_log10[h.nhist - 1] - h.hist_log10[h.nhist - 2] > plateau) break;
+      if (h.last_log10 > prev + 3.0) break;  // FP32 diverging: hand over to FP64
+      prev = h.last_log10;
+    }
+    steps = s->hstate->steps;
+    res->fp32_steps = steps;
+    launch_convert_f2d(2 * nt, s->y32, f.y_n_gfs, s->stream);
+    reset_state(s);
+    s->check(s->stream, false, 0, s->opt.log10_target);
+    read_state(s);
+  }
+  if (s->hstate->done == 0) {
+    if (!s->graph64 || s->graph64_steps != s->opt.steps_per_graph) {
+      if (s->graph64) hipGraphExecDestroy(s->graph64);
+      s->graph64 = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
+      s->graph64_steps = s->opt.steps_per_graph;
+    }
+    reset_state(s);
+    while (true) {
+      HIPCHK(hipGraphLaunch(s->graph64, s->stream));
+      read_state(s);
+      if (s->hstate->done || steps + s->hstate->steps >= s->opt.max_steps) break;
+    }
+    steps += s->hstate->steps;
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  res->steps = steps;
+  res->log10_l2 = s->hstate->last_log10;
+  res->seconds = std::chrono::duration<double>(t1 - t0).count();
+  res->converged = s->hstate->done == 1;
+  if (s->hstate->done == 2) { set_last_error("residual became NaN"); return NRPYELL_ENAN; }
+  if (!res->converged) { set_last_error("max_steps reached before the residual target"); return NRPYELL_ENOTCONVERGED; }
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_per_launch) {
+  MoL_gridfunctions_struct &f = s->gd->gridfuncs;
+  reset_state(s);
+  // the kernels write y_n only in stage 4; time on scratch copies so the state is untouched
+  const SubstepArgs<double> a =
+      s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, f.auxevol_gfs,
+                      s->coef64, s->geo64);
+  SubstepArgs<double> b = a;
+  double *tmp = nullptr;
+  if (stage == 4) {
+    tmp = dalloc<double>(2 * s->g.ntot);
+    HIPCHK(hipMemcpyAsync(tmp, f.y_n_gfs, 2 * s->g.ntot * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+    b.out = tmp;
+    b.yn = tmp;
+  }
+  for (int k = 0; k < 3; k++) launch_substep<double>(b, stage, s->stream);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s->stream));
+  for (int k = 0; k < n; k++) launch_substep<double>(b, stage, s->stream);
+  HIPCHK(hipEventRecord(e1, s->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  *us_per_launch = 1000.0 * ms / n;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (tmp) hipFree(tmp);
+  return NRPYELL_OK;
+}
+
+// ---------------------------------------------------------------------------------------
+// MoL_step_forward_in_time: one RK4 step through a solver cached per griddata.
+// ---------------------------------------------------------------------------------------
+struct StepCacheEntry {
+  nrpyell_solver *s;
+  const double *yn, *aux;
+  const void *rfm0;
+  const innerpt_bc_struct *inner;
+};
+static std::map<const griddata_struct *, StepCacheEntry> g_step_cache;
+
+extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL dt) {
+  nrpyell_solver *s = nullptr;
+  {
+    auto it = g_step_cache.find(griddata);
+    if (it != g_step_cache.end() && it->second.yn == griddata->gridfuncs.y_n_gfs &&
+        it->second.aux == griddata->gridfuncs.auxevol_gfs && it->second.rfm0 == griddata->rfmstruct.f0_of_xx0 &&
+        it->second.inner == griddata->bcstruct.inner_bc_array) {
+      s = it->second.s;
+    } else {
+      if (it != g_step_cache.end()) {
+        nrpyell_solver_destroy(it->second.s);
+        g_step_cache.erase(it);
+      }
+      nrpyell_solve_options o;
+      nrpyell_default_options(&o);
+      if (nrpyell_solver_create(griddata, dt, &o, &s) != NRPYELL_OK) fatal("MoL_step_forward_in_time", g_last_error);
+      g_step_cache[griddata] = {s, griddata->gridfuncs.y_n_gfs, griddata->gridfuncs.auxevol_gfs,
+                                griddata->rfmstruct.f0_of_xx0, griddata->bcstruct.inner_bc_array};
+    }
+  }
+  s->dt = dt;
+  HIPCHK(hipMemsetAsync(&s->dstate->done, 0, sizeof(int), s->stream));
+  s->step64(s->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(s->stream));
+}

@@ -1,0 +1,97 @@
+// nrpyell_common.h -- shared host/device definitions for libnrpyell_hip.so.
+//
+// Grid layout is the reference's (NRPy_basic_defines.h:247-250): gridfunction-major SoA,
+// i0 fastest, ghost zones of width NGHOSTS=5 stored in place.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NRPYELL_NG 5
+
+struct GridDims {
+  int N0, N1, N2;  // interior points per direction
+  int G0, G1, G2;  // with ghosts
+  long ntot;       // G0*G1*G2 points per gridfunction
+  double invdx0, invdx1, invdx2;
+};
+
+__host__ __device__ inline long idx3(const GridDims &g, int i0, int i1, int i2) {
+  return (long)i0 + (long)g.G0 * ((long)i1 + (long)g.G1 * (long)i2);
+}
+
+// Periodic index in the phi (xx2) direction: interior i2 runs over [NG, N2+NG).
+__host__ __device__ inline int wrap2(const GridDims &g, int j) {
+  int m = (j - NRPYELL_NG) % g.N2;
+  if (m < 0) m += g.N2;
+  return m + NRPYELL_NG;
+}
+
+// Ghost -> source map of the SinhSymTP grid, the closed form of the eigen-coordinate round
+// trip of bcstruct_set_up.c:27-194:
+//   xx0 < 0 : (xx0,xx1,xx2) ~ (-xx0, xx1, xx2+pi)   i0 -> 2*NG-1-i0, half turn in phi
+//   xx1 < 0 : (xx0,xx1,xx2) ~ (xx0, -xx1, xx2+pi)   i1 -> 2*NG-1-i1, half turn
+//   xx1 > pi: (xx0,xx1,xx2) ~ (xx0, 2pi-xx1, xx2+pi) i1 -> 2*(N1+NG)-1-i1, half turn
+//   xx2 outside [-pi,pi): periodic.
+// Points with i0 >= N0+NG (outside the outer radius) keep i0. A point that maps to itself is
+// a pure outer point. The library checks this closed form against the bcstruct table of
+// every grid it runs on (ghostmap_verify) and refuses grids where they differ.
+__host__ __device__ inline void ghost_source(const GridDims &g, int i0, int i1, int i2, int &s0, int &s1,
+                                             int &s2) {
+  int flips = 0;
+  s0 = i0;
+  s1 = i1;
+  if (i0 < NRPYELL_NG) { s0 = 2 * NRPYELL_NG - 1 - i0; flips++; }
+  if (i1 < NRPYELL_NG) { s1 = 2 * NRPYELL_NG - 1 - i1; flips++; }
+  else if (i1 >= g.N1 + NRPYELL_NG) { s1 = 2 * (g.N1 + NRPYELL_NG) - 1 - i1; flips++; }
+  s2 = wrap2(g, i2 + ((flips & 1) ? g.N2 / 2 : 0));
+}
+
+__host__ __device__ inline bool is_interior(const GridDims &g, int i0, int i1, int i2) {
+  return i0 >= NRPYELL_NG && i0 < g.N0 + NRPYELL_NG && i1 >= NRPYELL_NG && i1 < g.N1 + NRPYELL_NG &&
+         i2 >= NRPYELL_NG && i2 < g.N2 + NRPYELL_NG;
+}
+
+// Enumerate every ghost point whose source (ghost_source) is (i0,i1,i2), an interior or pure
+// outer point, calling f(dst_index). This is the exact inverse of ghost_source: candidate i0
+// and i1 reflections, then every periodic phi copy shifted by the half turn their parity needs.
+template <class F>
+__host__ __device__ inline void for_each_image(const GridDims &g, int i0, int i1, int i2, F &&f) {
+  int a_c[2], a_f[2], na = 0;
+  a_c[na] = i0; a_f[na++] = 0;
+  if (i0 >= NRPYELL_NG && i0 < 2 * NRPYELL_NG) { a_c[na] = 2 * NRPYELL_NG - 1 - i0; a_f[na++] = 1; }
+  int b_c[3], b_f[3], nb = 0;
+  b_c[nb] = i1; b_f[nb++] = 0;
+  if (i1 >= NRPYELL_NG && i1 < 2 * NRPYELL_NG) { b_c[nb] = 2 * NRPYELL_NG - 1 - i1; b_f[nb++] = 1; }
+  if (i1 >= g.N1 && i1 < g.N1 + NRPYELL_NG && i1 >= NRPYELL_NG) {
+    b_c[nb] = 2 * (g.N1 + NRPYELL_NG) - 1 - i1; b_f[nb++] = 1;
+  }
+  for (int ia = 0; ia < na; ia++)
+    for (int ib = 0; ib < nb; ib++) {
+      const int a = a_c[ia], b = b_c[ib];
+      const int shift = ((a_f[ia] + b_f[ib]) & 1) ? g.N2 / 2 : 0;
+      // candidates j2 with wrap2(j2 + shift) == i2, j2 in [0, G2)
+      int j2 = i2 - shift;
+      while (j2 - g.N2 >= 0) j2 -= g.N2;
+      while (j2 < 0) j2 += g.N2;
+      for (; j2 < g.G2; j2 += g.N2) {
+        const bool interior_like = (a >= NRPYELL_NG) && (b >= NRPYELL_NG && b < g.N1 + NRPYELL_NG) &&
+                                   (j2 >= NRPYELL_NG && j2 < g.N2 + NRPYELL_NG);
+        // (a,b,j2) with interior b and j2 and a >= NG is the point itself (interior or pure outer)
+        if (interior_like) continue;
+        f(idx3(g, a, b, j2));
+      }
+    }
+}
+
+// 10th-order centred FD weights (rhs_eval.c:23-27; Fornberg).
+#define FD1_W1 (5.0 / 6.0)
+#define FD1_W2 (5.0 / 21.0)
+#define FD1_W3 (5.0 / 84.0)
+#define FD1_W4 (5.0 / 504.0)
+#define FD1_W5 (1.0 / 1260.0)
+#define FD2_W0 (5269.0 / 1800.0)
+#define FD2_W1 (5.0 / 3.0)
+#define FD2_W2 (5.0 / 21.0)
+#define FD2_W3 (5.0 / 126.0)
+#define FD2_W4 (5.0 / 1008.0)
+#define FD2_W5 (1.0 / 3150.0)

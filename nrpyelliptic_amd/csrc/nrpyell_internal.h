@@ -1,0 +1,101 @@
+// nrpyell_internal.h -- declarations shared by the translation units of libnrpyell_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/nrpyelliptic_amd.h"
+#include "nrpyell_common.h"
+
+#define NRPYELL_HIST 4096
+
+// Device-resident convergence state of one solve.
+struct SolverDeviceState {
+  int done;          // 0 running, 1 converged, 2 NaN
+  int steps;         // RK4 steps accounted by residual checks
+  int nhist;
+  int pad;
+  double last_log10;
+  int hist_step[NRPYELL_HIST];
+  double hist_log10[NRPYELL_HIST];
+};
+
+struct Punctures {
+  double m[2];
+  double pos[2][3];
+  double P[2][3];
+  double S[2][3];
+};
+
+// One outer point for the drop-in radiation BC: point p, interior neighbour q,
+// clamped stencil offsets along x0,x1,x2 for each, geometry r, dxx0/dr, dxx1/dr, dxx2/dr.
+struct RadPoint {
+  long p, q;
+  int off[2][3];
+  double geo[2][4];
+};
+
+// Arguments of the fused RK4 substep kernel.
+template <typename T>
+struct SubstepArgs {
+  GridDims g;
+  const T *in;    // stage input (u | v)
+  T *out;         // stage output (stage 1-3) or y_n (stage 4)
+  const T *yn;    // y_n
+  T *rt;          // running total
+  const T *aux;   // AA^2 | psi | wavespeed
+  const T *coef;  // 5 x (G0*G1)
+  const T *geo;   // outer-face geometry: [pt 0..5][i1][r, dxx0/dr, dxx1/dr], pt 0 = last interior i0
+  T cOB_u, cOB_v;
+  T eta, dt;
+  const int *done;  // nonzero: converged, every launch returns at once
+  int tx;           // tile width along i0 (threads), tile height along i1 = blockDim/tx
+  int ntile0, ntile1;
+};
+
+
+// kernel launchers (nrpyell_kernels.hip)
+void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in, double *out,
+                      double eta, hipStream_t st);
+void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in,
+                           double *out, hipStream_t st);
+void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st);
+template <typename T>
+void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
+long launch_residual_l2(const GridDims &g, const void *u, bool u_is_float, const double *aux, const double *coef,
+                        const double *w2d, double *partial, const int *done, double *res_out, hipStream_t st);
+long launch_l2_partial(const GridDims &g, const double *gf, const double *w2d, double *partial, hipStream_t st);
+void launch_l2_finalize(const double *partial, long npart, double volume, SolverDeviceState *dst, int steps_per_check,
+                        double target, double *log10_out, hipStream_t st);
+void launch_inner_gather(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs,
+                         hipStream_t st);
+void launch_radiation_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs, double c_u,
+                           double c_v, double finf_u, double finf_v, hipStream_t st);
+void launch_rfm(const GridDims &g, double AMAX, double w, double b, const double *xx0_d, const double *xx1_d,
+                const rfm_struct &r, hipStream_t st);
+void launch_auxevol(const GridDims &g, double AMAX, double w, double b, const Punctures &pu, const double *xx0_d,
+                    const double *xx1_d, const double *xx2_d, double *aux, hipStream_t st);
+void launch_wavespeed(const GridDims &g, const double *ds2d, double CFL, double dt, double *aux, hipStream_t st);
+void launch_convert_d2f(long n, const double *in, float *out, hipStream_t st);
+void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st);
+void launch_fill(long n, double v, double *out, hipStream_t st);
+
+// host setup (nrpyell_host.cpp)
+GridDims dims_from_params(const paramstruct *p);
+void host_min_ds_2d(const paramstruct *p, REAL *const xx[3], double *ds2d);  // G0*G1, interior entries
+void host_outer_geometry(const paramstruct *p, REAL *const xx[3], double *geo);  // 6*G1*3
+void host_l2_weights(const paramstruct *p, REAL *const xx[3], double radius, double *w2d, double *volume);
+void host_radial_geometry(const paramstruct *p, double x0, double x1, double out[4]);
+int host_bcstruct(const paramstruct *params, REAL *xx[3], bc_struct *bc, std::string *err);
+// checks the closed-form ghost map against a bcstruct table; returns 0 when identical
+int ghostmap_verify(const GridDims &g, const bc_struct *bc, std::string *err);
+
+// error plumbing (nrpyell_capi.hip)
+void set_last_error(const std::string &s);
+[[noreturn]] void fatal(const char *what, const std::string &msg);
+#define HIPCHK(x)                                                                                 \
+  do {                                                                                            \
+    hipError_t e_ = (x);                                                                          \
+    if (e_ != hipSuccess) fatal(#x, std::string(hipGetErrorString(e_)) + " at " + __FILE__ + ":" + \
+                                        std::to_string(__LINE__));                                \
+  } while (0)

@@ -1,0 +1,690 @@
+// nrpyell_kernels.hip -- gfx950 kernels of the hyperbolic-relaxation hot path.
+//
+// Kernels (all one thread per grid point, i0 across lanes so every load is coalesced):
+//   rhs_exact_kernel        rhs_eval.c:7-117 with the reference FMA tree (drop-in rhs_eval)
+//   residual_exact_kernel   residual_all_points.c:7-112 (drop-in residual_all_points)
+//   substep_kernel<T,S>     one RK4 substep fused: stencil RHS + radiation BC on the outer
+//                           ghost layers + stage update + parity-image writes of the ghost
+//                           zones (MoL_step_forward_in_time.c:11-176 with
+//                           apply_bcs_outerradiation_and_inner.c:369-469 folded in)
+//   residual_l2_kernel<T>   residual + masked, volume-weighted sum of squares, per-block
+//                           partials reduced in a fixed order (L2_norm_of_gf.c:6-43)
+//   radiation_bc_kernel, inner_gather_kernel   drop-in apply_bcs_* entry points
+//   setup kernels           rfm factors, auxevol (psi_bg, AA^2), wavespeed, coefficient tables
+//
+// Built with -ffp-contract=off: every fused multiply-add below is an explicit fma().
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "nrpyell_common.h"
+#include "nrpyell_internal.h"
+
+#define NG NRPYELL_NG
+
+// ---------------------------------------------------------------------------------------
+// Reference-exact interior operator (rhs_eval.c:86-110; SIMD_intrinsics.h FMA semantics).
+// ---------------------------------------------------------------------------------------
+struct RfmPtrs {
+  const double *f0, *f0_D0, *f0_DD00, *f1, *f1_D1, *f2, *f2_D0, *f3, *f3_D0;
+};
+
+__device__ inline double fd1_exact(const double m[6], const double p[6]) {
+  return fma(5.0 / 504.0, m[4] - p[4],
+             fma(5.0 / 6.0, p[1] - m[1],
+                 fma(5.0 / 84.0, p[3] - m[3], fma(1.0 / 1260.0, p[5] - m[5], (5.0 / 21.0) * (m[2] - p[2])))));
+}
+__device__ inline double fd2_exact(const double m[6], const double p[6], double c0) {
+  return fma(5.0 / 126.0, m[3] + p[3],
+             fma(5.0 / 3.0, m[1] + p[1],
+                 fma(1.0 / 3150.0, m[5] + p[5], -fma(5.0 / 1008.0, m[4] + p[4], fma(5.0 / 21.0, m[2] + p[2], c0)))));
+}
+
+// Flat-space Laplacian in SinhSymTP plus AA^2/(8 (psi+u)^7), reference expression tree.
+__device__ inline double lap_source_exact(const GridDims &g, const RfmPtrs &r, const double *u, long p, int i0,
+                                          int i1, double psi, double AA) {
+  double m0[6], p0[6], m1[6], p1[6], m2[6], p2[6];
+  const long s1 = g.G0, s2 = (long)g.G0 * g.G1;
+#pragma unroll
+  for (int k = 1; k <= 5; k++) {
+    m0[k] = u[p - k]; p0[k] = u[p + k];
+    m1[k] = u[p - k * s1]; p1[k] = u[p + k * s1];
+    m2[k] = u[p - k * s2]; p2[k] = u[p + k * s2];
+  }
+  const double uu = u[p];
+  const double c0 = (5269.0 / 1800.0) * uu;
+  const double dD0 = g.invdx0 * fd1_exact(m0, p0);
+  const double dD1 = g.invdx1 * fd1_exact(m1, p1);
+  const double dDD00 = (g.invdx0 * g.invdx0) * fd2_exact(m0, p0, c0);
+  const double dDD11 = (g.invdx1 * g.invdx1) * fd2_exact(m1, p1, c0);
+  const double dDD22 = (g.invdx2 * g.invdx2) * fd2_exact(m2, p2, c0);
+  const long k2 = i0 + (long)g.G0 * i1;
+  const double f0 = r.f0[i0], f0p = r.f0_D0[i0], f0pp = r.f0_DD00[i0];
+  const double f1 = r.f1[i1], f1p = r.f1_D1[i1];
+  const double f2 = r.f2[k2], f2_0 = r.f2_D0[k2];
+  const double f3 = r.f3[i0], f3p = r.f3_D0[i0];
+  const double a0 = 1.0 / (f2 * f2);
+  const double a2 = (f3 * f3) / (f0p * f0p);
+  const double a3 = 1.0 / ((f2 * f2) * f2);
+  const double X = psi + uu;
+  const double X7 = (((((X * X) * X) * X) * X) * X) * X;
+  const double num = fma(f3, fma(f0p, f2_0, f0pp * f2), -(f0p * (f2 * f3p)));
+  const double tA = num / ((f0p * f0p) * f0p);
+  const double tB = ((a0 * -1.0) * ((f3 * f3) / f0p)) / f0;
+  const double inner = fma(a2, a3 * f2_0, -tB);
+  const double B0 = fma(a3 * f3, tA, -inner);
+  double e = fma(a0, dDD11, -(dD0 * B0));
+  e = fma(a0 * f1p, dD1 / f1, e);
+  e = fma(1.0 / 8.0, AA / X7, e);
+  e = fma(a0, a2 * dDD00, e);
+  e = fma(dDD22, (1.0 / (f1 * f1)) / (f0 * f0), e);
+  return e;
+}
+
+__global__ void __launch_bounds__(256) rhs_exact_kernel(GridDims g, RfmPtrs r, const double *aux, const double *in,
+                                                        double *out, double eta) {
+  const int i0 = NG + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = NG + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i2 = NG + blockIdx.z;
+  if (i0 >= g.N0 + NG || i1 >= g.N1 + NG) return;
+  const long p = idx3(g, i0, i1, i2), nt = g.ntot;
+  const double *u = in, *v = in + nt;
+  const double e = lap_source_exact(g, r, u, p, i0, i1, aux[PSI_BACKGROUNDGF * nt + p], aux[ADD_TIMES_AUUGF * nt + p]);
+  const double c = aux[WAVESPEEDGF * nt + p];
+  out[p] = fma(-eta, u[p], v[p]);
+  out[nt + p] = (c * c) * e;
+}
+
+__global__ void __launch_bounds__(256) residual_exact_kernel(GridDims g, RfmPtrs r, const double *aux,
+                                                             const double *in, double *out) {
+  const int i0 = NG + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = NG + blockIdx.y * blockDim.y + threadIdx.y;
+  const int i2 = NG + blockIdx.z;
+  if (i0 >= g.N0 + NG || i1 >= g.N1 + NG) return;
+  const long p = idx3(g, i0, i1, i2), nt = g.ntot;
+  out[p] = lap_source_exact(g, r, in, p, i0, i1, aux[PSI_BACKGROUNDGF * nt + p], aux[ADD_TIMES_AUUGF * nt + p]);
+}
+
+void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in, double *out,
+                      double eta, hipStream_t st) {
+  RfmPtrs r{rfm->f0_of_xx0, rfm->f0_of_xx0__D0, rfm->f0_of_xx0__DD00, rfm->f1_of_xx1, rfm->f1_of_xx1__D1,
+            rfm->f2_of_xx0_xx1, rfm->f2_of_xx0_xx1__D0, rfm->f3_of_xx0, rfm->f3_of_xx0__D0};
+  dim3 blk(64, 4), grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2);
+  if (out)
+    hipLaunchKernelGGL(rhs_exact_kernel, grd, blk, 0, st, g, r, aux, in, out, eta);
+}
+
+void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in,
+                           double *out, hipStream_t st) {
+  RfmPtrs r{rfm->f0_of_xx0, rfm->f0_of_xx0__D0, rfm->f0_of_xx0__DD00, rfm->f1_of_xx1, rfm->f1_of_xx1__D1,
+            rfm->f2_of_xx0_xx1, rfm->f2_of_xx0_xx1__D0, rfm->f3_of_xx0, rfm->f3_of_xx0__D0};
+  dim3 blk(64, 4), grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2);
+  hipLaunchKernelGGL(residual_exact_kernel, grd, blk, 0, st, g, r, aux, in, out);
+}
+
+// ---------------------------------------------------------------------------------------
+// Precombined metric coefficients for the fused solver: the (i0,i1)-dependent factors of the
+// reference expression (rhs_eval.c:98) with the FD 1/dx factors folded in, so the per-point
+// work is 5 FMAs and one division:
+//   L u = K00 s00 + K11 s11 + K22 s22 + K1 s1 + K0 s0
+//   s00 = sum_k w2_k (u[i+k]+u[i-k]) - w2_0 u  (raw second difference along x0; s11, s22 alike)
+//   s0  = sum_k w1_k (u[i+k]-u[i-k])           (raw first difference along x0; s1 alike)
+//   K00 = f3^2/(f0'^2 f2^2) dx0^-2, K11 = dx1^-2/f2^2, K22 = dx2^-2/(f0^2 f1^2),
+//   K1  = f1'/(f1 f2^2) dx1^-1,     K0 = -B dx0^-1,
+//   B   = f3 (f3 (f0' d0f2 + f0'' f2) - f0' f2 f3')/(f0'^3 f2^3) - d0f2 f3^2/(f0'^2 f2^3)
+//         - f3^2/(f0 f0' f2^2)
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__global__ void coef_kernel(GridDims g, RfmPtrs r, T *coef) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = blockIdx.y;
+  if (i0 >= g.G0) return;
+  const long k2 = i0 + (long)g.G0 * i1, n2 = (long)g.G0 * g.G1;
+  const double f0 = r.f0[i0], f0p = r.f0_D0[i0], f0pp = r.f0_DD00[i0];
+  const double f1 = r.f1[i1], f1p = r.f1_D1[i1];
+  const double f2 = r.f2[k2], f2_0 = r.f2_D0[k2];
+  const double f3 = r.f3[i0], f3p = r.f3_D0[i0];
+  const double a0 = 1.0 / (f2 * f2), a2 = (f3 * f3) / (f0p * f0p), a3 = a0 / f2;
+  const double B = f3 * a3 * (f3 * (f0p * f2_0 + f0pp * f2) - f0p * f2 * f3p) / (f0p * f0p * f0p) -
+                   a2 * a3 * f2_0 - a0 * (f3 * f3) / (f0p * f0);
+  coef[0 * n2 + k2] = (T)(a0 * a2 * (g.invdx0 * g.invdx0));
+  coef[1 * n2 + k2] = (T)(a0 * (g.invdx1 * g.invdx1));
+  coef[2 * n2 + k2] = (T)((1.0 / (f1 * f1)) / (f0 * f0) * (g.invdx2 * g.invdx2));
+  coef[3 * n2 + k2] = (T)(a0 * f1p / f1 * g.invdx1);
+  coef[4 * n2 + k2] = (T)(-B * g.invdx0);
+}
+
+void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st) {
+  RfmPtrs r{rfm->f0_of_xx0, rfm->f0_of_xx0__D0, rfm->f0_of_xx0__DD00, rfm->f1_of_xx1, rfm->f1_of_xx1__D1,
+            rfm->f2_of_xx0_xx1, rfm->f2_of_xx0_xx1__D0, rfm->f3_of_xx0, rfm->f3_of_xx0__D0};
+  dim3 blk(64), grd((g.G0 + 63) / 64, g.G1);
+  if (coef64) hipLaunchKernelGGL(coef_kernel<double>, grd, blk, 0, st, g, r, coef64);
+  if (coef32) hipLaunchKernelGGL(coef_kernel<float>, grd, blk, 0, st, g, r, coef32);
+}
+
+// ---------------------------------------------------------------------------------------
+// Fused RK4 substep.
+// ---------------------------------------------------------------------------------------
+template <typename T>
+__device__ inline T fast_lap_source(const T *__restrict__ u, long p, long st1, long st2, const T K[5], T psi, T AA,
+                                    T &uu_out) {
+  T m0[6], p0[6], m1[6], p1[6], m2[6], p2[6];
+#pragma unroll
+  for (int k = 1; k <= 5; k++) {
+    m0[k] = u[p - k]; p0[k] = u[p + k];
+    m1[k] = u[p - k * st1]; p1[k] = u[p + k * st1];
+    m2[k] = u[p - k * st2]; p2[k] = u[p + k * st2];
+  }
+  const T uu = u[p];
+  uu_out = uu;
+  const T w0 = (T)FD2_W0;
+  T s00 = -w0 * uu, s11 = -w0 * uu, s22 = -w0 * uu;
+  s00 = fma((T)FD2_W1, m0[1] + p0[1], s00); s11 = fma((T)FD2_W1, m1[1] + p1[1], s11); s22 = fma((T)FD2_W1, m2[1] + p2[1], s22);
+  s00 = fma((T)-FD2_W2, m0[2] + p0[2], s00); s11 = fma((T)-FD2_W2, m1[2] + p1[2], s11); s22 = fma((T)-FD2_W2, m2[2] + p2[2], s22);
+  s00 = fma((T)FD2_W3, m0[3] + p0[3], s00); s11 = fma((T)FD2_W3, m1[3] + p1[3], s11); s22 = fma((T)FD2_W3, m2[3] + p2[3], s22);
+  s00 = fma((T)-FD2_W4, m0[4] + p0[4], s00); s11 = fma((T)-FD2_W4, m1[4] + p1[4], s11); s22 = fma((T)-FD2_W4, m2[4] + p2[4], s22);
+  s00 = fma((T)FD2_W5, m0[5] + p0[5], s00); s11 = fma((T)FD2_W5, m1[5] + p1[5], s11); s22 = fma((T)FD2_W5, m2[5] + p2[5], s22);
+  T s0 = (T)FD1_W1 * (p0[1] - m0[1]), s1 = (T)FD1_W1 * (p1[1] - m1[1]);
+  s0 = fma((T)-FD1_W2, p0[2] - m0[2], s0); s1 = fma((T)-FD1_W2, p1[2] - m1[2], s1);
+  s0 = fma((T)FD1_W3, p0[3] - m0[3], s0); s1 = fma((T)FD1_W3, p1[3] - m1[3], s1);
+  s0 = fma((T)-FD1_W4, p0[4] - m0[4], s0); s1 = fma((T)-FD1_W4, p1[4] - m1[4], s1);
+  s0 = fma((T)FD1_W5, p0[5] - m0[5], s0); s1 = fma((T)FD1_W5, p1[5] - m1[5], s1);
+  const T X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
+  T e = (T)0.125 * (AA / X7);
+  e = fma(K[0], s00, e);
+  e = fma(K[1], s11, e);
+  e = fma(K[2], s22, e);
+  e = fma(K[3], s1, e);
+  e = fma(K[4], s0, e);
+  return e;
+}
+
+// Stage updates, MoL_step_forward_in_time.c:44-45 (k1), :88-89 (k2), :129-130 (k3), :170 (k4).
+template <typename T, int STAGE>
+__device__ inline void stage_update(T k, T dt, T yn, T &rt, T &out) {
+  if (STAGE == 1) {
+    rt = (T)(1.0 / 6.0) * (dt * k);
+    out = fma((T)0.5, dt * k, yn);
+  } else if (STAGE == 2) {
+    rt = fma((T)(1.0 / 3.0), dt * k, rt);
+    out = fma((T)0.5, dt * k, yn);
+  } else if (STAGE == 3) {
+    rt = fma((T)(1.0 / 3.0), dt * k, rt);
+    out = fma(dt, k, yn);
+  } else {
+    out = yn + fma((T)(1.0 / 6.0), dt * k, rt);
+  }
+}
+
+// 11-point first derivative with the stencil shifted `off` points (-5..-1, or 0 centred);
+// weights are the Fornberg rationals of apply_bcs_outerradiation_and_inner.c:11-141.
+__constant__ double c_fd1_shift[6][11] = {
+    {-1.0 / 1260, 5.0 / 504, -5.0 / 84, 5.0 / 21, -5.0 / 6, 0.0, 5.0 / 6, -5.0 / 21, 5.0 / 84, -5.0 / 504, 1.0 / 1260},
+    {1.0 / 1260, -1.0 / 105, 3.0 / 56, -4.0 / 21, 1.0 / 2, -6.0 / 5, 11.0 / 30, 4.0 / 7, -3.0 / 28, 1.0 / 63, -1.0 / 840},
+    {-1.0 / 840, 1.0 / 72, -3.0 / 40, 1.0 / 4, -7.0 / 12, 21.0 / 20, -7.0 / 4, 319.0 / 420, 3.0 / 8, -1.0 / 24, 1.0 / 360},
+    {1.0 / 360, -2.0 / 63, 1.0 / 6, -8.0 / 15, 7.0 / 6, -28.0 / 15, 7.0 / 3, -8.0 / 3, 341.0 / 280, 2.0 / 9, -1.0 / 90},
+    {-1.0 / 90, 1.0 / 8, -9.0 / 14, 2.0, -21.0 / 5, 63.0 / 10, -7.0, 6.0, -9.0 / 2, 4609.0 / 2520, 1.0 / 10},
+    {1.0 / 10, -10.0 / 9, 45.0 / 8, -120.0 / 7, 35.0, -252.0 / 5, 105.0 / 2, -40.0, 45.0 / 2, -10.0, 7381.0 / 2520}};
+
+
+template <typename T, int STAGE>
+__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
+  if (a.done && *a.done) return;
+  const GridDims &g = a.g;
+  const int tid = threadIdx.x;
+  const int ty_n = blockDim.x / a.tx;
+  const int tx = tid % a.tx, ty = tid / a.tx;
+  const int tile = blockIdx.x;
+  const int t0 = tile % a.ntile0, t1 = tile / a.ntile0;
+  const int i2 = NG + blockIdx.y;
+  const int i0 = NG + t0 * a.tx + tx;
+  const int i1 = NG + t1 * ty_n + ty;
+  const bool valid = (i0 < g.N0 + NG) && (i1 < g.N1 + NG);
+  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1, n2 = (long)g.G0 * g.G1;
+  const int ilast = g.N0 + NG - 1;  // last interior i0
+  const bool has_bc = (NG + t0 * a.tx <= ilast) && (ilast < NG + (t0 + 1) * a.tx);
+
+  __shared__ T sh_rhs[16][2][6];  // [row][gf][pt]: pt 0 = last interior, 1..5 = outer layers
+  __shared__ T sh_dr[16][2][6];
+
+  if (valid) {
+    const long p = idx3(g, i0, i1, i2);
+    const long k2 = i0 + (long)g.G0 * i1;
+    T K[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) K[k] = a.coef[k * n2 + k2];
+    const T psi = a.aux[PSI_BACKGROUNDGF * nt + p], AA = a.aux[ADD_TIMES_AUUGF * nt + p];
+    const T c = a.aux[WAVESPEEDGF * nt + p];
+    T uu;
+    const T L = fast_lap_source<T>(a.in, p, s1, s2, K, psi, AA, uu);
+    const T vv = a.in[nt + p];
+    const T ku = fma(-a.eta, uu, vv);  // rhs_eval.c:109 (NegFusedMulAdd)
+    const T kv = (c * c) * L;
+    if (has_bc && i0 == ilast) {
+      sh_rhs[ty][0][0] = ku;
+      sh_rhs[ty][1][0] = kv;
+    }
+    T rt_u = (STAGE == 1) ? (T)0 : a.rt[p], rt_v = (STAGE == 1) ? (T)0 : a.rt[nt + p];
+    T o_u, o_v;
+    stage_update<T, STAGE>(ku, a.dt, a.yn[p], rt_u, o_u);
+    stage_update<T, STAGE>(kv, a.dt, a.yn[nt + p], rt_v, o_v);
+    if (STAGE != 4) {
+      a.rt[p] = rt_u;
+      a.rt[nt + p] = rt_v;
+    }
+    a.out[p] = o_u;
+    a.out[nt + p] = o_v;
+    for_each_image(g, i0, i1, i2, [&](long d) {
+      a.out[d] = o_u;
+      a.out[nt + d] = o_v;
+    });
+  }
+  if (!has_bc) return;  // uniform per block
+
+  // ---- radiation BC on the x0-max face for this tile's rows (apply_bcs_outerradiation_and_inner.c:369-408)
+  // Phase B: d_r f at the last interior point (pt 0) and the 5 outer layers (pt 1..5).
+  const int rows = ty_n;
+  {
+    const int nwork = rows * 12;
+    for (int w = tid; w < nwork; w += blockDim.x) {
+      const int row = w / 12, rem = w % 12, gf = rem / 6, pt = rem % 6;
+      const int j1 = NG + t1 * ty_n + row;
+      if (j1 >= g.N1 + NG) continue;
+      const int j0 = ilast + pt;
+      const int off = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
+      const T *f = a.in + gf * nt;
+      const long q = idx3(g, j0, j1, i2);
+      T d0 = 0, d1 = 0;
+#pragma unroll
+      for (int j = 0; j < 11; j++) {
+        d0 = fma((T)c_fd1_shift[off][j], f[q + (j - 5 - off)], d0);
+        d1 = fma((T)c_fd1_shift[0][j], f[q + (long)(j - 5) * s1], d1);
+      }
+      d0 *= (T)g.invdx0;
+      d1 *= (T)g.invdx1;
+      const T *geo = a.geo + ((long)pt * g.G1 + j1) * 3;
+      sh_dr[row][gf][pt] = geo[1] * d0 + geo[2] * d1;
+    }
+  }
+  __syncthreads();
+  // Phase C: the innermost-first layer recurrence (serial in the layer, parallel over rows x gf).
+  if (tid < rows * 2) {
+    const int row = tid / 2, gf = tid % 2;
+    const int j1 = NG + t1 * ty_n + row;
+    if (j1 < g.N1 + NG) {
+      const T c = gf == 0 ? a.cOB_u : a.cOB_v;
+      const T *f = a.in + gf * nt;
+      T rhs_prev = sh_rhs[row][gf][0];
+      for (int pt = 1; pt <= 5; pt++) {
+        const T *gq = a.geo + ((long)(pt - 1) * g.G1 + j1) * 3;
+        const T *gp = a.geo + ((long)pt * g.G1 + j1) * 3;
+        const T fq = f[idx3(g, ilast + pt - 1, j1, i2)], fp = f[idx3(g, ilast + pt, j1, i2)];
+        const T rq = gq[0], rp = gp[0];
+        const T out_int = -c * (sh_dr[row][gf][pt - 1] + fq / rq);
+        const T kk = rq * rq * rq * (rhs_prev - out_int);
+        const T rinv = (T)1 / rp;
+        const T r = -c * (sh_dr[row][gf][pt] + fp * rinv) + kk * rinv * rinv * rinv;
+        sh_rhs[row][gf][pt] = r;
+        rhs_prev = r;
+      }
+    }
+  }
+  __syncthreads();
+  // Phase D: stage update at the outer points and their parity images.
+  {
+    const int nwork = rows * 10;
+    for (int w = tid; w < nwork; w += blockDim.x) {
+      const int row = w / 10, rem = w % 10, gf = rem / 5, pt = rem % 5 + 1;
+      const int j1 = NG + t1 * ty_n + row;
+      if (j1 >= g.N1 + NG) continue;
+      const int j0 = ilast + pt;
+      const long p = idx3(g, j0, j1, i2) + gf * nt;
+      T rt_ = (STAGE == 1) ? (T)0 : a.rt[p];
+      T o;
+      stage_update<T, STAGE>(sh_rhs[row][gf][pt], a.dt, a.yn[p], rt_, o);
+      if (STAGE != 4) a.rt[p] = rt_;
+      a.out[p] = o;
+      T *outg = a.out + gf * nt;
+      for_each_image(g, j0, j1, i2, [&](long d) { outg[d] = o; });
+    }
+  }
+}
+
+template <typename T>
+void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st) {
+  dim3 grd(a.ntile0 * a.ntile1, a.g.N2), blk(256);
+  switch (stage) {
+    case 1: hipLaunchKernelGGL((substep_kernel<T, 1>), grd, blk, 0, st, a); break;
+    case 2: hipLaunchKernelGGL((substep_kernel<T, 2>), grd, blk, 0, st, a); break;
+    case 3: hipLaunchKernelGGL((substep_kernel<T, 3>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((substep_kernel<T, 4>), grd, blk, 0, st, a); break;
+  }
+}
+template void launch_substep<double>(const SubstepArgs<double> &, int, hipStream_t);
+template void launch_substep<float>(const SubstepArgs<float> &, int, hipStream_t);
+
+// ---------------------------------------------------------------------------------------
+// Fused residual + L2 (residual_all_points.c + L2_norm_of_gf.c): per point
+// w(i0,i1) * R^2 where w = dV inside the integration radius and 0 outside; each block writes
+// one partial sum, then one block adds the partials in index order (deterministic), and
+// finishes log10(1e-16 + sqrt(S/V)). With a `state` pointer it also advances the solver's
+// device-side convergence state (step counter, done flag, history).
+// ---------------------------------------------------------------------------------------
+template <typename TIN>
+__global__ void __launch_bounds__(256) residual_l2_partial_kernel(GridDims g, const TIN *u, const double *aux,
+                                                                  const double *coef, const double *w2d,
+                                                                  double *partial, const int *done,
+                                                                  double *res_out) {
+  if (done && *done) return;
+  const int i0 = NG + blockIdx.x * 64 + (threadIdx.x % 64);
+  const int i1 = NG + blockIdx.y * 4 + threadIdx.x / 64;
+  const int i2 = NG + blockIdx.z;
+  double acc = 0.0;
+  if (i0 < g.N0 + NG && i1 < g.N1 + NG) {
+    const long p = idx3(g, i0, i1, i2), nt = g.ntot, n2 = (long)g.G0 * g.G1, k2 = i0 + (long)g.G0 * i1;
+    const double w = w2d[k2];
+    double K[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++) K[k] = coef[k * n2 + k2];
+    double m0[6], p0[6], m1[6], p1[6], m2[6], p2[6];
+    const long s1 = g.G0, s2 = n2;
+#pragma unroll
+    for (int k = 1; k <= 5; k++) {
+      m0[k] = (double)u[p - k]; p0[k] = (double)u[p + k];
+      m1[k] = (double)u[p - k * s1]; p1[k] = (double)u[p + k * s1];
+      m2[k] = (double)u[p - k * s2]; p2[k] = (double)u[p + k * s2];
+    }
+    const double uu = (double)u[p];
+    double s00 = -FD2_W0 * uu, s11 = s00, s22 = s00;
+    s00 = fma(FD2_W1, m0[1] + p0[1], s00); s11 = fma(FD2_W1, m1[1] + p1[1], s11); s22 = fma(FD2_W1, m2[1] + p2[1], s22);
+    s00 = fma(-FD2_W2, m0[2] + p0[2], s00); s11 = fma(-FD2_W2, m1[2] + p1[2], s11); s22 = fma(-FD2_W2, m2[2] + p2[2], s22);
+    s00 = fma(FD2_W3, m0[3] + p0[3], s00); s11 = fma(FD2_W3, m1[3] + p1[3], s11); s22 = fma(FD2_W3, m2[3] + p2[3], s22);
+    s00 = fma(-FD2_W4, m0[4] + p0[4], s00); s11 = fma(-FD2_W4, m1[4] + p1[4], s11); s22 = fma(-FD2_W4, m2[4] + p2[4], s22);
+    s00 = fma(FD2_W5, m0[5] + p0[5], s00); s11 = fma(FD2_W5, m1[5] + p1[5], s11); s22 = fma(FD2_W5, m2[5] + p2[5], s22);
+    double s0 = FD1_W1 * (p0[1] - m0[1]), s1d = FD1_W1 * (p1[1] - m1[1]);
+    s0 = fma(-FD1_W2, p0[2] - m0[2], s0); s1d = fma(-FD1_W2, p1[2] - m1[2], s1d);
+    s0 = fma(FD1_W3, p0[3] - m0[3], s0); s1d = fma(FD1_W3, p1[3] - m1[3], s1d);
+    s0 = fma(-FD1_W4, p0[4] - m0[4], s0); s1d = fma(-FD1_W4, p1[4] - m1[4], s1d);
+    s0 = fma(FD1_W5, p0[5] - m0[5], s0); s1d = fma(FD1_W5, p1[5] - m1[5], s1d);
+    const double psi = aux[PSI_BACKGROUNDGF * nt + p], AA = aux[ADD_TIMES_AUUGF * nt + p];
+    const double X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
+    double e = 0.125 * (AA / X7);
+    e = fma(K[0], s00, e);
+    e = fma(K[1], s11, e);
+    e = fma(K[2], s22, e);
+    e = fma(K[3], s1d, e);
+    e = fma(K[4], s0, e);
+    if (res_out) res_out[p] = e;
+    acc = w * e * e;
+  }
+  // block reduction: wave64 shuffles then LDS across the 4 waves
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  __shared__ double sw[4];
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    partial[b] = ((sw[0] + sw[1]) + (sw[2] + sw[3]));
+  }
+}
+
+// L2 of an arbitrary gridfunction (drop-in L2_norm_of_gf): w * gf^2 partials.
+__global__ void __launch_bounds__(256) l2_partial_kernel(GridDims g, const double *gf, const double *w2d,
+                                                         double *partial) {
+  const int i0 = NG + blockIdx.x * 64 + (threadIdx.x % 64);
+  const int i1 = NG + blockIdx.y * 4 + threadIdx.x / 64;
+  const int i2 = NG + blockIdx.z;
+  double acc = 0.0;
+  if (i0 < g.N0 + NG && i1 < g.N1 + NG) {
+    const double v = gf[idx3(g, i0, i1, i2)];
+    acc = w2d[i0 + (long)g.G0 * i1] * v * v;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  __shared__ double sw[4];
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+    partial[b] = ((sw[0] + sw[1]) + (sw[2] + sw[3]));
+  }
+}
+
+// Fixed-order sum of the partials by one block; writes log10(1e-16+sqrt(S/V)) and, when `st`
+// is given, updates the solver convergence state.
+__global__ void __launch_bounds__(256) l2_finalize_kernel(const double *partial, long npart, double volume,
+                                                          SolverDeviceState *st, int steps_per_check,
+                                                          double target, double *log10_out) {
+  if (st && st->done) return;
+  double acc = 0.0;
+  for (long i = threadIdx.x; i < npart; i += 256) acc += partial[i];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  __shared__ double sw[4];
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double S = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+    const double l = log10(1e-16 + sqrt(S / volume));
+    if (log10_out) *log10_out = l;
+    if (st) {
+      st->steps += steps_per_check;
+      st->last_log10 = l;
+      if (st->nhist < NRPYELL_HIST) {
+        st->hist_step[st->nhist] = st->steps;
+        st->hist_log10[st->nhist] = l;
+        st->nhist++;
+      }
+      if (!(l == l)) st->done = 2;  // NaN: stop, reported as NRPYELL_ENAN
+      else if (l <= target) st->done = 1;
+    }
+  }
+}
+
+long launch_residual_l2(const GridDims &g, const void *u, bool u_is_float, const double *aux, const double *coef,
+                        const double *w2d, double *partial, const int *done, double *res_out, hipStream_t st) {
+  dim3 grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2), blk(256);
+  if (u_is_float)
+    hipLaunchKernelGGL(residual_l2_partial_kernel<float>, grd, blk, 0, st, g, (const float *)u, aux, coef, w2d,
+                       partial, done, res_out);
+  else
+    hipLaunchKernelGGL(residual_l2_partial_kernel<double>, grd, blk, 0, st, g, (const double *)u, aux, coef, w2d,
+                       partial, done, res_out);
+  return (long)grd.x * grd.y * grd.z;
+}
+
+long launch_l2_partial(const GridDims &g, const double *gf, const double *w2d, double *partial, hipStream_t st) {
+  dim3 grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2), blk(256);
+  hipLaunchKernelGGL(l2_partial_kernel, grd, blk, 0, st, g, gf, w2d, partial);
+  return (long)grd.x * grd.y * grd.z;
+}
+
+void launch_l2_finalize(const double *partial, long npart, double volume, SolverDeviceState *dst, int steps_per_check,
+                        double target, double *log10_out, hipStream_t st) {
+  hipLaunchKernelGGL(l2_finalize_kernel, dim3(1), dim3(256), 0, st, partial, npart, volume, dst, steps_per_check,
+                     target, log10_out);
+}
+
+// ---------------------------------------------------------------------------------------
+// Drop-in boundary kernels.
+// ---------------------------------------------------------------------------------------
+// apply_bcs_inner_only.c:18-26: gfs[dst] = parity[0] * gfs[src] for both evolved gfs.
+__global__ void inner_gather_kernel(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const double s = (double)par0[k];
+  gfs[dst[k]] = s * gfs[src[k]];
+  gfs[nt + dst[k]] = s * gfs[nt + src[k]];
+}
+
+void launch_inner_gather(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs,
+                         hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(inner_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nt, n, dst, src, par0, gfs);
+}
+
+// One pure outer point of one (which_gz, dirn) list (radiation_bcs,
+// apply_bcs_outerradiation_and_inner.c:369-408). geo holds r, dxx0/dr, dxx1/dr, dxx2/dr for
+// the point and for its interior neighbour; offsets are the clamped stencil shifts.
+__global__ void radiation_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs,
+                                      double c_u, double c_v, double finf_u, double finf_v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const RadPoint P = pts[k];
+  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
+  for (int gf = 0; gf < 2; gf++) {
+    const double *f = gfs + gf * nt;
+    const double c = gf == 0 ? c_u : c_v, finf = gf == 0 ? finf_u : finf_v;
+    double dr[2];
+    for (int w = 0; w < 2; w++) {
+      const long q = w == 0 ? P.p : P.q;
+      const int o0 = P.off[w][0], o1 = P.off[w][1], o2 = P.off[w][2];
+      double d0 = 0, d1 = 0, d2 = 0;
+      for (int j = 0; j < 11; j++) {
+        const double c0 = o0 <= 0 ? c_fd1_shift[-o0][j] : -c_fd1_shift[o0][10 - j];
+        const double c1 = o1 <= 0 ? c_fd1_shift[-o1][j] : -c_fd1_shift[o1][10 - j];
+        d0 = fma(c0, f[q + (j - 5 + o0)], d0);
+        d1 = fma(c1, f[q + (long)(j - 5 + o1) * s1], d1);
+        (void)o2;
+        (void)s2;
+      }
+      dr[w] = P.geo[w][1] * d0 * g.invdx0 + P.geo[w][2] * d1 * g.invdx1;
+    }
+    const double r = P.geo[0][0], r_int = P.geo[1][0];
+    const double out_int = -c * (dr[1] + (f[P.q] - finf) / r_int);
+    const double kk = r_int * r_int * r_int * (rhs[gf * nt + P.q] - out_int);
+    const double rinv = 1.0 / r;
+    rhs[gf * nt + P.p] = -c * (dr[0] + (f[P.p] - finf) * rinv) + kk * rinv * rinv * rinv;
+  }
+}
+
+void launch_radiation_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs, double c_u,
+                           double c_v, double finf_u, double finf_v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(radiation_list_kernel, dim3((n + 127) / 128), dim3(128), 0, st, g, n, pts, gfs, rhs, c_u, c_v,
+                     finf_u, finf_v);
+}
+
+// ---------------------------------------------------------------------------------------
+// Setup kernels (device-side setup for rfm factors, auxevol and wavespeed).
+// ---------------------------------------------------------------------------------------
+struct SinhParams {
+  double AMAX, w, b, D;  // D = exp(1/w) - exp(-1/w)
+};
+
+// rfm_precompute_rfmstruct_define.c:8-91 (closed forms of the SinhSymTP scale factors).
+__global__ void rfm_kernel(GridDims g, SinhParams s, const double *xx0, const double *xx1, rfm_struct r) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = blockIdx.y;
+  if (i0 >= g.G0) return;
+  const double x = xx0[i0], ep = exp(x / s.w), em = exp(-x / s.w);
+  const double f0 = s.AMAX * (ep - em) / s.D;
+  const double f0p = s.AMAX * (ep / s.w + em / s.w) / s.D;
+  const double f0pp = s.AMAX * (ep / (s.w * s.w) - em / (s.w * s.w)) / s.D;
+  const double f3sq = f0 * f0 + s.b * s.b, f3 = sqrt(f3sq);
+  if (i1 == 0) {
+    r.f0_of_xx0[i0] = f0;
+    r.f0_of_xx0__D0[i0] = f0p;
+    r.f0_of_xx0__DD00[i0] = f0pp;
+    r.f0_of_xx0__DDD000[i0] = s.AMAX * (ep / (s.w * s.w * s.w) + em / (s.w * s.w * s.w)) / s.D;
+    r.f3_of_xx0[i0] = f3;
+    r.f3_of_xx0__D0[i0] = f0 * f0p / f3;
+    r.f3_of_xx0__DD00[i0] = (f0p * f0p + f0 * f0pp) / f3 - (f0 * f0p) * (f0 * f0p) / (f3sq * f3);
+  }
+  const double sn = sin(xx1[i1]), cs = cos(xx1[i1]);
+  if (i0 == 0) {
+    r.f1_of_xx1[i1] = sn;
+    r.f1_of_xx1__D1[i1] = cs;
+    r.f1_of_xx1__DD11[i1] = -sn;
+  }
+  const double F = f0 * f0 + s.b * s.b * sn * sn, f2 = sqrt(F);
+  const long k = i0 + (long)g.G0 * i1;
+  r.f2_of_xx0_xx1[k] = f2;
+  r.f2_of_xx0_xx1__D0[k] = f0 * f0p / f2;
+  r.f2_of_xx0_xx1__D1[k] = s.b * s.b * sn * cs / f2;
+  r.f2_of_xx0_xx1__DD00[k] = (f0p * f0p + f0 * f0pp) / f2 - (f0 * f0p) * (f0 * f0p) / (F * f2);
+  r.f2_of_xx0_xx1__DD11[k] = s.b * s.b * (cs * cs - sn * sn) / f2 - (s.b * s.b * sn * cs) * (s.b * s.b * sn * cs) / (F * f2);
+}
+
+void launch_rfm(const GridDims &g, double AMAX, double w, double b, const double *xx0_d, const double *xx1_d,
+                const rfm_struct &r, hipStream_t st) {
+  SinhParams s{AMAX, w, b, exp(1.0 / w) - exp(-1.0 / w)};
+  hipLaunchKernelGGL(rfm_kernel, dim3((g.G0 + 63) / 64, g.G1), dim3(64), 0, st, g, s, xx0_d, xx1_d, r);
+}
+
+// auxevol_gfs_all_points.c:9-95 at every point: psi_bg = 1 + sum m_n/(2 r_n) and the
+// Bowen-York A_ij A_ij with, per puncture (x relative to it, r=|x|),
+//   A_ij = (3/2)(P_i x_j + P_j x_i)/r^3 - (3/2)(x.P) delta_ij/r^3 + (3/2)(x.P) x_i x_j/r^5
+//          - 3[(x cross S)_i x_j + (x cross S)_j x_i]/r^5
+// (NRPyElliptic_SourceTerm.py:48-143 differentiated by hand).
+__global__ void auxevol_kernel(GridDims g, SinhParams s, Punctures pu, const double *xx0, const double *xx1,
+                               const double *xx2, double *aux) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = blockIdx.y, i2 = blockIdx.z;
+  if (i0 >= g.G0) return;
+  const double x0 = xx0[i0], x1 = xx1[i1], x2 = xx2[i2];
+  const double f0 = s.AMAX * (exp(x0 / s.w) - exp(-x0 / s.w)) / s.D;
+  const double X[3] = {f0 * sin(x1) * cos(x2), f0 * sin(x1) * sin(x2), sqrt(f0 * f0 + s.b * s.b) * cos(x1)};
+  double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, psi = 1.0;
+  for (int n = 0; n < 2; n++) {
+    const double x[3] = {X[0] - pu.pos[n][0], X[1] - pu.pos[n][1], X[2] - pu.pos[n][2]};
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], r = sqrt(r2);
+    const double ir3 = 1.0 / (r2 * r), ir5 = ir3 / r2;
+    const double *P = pu.P[n], *S = pu.S[n];
+    const double xP = x[0] * P[0] + x[1] * P[1] + x[2] * P[2];
+    const double xS[3] = {x[1] * S[2] - x[2] * S[1], x[2] * S[0] - x[0] * S[2], x[0] * S[1] - x[1] * S[0]};
+    psi += pu.m[n] / (2.0 * r);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        A[i][j] += 1.5 * (P[i] * x[j] + P[j] * x[i]) * ir3 - (i == j ? 1.5 * xP * ir3 : 0.0) +
+                   1.5 * xP * x[i] * x[j] * ir5 - 3.0 * (xS[i] * x[j] + xS[j] * x[i]) * ir5;
+  }
+  double AA = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) AA += A[i][j] * A[i][j];
+  const long p = idx3(g, i0, i1, i2);
+  aux[ADD_TIMES_AUUGF * g.ntot + p] = AA;
+  aux[PSI_BACKGROUNDGF * g.ntot + p] = psi;
+}
+
+void launch_auxevol(const GridDims &g, double AMAX, double w, double b, const Punctures &pu, const double *xx0_d,
+                    const double *xx1_d, const double *xx2_d, double *aux, hipStream_t st) {
+  SinhParams s{AMAX, w, b, exp(1.0 / w) - exp(-1.0 / w)};
+  hipLaunchKernelGGL(auxevol_kernel, dim3((g.G0 + 63) / 64, g.G1, g.G2), dim3(64), 0, st, g, s, pu, xx0_d, xx1_d,
+                     xx2_d, aux);
+}
+
+// wavespeed_gf_all_points.c:9-44 from a 2-D table of min proper spacing (host-computed).
+__global__ void wavespeed_kernel(GridDims g, const double *ds2d, double scale, double *aux) {
+  const int i0 = NG + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = NG + blockIdx.y, i2 = NG + blockIdx.z;
+  if (i0 >= g.N0 + NG) return;
+  aux[WAVESPEEDGF * g.ntot + idx3(g, i0, i1, i2)] = scale * ds2d[i0 + (long)g.G0 * i1];
+}
+
+void launch_wavespeed(const GridDims &g, const double *ds2d, double CFL, double dt, double *aux, hipStream_t st) {
+  (void)dt;
+  hipLaunchKernelGGL(wavespeed_kernel, dim3((g.N0 + 63) / 64, g.N1, g.N2), dim3(64), 0, st, g, ds2d, CFL, aux);
+}
+
+// Precision conversion over whole buffers (FP64 <-> FP32 shadow state).
+template <typename TO, typename TI>
+__global__ void convert_kernel(long n, const TI *in, TO *out) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (TO)in[k];
+}
+void launch_convert_d2f(long n, const double *in, float *out, hipStream_t st) {
+  hipLaunchKernelGGL((convert_kernel<float, double>), dim3((n + 255) / 256), dim3(256), 0, st, n, in, out);
+}
+void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st) {
+  hipLaunchKernelGGL((convert_kernel<double, float>), dim3((n + 255) / 256), dim3(256), 0, st, n, in, out);
+}
+
+__global__ void fill_kernel(long n, double v, double *out) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = v;
+}
+void launch_fill(long n, double v, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, v, out);
+}

@@ -1,0 +1,319 @@
+"""Python host layer over libnrpyell_hip.so.
+
+`Problem` mirrors the reference driver's set-up sequence (NRPyElliptic_Ccodes/main.c:12-105)
+call for call, through this library's C ABI, so the device holds the same griddata the
+reference would build on the host.  `Solver` wraps the resident hipGraph solver.
+
+The product path never falls back to CPU code: if the HIP library is missing or no GPU is
+visible, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import abi
+from .config import PunctureConfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnrpyell_hip.so")
+
+NRPYELL_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "EGHOSTMAP", -4: "ENOTCONVERGED", -5: "ENAN"}
+
+
+class solve_options(C.Structure):
+    _fields_ = [("precision", C.c_int), ("check_every", C.c_int), ("steps_per_graph", C.c_int),
+                ("log10_target", C.c_double), ("max_steps", C.c_int), ("fp32_switch_log10", C.c_double),
+                ("fp32_plateau_ratio", C.c_double), ("integration_radius", C.c_double),
+                ("tile_i0", C.c_int), ("tile_i1", C.c_int)]
+
+
+class solve_result(C.Structure):
+    _fields_ = [("steps", C.c_int), ("fp32_steps", C.c_int), ("log10_l2", C.c_double),
+                ("log10_l2_initial", C.c_double), ("seconds", C.c_double), ("converged", C.c_int)]
+
+
+class NrpyellError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def lib() -> C.CDLL:
+    """Load libnrpyell_hip.so (built by nrpyelliptic_amd.build / __graft_entry__.build)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise NrpyellError(f"{LIB_PATH} is not built; run __graft_entry__.build() (hipcc --offload-arch=gfx950)")
+    L = abi.bind(C.CDLL(LIB_PATH))
+    vp = C.c_void_p
+    L.nrpyell_device_synchronize.restype = C.c_int
+    L.nrpyell_malloc_device.argtypes = [C.POINTER(vp), C.c_size_t]
+    L.nrpyell_free_device.argtypes = [vp]
+    L.nrpyell_memcpy_h2d.argtypes = [vp, vp, C.c_size_t]
+    L.nrpyell_memcpy_d2h.argtypes = [vp, vp, C.c_size_t]
+    L.nrpyell_memset_device.argtypes = [vp, C.c_int, C.c_size_t]
+    L.nrpyell_last_error.restype = C.c_char_p
+    L.nrpyell_bcstruct_set_up_checked.argtypes = [C.POINTER(abi.paramstruct), abi.XX3, C.POINTER(abi.bc_struct)]
+    L.nrpyell_bcstruct_free.argtypes = [C.POINTER(abi.bc_struct)]
+    L.nrpyell_bcstruct_free.restype = None
+    L.nrpyell_ghostmap_check.argtypes = [C.POINTER(abi.paramstruct), C.POINTER(abi.bc_struct)]
+    L.nrpyell_default_options.argtypes = [C.POINTER(solve_options)]
+    L.nrpyell_default_options.restype = None
+    L.nrpyell_solver_create.argtypes = [C.POINTER(abi.griddata_struct), C.c_double, C.POINTER(solve_options),
+                                        C.POINTER(vp)]
+    L.nrpyell_solver_destroy.argtypes = [vp]
+    L.nrpyell_solver_solve.argtypes = [vp, C.POINTER(solve_result)]
+    L.nrpyell_solver_steps.argtypes = [vp, C.c_int]
+    L.nrpyell_solver_residual_l2.argtypes = [vp, C.POINTER(C.c_double)]
+    L.nrpyell_solver_time_substep.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    _LIB = L
+    return L
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != NRPYELL_OK:
+        msg = lib().nrpyell_last_error().decode(errors="replace")
+        raise NrpyellError(f"{what}: {ERRORS.get(rc, rc)}: {msg}")
+
+
+def default_options(**kw) -> solve_options:
+    o = solve_options()
+    lib().nrpyell_default_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class DeviceArray:
+    """A device allocation of float64 (or another dtype) made through the library."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n, self.dtype = int(n), np.dtype(dtype)
+        self.ptr = C.c_void_p()
+        check(lib().nrpyell_malloc_device(C.byref(self.ptr), self.nbytes), "malloc")
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(lib().nrpyell_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes), "h2d")
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        check(lib().nrpyell_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "d2h")
+        return out
+
+    def as_real_p(self):
+        return C.cast(self.ptr, abi.REALP)
+
+    def free(self):
+        if self.ptr:
+            lib().nrpyell_free_device(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def download_ptr(p, n: int, dtype=np.float64) -> np.ndarray:
+    out = np.empty(n, dtype=dtype)
+    check(lib().nrpyell_memcpy_d2h(out.ctypes.data, C.cast(p, C.c_void_p), out.nbytes), "d2h")
+    return out
+
+
+def upload_ptr(p, a: np.ndarray) -> None:
+    a = np.ascontiguousarray(a)
+    check(lib().nrpyell_memcpy_h2d(C.cast(p, C.c_void_p), a.ctypes.data, a.nbytes), "h2d")
+
+
+class Problem:
+    """One puncture configuration set up on the GPU exactly as main.c:12-105 sets it up on
+    the host: eigen grid -> bcstruct, SinhSymTP grid -> dt, MoL buffers, rfm factors, zero
+    initial guess, auxevol (psi_bg, AA^2), wavespeed, wavespeed at the outer boundary."""
+
+    def __init__(self, cfg: PunctureConfig):
+        L = lib()
+        self.cfg = cfg
+        gd = abi.griddata_struct()
+        self.gd = gd
+        p = gd.params
+        L.set_Cparameters_to_default(C.byref(p))
+        p.outer_bc_type = abi.RADIATION_OUTER_BCS
+        cfg.fill_params(p)
+        Nxx = (C.c_int * 3)(*[int(n) for n in cfg.Nxx])
+        L.set_Nxx_dxx_invdx_params__and__xx(1, C.byref(Nxx), C.byref(p), gd.xx)
+        check(L.nrpyell_bcstruct_set_up_checked(C.byref(p), gd.xx, C.byref(gd.bcstruct)), "bcstruct_set_up")
+        self._free_xx()
+        L.set_Nxx_dxx_invdx_params__and__xx(0, C.byref(Nxx), C.byref(p), gd.xx)
+        self.Ng = [p.Nxx_plus_2NGHOSTS0, p.Nxx_plus_2NGHOSTS1, p.Nxx_plus_2NGHOSTS2]
+        self.ntot = self.Ng[0] * self.Ng[1] * self.Ng[2]
+        self.xx = [np.ctypeslib.as_array(gd.xx[d], shape=(self.Ng[d],)) for d in range(3)]
+        self.dt = L.find_timestep(C.byref(p), gd.xx, cfg.CFL)
+        L.MoL_malloc_y_n_gfs(C.byref(p), C.byref(gd.gridfuncs))
+        L.rfm_precompute_rfmstruct_malloc(C.byref(p), C.byref(gd.rfmstruct))
+        L.rfm_precompute_rfmstruct_define(C.byref(p), gd.xx, C.byref(gd.rfmstruct))
+        L.initial_guess_all_points(C.byref(p), gd.xx, gd.gridfuncs.y_n_gfs)
+        L.MoL_malloc_non_y_n_gfs(C.byref(p), C.byref(gd.gridfuncs))
+        L.auxevol_gfs_all_points(C.byref(p), gd.xx, gd.gridfuncs.auxevol_gfs)
+        L.wavespeed_gf_all_points(C.byref(p), cfg.CFL, self.dt, gd.xx, gd.gridfuncs.auxevol_gfs)
+        c = L.compute_wavespeed_at_OB(C.byref(p), gd.gridfuncs.auxevol_gfs)
+        p.uu_wavespeed_at_OB = p.vv_wavespeed_at_OB = c
+        self._solver: Optional[Solver] = None
+        self._closed = False
+
+    def _free_xx(self):
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        for d in range(3):
+            libc.free(C.cast(self.gd.xx[d], C.c_void_p))
+            self.gd.xx[d] = abi.REALP()
+
+    # --- device arrays of the griddata ----------------------------------------------------
+    def get(self, name: str) -> np.ndarray:
+        n = (3 if name == "auxevol_gfs" else 2) * self.ntot
+        return download_ptr(getattr(self.gd.gridfuncs, name), n)
+
+    def set(self, name: str, a: np.ndarray) -> None:
+        upload_ptr(getattr(self.gd.gridfuncs, name), np.ascontiguousarray(a, dtype=np.float64))
+
+    def rfm(self, field: str) -> np.ndarray:
+        n = self.Ng[0] if field.startswith(("f0", "f3")) else self.Ng[1] if field.startswith("f1") \
+            else self.Ng[0] * self.Ng[1]
+        return download_ptr(getattr(self.gd.rfmstruct, field), n)
+
+    # --- the reference-surface operators ----------------------------------------------------
+    def rhs(self, state: np.ndarray) -> np.ndarray:
+        L = lib()
+        din = DeviceArray(2 * self.ntot).upload(state)
+        dout = DeviceArray(2 * self.ntot).upload(np.zeros(2 * self.ntot))
+        L.rhs_eval(C.byref(self.gd.params), C.byref(self.gd.rfmstruct), self.gd.gridfuncs.auxevol_gfs,
+                   din.as_real_p(), dout.as_real_p())
+        return dout.download()
+
+    def residual(self, state: np.ndarray) -> np.ndarray:
+        din = DeviceArray(2 * self.ntot).upload(state)
+        dout = DeviceArray(self.ntot).upload(np.zeros(self.ntot))
+        lib().residual_all_points(C.byref(self.gd.params), C.byref(self.gd.rfmstruct), self.gd.gridfuncs.auxevol_gfs,
+                                  din.as_real_p(), dout.as_real_p())
+        return dout.download()
+
+    def apply_radiation_and_inner(self, gfs: np.ndarray, rhs: np.ndarray) -> np.ndarray:
+        p = self.gd.params
+        dg = DeviceArray(2 * self.ntot).upload(gfs)
+        dr = DeviceArray(2 * self.ntot).upload(rhs)
+        ws = (C.c_double * 2)(p.uu_wavespeed_at_OB, p.vv_wavespeed_at_OB)
+        finf = (C.c_double * 2)(0.0, 0.0)
+        lib().apply_bcs_outerradiation_and_inner(C.byref(p), C.byref(self.gd.bcstruct), self.gd.xx, C.byref(ws),
+                                                 C.byref(finf), dg.as_real_p(), dr.as_real_p())
+        return dr.download()
+
+    def apply_inner_only(self, gfs: np.ndarray) -> np.ndarray:
+        dg = DeviceArray(2 * self.ntot).upload(gfs)
+        lib().apply_bcs_inner_only(C.byref(self.gd.params), C.byref(self.gd.bcstruct), dg.as_real_p())
+        return dg.download()
+
+    def mol_step(self, dt: Optional[float] = None) -> None:
+        lib().MoL_step_forward_in_time(C.byref(self.gd), self.dt if dt is None else dt)
+
+    def l2(self, gf: np.ndarray, radius: float = 100.0, gf_index: int = 0) -> float:
+        d = DeviceArray(gf.size).upload(gf)
+        return lib().L2_norm_of_gf(C.byref(self.gd.params), gf_index, radius, self.gd.xx, d.as_real_p())
+
+    def log10_residual(self, radius: float = 100.0) -> float:
+        """residual_all_points + L2_norm_of_gf on y_n, as main.c:146-151 does each step."""
+        L = lib()
+        p = self.gd.params
+        gf = self.gd.gridfuncs
+        L.residual_all_points(C.byref(p), C.byref(self.gd.rfmstruct), gf.auxevol_gfs, gf.y_n_gfs,
+                              gf.diagnostic_output_gfs)
+        return L.L2_norm_of_gf(C.byref(p), abi.UUGF, radius, self.gd.xx, gf.diagnostic_output_gfs)
+
+    def bc_tables(self):
+        bc = self.gd.bcstruct
+        n = bc.bc_info.num_inner_boundary_points
+        raw = C.string_at(C.cast(bc.inner_bc_array, C.c_void_p), n * C.sizeof(abi.innerpt_bc_struct))
+        rec = np.frombuffer(raw, dtype=np.dtype([("dst", "<i4"), ("src", "<i4"), ("par", "i1", 10), ("pad", "V2")]))
+        outer = []
+        for gz in range(abi.NGHOSTS):
+            for dirn in range(3):
+                m = bc.bc_info.num_pure_outer_boundary_points[gz][dirn]
+                arr = bc.pure_outer_bc_array[dirn + 3 * gz]
+                outer.append([(arr[k].i0, arr[k].i1, arr[k].i2, arr[k].FACEX0, arr[k].FACEX1, arr[k].FACEX2)
+                              for k in range(m)])
+        return rec, outer
+
+    # --- resident solver ----------------------------------------------------------------------
+    def solver(self, **opts) -> "Solver":
+        return Solver(self, **opts)
+
+    def close(self):
+        if self._closed:
+            return
+        L = lib()
+        p = self.gd.params
+        if self.gd.gridfuncs.y_n_gfs:
+            L.MoL_free_memory_y_n_gfs(C.byref(p), C.byref(self.gd.gridfuncs))
+        if self.gd.gridfuncs.k_odd_gfs:
+            L.MoL_free_memory_non_y_n_gfs(C.byref(p), C.byref(self.gd.gridfuncs))
+        if self.gd.rfmstruct.f0_of_xx0:
+            L.rfm_precompute_rfmstruct_freemem(C.byref(p), C.byref(self.gd.rfmstruct))
+        L.nrpyell_bcstruct_free(C.byref(self.gd.bcstruct))
+        self._closed = True
+
+
+class Solver:
+    """Resident solver: hipGraph-captured RK4 steps with an on-device residual check every
+    `check_every` steps (FP64, or FP32 bulk then FP64 finish with precision=32)."""
+
+    def __init__(self, prob: Problem, **opts):
+        self.prob = prob
+        self.opts = default_options(**opts)
+        self.h = C.c_void_p()
+        check(lib().nrpyell_solver_create(C.byref(prob.gd), prob.dt, C.byref(self.opts), C.byref(self.h)),
+              "solver_create")
+
+    def solve(self) -> solve_result:
+        r = solve_result()
+        rc = lib().nrpyell_solver_solve(self.h, C.byref(r))
+        self.last = r
+        if rc not in (NRPYELL_OK, -4):
+            check(rc, "solve")
+        return r
+
+    def steps(self, n: int) -> None:
+        check(lib().nrpyell_solver_steps(self.h, n), "steps")
+
+    def log10_residual(self) -> float:
+        v = C.c_double()
+        check(lib().nrpyell_solver_residual_l2(self.h, C.byref(v)), "residual_l2")
+        return v.value
+
+    def time_substep(self, stage: int, n: int) -> float:
+        v = C.c_double()
+        check(lib().nrpyell_solver_time_substep(self.h, stage, n, C.byref(v)), "time_substep")
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().nrpyell_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
