@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Benchmark of the hyperbolic-relaxation hot path on one MI355X.
+
+Workload: the 128x128x16 SinhSymTP equal-mass boosted-puncture solve (m = 0.5 each at
+z = +-5, P = (0, +-0.0953, 0), no spins; AMAX 1e6, SINHWAA 0.07, bScale 5, eta 20, CFL 0.7),
+zero initial guess, FP64.  A "step" is one RK4 step (4 fused substep kernels, residual + L2
+reduced every step as the reference driver does), graph-replayed from HBM-resident state.
+
+Prints ONE JSON line: gp-substeps/s (4 * 262144 interior points * steps / s) for exactly
+--steps steps after --warmup steps, the roofline of the fused substep kernel, the compiled
+reference C timed on this host's cores on a bounded sample, and the time to log10 L2 <= -10.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-residual 1e-10 (NN-seeded vs plain FP64); grid-pt RK4 substeps/s, %HBM"
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per gp-substep of the fused RK4 substep (FP64): stage k1 72 B, k2/k3 104 B,
+# k4 88 B (reads of stage input, aux c/psi/AA2, y_n, running total; writes of stage output and
+# running total), mean 92 B
+BYTES_PER_GP_SUBSTEP = 92.0
+# 9000 steps at 128x128x16 in 146.68 s on 6 cores (Tutorial-Start_to_Finish-NRPyElliptic.ipynb:1477)
+PUBLISHED_GP_SUBSTEPS = 4 * 262144 * 9001 / 146.68
+
+
+def cpu_baseline(steps: int, warmup: int) -> dict:
+    """The reference driver's loop body (residual + L2 + MoL step, main.c:138-187) of the
+    compiled reference C at 128x128x16, or the C restatement when the reference build is
+    absent. Runs in this (child) process; OMP threads come from the environment."""
+    import oracle
+    from nrpyelliptic_amd import config
+
+    cfg = config.equal_mass_boosted()
+    kind = "reference" if oracle.ref_lib() is not None else "port"
+    if kind == "reference":
+        R = oracle.RefProblem(cfg)
+
+        def step():
+            R.l2(R.residual(R.arr("y_n_gfs")))
+            R.mol_step()
+    else:
+        O = oracle.OracleProblem(cfg)
+        st = O.new_state()
+
+        def step():
+            O.log10_residual(st["y_n"])
+            O.mol_step(st)
+    for _ in range(warmup):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    dt = time.perf_counter() - t0
+    return {"value": 4 * 262144 * steps / dt, "unit": "gp-substeps/s", "kind": kind, "seconds": dt,
+            "ms_per_step": 1e3 * dt / steps}
+
+
+def run_cpu_baseline_subprocess(steps=150, warmup=5):
+    cores = os.cpu_count() or 1
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    try:
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", str(steps), str(warmup)],
+                           env=env, capture_output=True, text=True, timeout=600)
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+    except Exception as e:  # the baseline is reported, never required
+        return {"value": None, "unit": "gp-substeps/s", "cores": cores, "kind": None, "sample": f"failed: {e}"}
+    cpu = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    out.update(cores=cores, sample=f"{steps} RK4 steps (+{warmup} warmup) of the 128x128x16 solve from u=v=0, "
+                                   f"residual+L2 every step as main.c; OpenMP {cores} threads on '{cpu}'")
+    return out
+
+
+def load_traffic():
+    """HBM bytes per substep launch from the committed rocprofv3 PMC summary, scaled by the
+    copy-kernel calibration recorded with it (null when absent)."""
+    path = os.path.join(ROOT, "profiles", "substep_pmc_128x128x16.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-baseline-only", nargs=2, type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.cpu_baseline_only:
+        print(json.dumps(cpu_baseline(*args.cpu_baseline_only)))
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (the solve is one GPU's full, unsharded "
+              "problem); --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from nrpyelliptic_amd import config, runtime
+
+    cfg = config.equal_mass_boosted()
+    P = runtime.Problem(cfg)
+    npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
+    S = P.solver(steps_per_graph=100)
+
+    # warmup, then exactly --steps production steps from u = v = 0 (HIP events, solver stream)
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    S.timed_steps(args.warmup)
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    runtime.lib().nrpyell_device_synchronize()
+    ms, _ = S.timed_steps(args.steps)
+    runtime.lib().nrpyell_device_synchronize()
+    sec = ms / 1e3
+    value = 4 * npts * args.steps / sec
+    us_per_launch = 1e3 * ms / (4 * args.steps)
+    achieved = BYTES_PER_GP_SUBSTEP * npts / (us_per_launch * 1e-6) / 1e9
+
+    # time to residual 1e-10 from the zero guess (the reference's every-step criterion)
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    r = S.solve()
+    # per-stage kernel durations (back-to-back launches, HIP events on the solver stream)
+    stage_us = {f"stage{k}": round(S.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)}
+
+    cpu = None if args.no_cpu_baseline else run_cpu_baseline_subprocess()
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "gp-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": value / PUBLISHED_GP_SUBSTEPS,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "equal_mass_boosted_128x128x16 SinhSymTP FP64 RK4 relaxation, zero initial guess, "
+                               "residual+L2 every step", "Nxx": list(cfg.Nxx), "global_batch": 1,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
+                     "kernel": "substep_kernel (fused RHS + radiation BC + RK4 stage update + ghost images)",
+                     "us_per_launch": us_per_launch, "bytes_per_launch": BYTES_PER_GP_SUBSTEP * npts,
+                     "stage_us": stage_us},
+        "cpu_baseline": cpu,
+        "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
+                             "log10_l2": r.log10_l2, "converged": bool(r.converged),
+                             "reference_steps": 3144, "check_interval_steps": 1},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -271,9 +271,14 @@ int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res);
 int nrpyell_solver_steps(nrpyell_solver *s, int n);
 /* Fused residual + L2 of the current y_n; returns log10 L2 in *log10_l2. */
 int nrpyell_solver_residual_l2(nrpyell_solver *s, double *log10_l2);
-/* Time n launches of the fused substep kernel for stage `stage` (1..4) with HIP events
- * on the solver's stream; returns the mean duration per launch in microseconds. */
+/* Time n launches of the fused substep kernel for stage `stage` (1..4; 0 = stage 1 with the
+ * fused residual reduction) with HIP events on the solver's stream; mean microseconds. */
 int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_per_launch);
+/* Exactly n RK4 steps of the production FP64 path (graph replay, residual reduced every
+ * step, no early stop), timed with HIP events on the solver's stream. */
+int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms, double *log10_l2);
+/* Residual history of the last solve: (steps completed, log10 L2) pairs; returns the count. */
+int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2);
 
 #ifdef __cplusplus
 }

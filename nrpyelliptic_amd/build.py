@@ -12,7 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnrpyell_hip.so")
-SOURCES = ["nrpyell_host.cpp", "nrpyell_kernels.hip", "nrpyell_capi.hip", "nrpyell_mlp.hip", "nrpyell_batch.hip"]
+SOURCES = ["nrpyell_host.cpp", "nrpyell_kernels.hip", "nrpyell_substep.hip", "nrpyell_capi.hip", "nrpyell_mlp.hip",
+           "nrpyell_batch.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -26,6 +27,7 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-Wl,-Bsymbolic",
+    "-Wl,--no-undefined",
 ]
 
 

@@ -486,6 +486,7 @@ struct nrpyell_solver {
   double volume = 0.0;
   double *partial = nullptr;
   double *l2_out = nullptr;
+  unsigned int *l2_count = nullptr;
   SolverDeviceState *dstate = nullptr;
   SolverDeviceState *hstate = nullptr;  // pinned
   // FP32 shadow state
@@ -498,7 +499,7 @@ struct nrpyell_solver {
     if (graph64) hipGraphExecDestroy(graph64);
     if (graph32) hipGraphExecDestroy(graph32);
     hipFree(coef64); hipFree(coef32); hipFree(geo64); hipFree(geo32); hipFree(w2d); hipFree(partial);
-    hipFree(l2_out); hipFree(dstate);
+    hipFree(l2_out); hipFree(l2_count); hipFree(dstate);
     if (hstate) hipHostFree(hstate);
     hipFree(y32); hipFree(ko32); hipFree(ke32); hipFree(rt32); hipFree(aux32);
     if (stream) hipStreamDestroy(stream);
@@ -529,15 +530,32 @@ struct nrpyell_solver {
     a.tx = tx;
     a.ntile0 = ntile0;
     a.ntile1 = ntile1;
+    a.w2d = nullptr;
+    a.l2_partial = nullptr;
+    a.l2_count = nullptr;
+    a.state = nullptr;
+    a.l2_volume = 0.0;
+    a.l2_target = 0.0;
     return a;
   }
 
-  void step64(hipStream_t st) {
+  // One FP64 RK4 step; with fuse_check the stage-1 kernel also reduces the residual L2 of
+  // y_n (before the step) into the convergence state.
+  void step64(hipStream_t st, bool fuse_check = false) {
     MoL_gridfunctions_struct &f = gd->gridfuncs;
-    for (int s = 1; s <= 4; s++)
-      launch_substep<double>(args<double>(s, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs,
-                                          f.auxevol_gfs, coef64, geo64),
-                             s, st);
+    for (int s = 1; s <= 4; s++) {
+      SubstepArgs<double> a = args<double>(s, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs,
+                                           f.auxevol_gfs, coef64, geo64);
+      if (s <= 2 && fuse_check) {  // stage 1 writes the partials, stage 2 reduces them
+        a.w2d = w2d;
+        a.l2_partial = partial;
+        a.l2_count = l2_count;
+        a.state = dstate;
+        a.l2_volume = volume;
+        a.l2_target = opt.log10_target;
+      }
+      launch_substep<double>(a, s, st);
+    }
   }
   void step32(hipStream_t st) {
     for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, aux32, coef32, geo32), s, st);
@@ -618,9 +636,12 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   std::vector<double> w(n2);
   host_l2_weights(&gd->params, gd->xx, s->opt.integration_radius, w.data(), &s->volume);
   s->w2d = upload(w.data(), w.size());
-  const long nblk = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
-  s->partial = dalloc<double>(nblk);
+  const long nblk_res = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
+  const long nblk_sub = (long)s->ntile0 * s->ntile1 * g.N2;
+  s->partial = dalloc<double>(nblk_res > nblk_sub ? nblk_res : nblk_sub);
   s->l2_out = dalloc<double>(1);
+  s->l2_count = dalloc<unsigned int>(1);
+  HIPCHK(hipMemset(s->l2_count, 0, sizeof(unsigned int)));
   s->dstate = dalloc<SolverDeviceState>(1);
   HIPCHK(hipHostMalloc((void **)&s->hstate, sizeof(SolverDeviceState), hipHostMallocDefault));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -652,8 +673,14 @@ static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double t
   hipGraph_t graph;
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 1; k <= nsteps; k++) {
-    if (fp32) s->step32(s->stream); else s->step64(s->stream);
-    if (k % s->opt.check_every == 0) s->check(s->stream, fp32, s->opt.check_every, target);
+    if (fp32) {
+      // FP32 bulk phase: the residual is evaluated in FP64 from the FP32 state every check_every steps
+      s->step32(s->stream);
+      if (k % s->opt.check_every == 0) s->check(s->stream, true, s->opt.check_every, target);
+    } else {
+      // FP64: every step's stage-1 kernel checks the residual of the state it starts from
+      s->step64(s->stream, true);
+    }
   }
   HIPCHK(hipStreamEndCapture(s->stream, &graph));
   hipGraphExec_t exec;
@@ -745,13 +772,70 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
   return NRPYELL_OK;
 }
 
+// Exactly n RK4 steps of the production FP64 path (graph replay, stage-1 residual check every
+// step against an unreachable target so nothing stops early), bracketed by HIP events on the
+// solver's stream. Returns the elapsed milliseconds and the final log10 L2.
+extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_out, double *log10_out) {
+  const double keep = s->opt.log10_target;
+  s->opt.log10_target = -1e300;
+  hipGraphExec_t g = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
+  s->opt.log10_target = keep;
+  const int per = s->opt.steps_per_graph;
+  hipGraphExec_t tail = nullptr;
+  if (n % per) {
+    s->opt.log10_target = -1e300;
+    tail = capture(s, false, n % per, s->opt.log10_target);
+    s->opt.log10_target = keep;
+  }
+  reset_state(s);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s->stream));
+  for (int k = 0; k < n / per; k++) HIPCHK(hipGraphLaunch(g, s->stream));
+  if (tail) HIPCHK(hipGraphLaunch(tail, s->stream));
+  HIPCHK(hipEventRecord(e1, s->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  read_state(s);
+  *ms_out = ms;
+  if (log10_out) *log10_out = s->hstate->last_log10;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  HIPCHK(hipGraphExecDestroy(g));
+  if (tail) HIPCHK(hipGraphExecDestroy(tail));
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2) {
+  read_state(s);
+  const int n = s->hstate->nhist < max ? s->hstate->nhist : max;
+  for (int k = 0; k < n; k++) {
+    steps[k] = s->hstate->hist_step[k];
+    log10_l2[k] = s->hstate->hist_log10[k];
+  }
+  return n;
+}
+
 extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_per_launch) {
   MoL_gridfunctions_struct &f = s->gd->gridfuncs;
   reset_state(s);
+  // stage 0 = stage 1 with the fused residual-L2 check (the form the solver runs)
+  const bool fuse = stage == 0;
+  if (fuse) stage = 1;
   // the kernels write y_n only in stage 4; time on scratch copies so the state is untouched
-  const SubstepArgs<double> a =
+  SubstepArgs<double> a =
       s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, f.auxevol_gfs,
                       s->coef64, s->geo64);
+  if (fuse) {
+    a.w2d = s->w2d;
+    a.l2_partial = s->partial;
+    a.l2_count = s->l2_count;
+    a.state = s->dstate;
+    a.l2_volume = s->volume;
+    a.l2_target = -1e300;  // never "converges": every launch does the full work
+  }
   SubstepArgs<double> b = a;
   double *tmp = nullptr;
   if (stage == 4) {

@@ -95,3 +95,14 @@ __host__ __device__ inline void for_each_image(const GridDims &g, int i0, int i1
 #define FD2_W3 (5.0 / 126.0)
 #define FD2_W4 (5.0 / 1008.0)
 #define FD2_W5 (1.0 / 3150.0)
+
+// 11-point first derivative with the stencil shifted by -s (s = 0..5; s = 0 centred); the
+// Fornberg rationals of apply_bcs_outerradiation_and_inner.c:11-141. Shift +s uses
+// w[j] = -c_fd1_shift[s][10-j].
+static __constant__ double c_fd1_shift[6][11] = {
+    {-1.0 / 1260, 5.0 / 504, -5.0 / 84, 5.0 / 21, -5.0 / 6, 0.0, 5.0 / 6, -5.0 / 21, 5.0 / 84, -5.0 / 504, 1.0 / 1260},
+    {1.0 / 1260, -1.0 / 105, 3.0 / 56, -4.0 / 21, 1.0 / 2, -6.0 / 5, 11.0 / 30, 4.0 / 7, -3.0 / 28, 1.0 / 63, -1.0 / 840},
+    {-1.0 / 840, 1.0 / 72, -3.0 / 40, 1.0 / 4, -7.0 / 12, 21.0 / 20, -7.0 / 4, 319.0 / 420, 3.0 / 8, -1.0 / 24, 1.0 / 360},
+    {1.0 / 360, -2.0 / 63, 1.0 / 6, -8.0 / 15, 7.0 / 6, -28.0 / 15, 7.0 / 3, -8.0 / 3, 341.0 / 280, 2.0 / 9, -1.0 / 90},
+    {-1.0 / 90, 1.0 / 8, -9.0 / 14, 2.0, -21.0 / 5, 63.0 / 10, -7.0, 6.0, -9.0 / 2, 4609.0 / 2520, 1.0 / 10},
+    {1.0 / 10, -10.0 / 9, 45.0 / 8, -120.0 / 7, 35.0, -252.0 / 5, 105.0 / 2, -40.0, 45.0 / 2, -10.0, 7381.0 / 2520}};

@@ -4,7 +4,7 @@
 
 #include <string>
 
-#include "../../include/nrpyelliptic_amd.h"
+#include "nrpyelliptic_amd.h"
 #include "nrpyell_common.h"
 
 #define NRPYELL_HIST 4096
@@ -12,9 +12,9 @@
 // Device-resident convergence state of one solve.
 struct SolverDeviceState {
   int done;          // 0 running, 1 converged, 2 NaN
-  int steps;         // RK4 steps accounted by residual checks
+  int steps;         // RK4 steps completed at the last residual check
   int nhist;
-  int pad;
+  int nchecks;       // residual checks made by fused stage-1 kernels
   double last_log10;
   int hist_step[NRPYELL_HIST];
   double hist_log10[NRPYELL_HIST];
@@ -51,6 +51,13 @@ struct SubstepArgs {
   const int *done;  // nonzero: converged, every launch returns at once
   int tx;           // tile width along i0 (threads), tile height along i1 = blockDim/tx
   int ntile0, ntile1;
+  // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
+  // is c^2 times the residual. Non-null l2_partial enables it.
+  const double *w2d;          // L2 weights per (i0,i1): dV inside the radius, 0 outside
+  double *l2_partial;         // one partial sum per block
+  unsigned int *l2_count;     // blocks finished; the last block reduces and resets it
+  SolverDeviceState *state;   // convergence state updated by the last block
+  double l2_volume, l2_target;
 };
 
 

@@ -164,203 +164,6 @@ void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float
 // ---------------------------------------------------------------------------------------
 // Fused RK4 substep.
 // ---------------------------------------------------------------------------------------
-template <typename T>
-__device__ inline T fast_lap_source(const T *__restrict__ u, long p, long st1, long st2, const T K[5], T psi, T AA,
-                                    T &uu_out) {
-  T m0[6], p0[6], m1[6], p1[6], m2[6], p2[6];
-#pragma unroll
-  for (int k = 1; k <= 5; k++) {
-    m0[k] = u[p - k]; p0[k] = u[p + k];
-    m1[k] = u[p - k * st1]; p1[k] = u[p + k * st1];
-    m2[k] = u[p - k * st2]; p2[k] = u[p + k * st2];
-  }
-  const T uu = u[p];
-  uu_out = uu;
-  const T w0 = (T)FD2_W0;
-  T s00 = -w0 * uu, s11 = -w0 * uu, s22 = -w0 * uu;
-  s00 = fma((T)FD2_W1, m0[1] + p0[1], s00); s11 = fma((T)FD2_W1, m1[1] + p1[1], s11); s22 = fma((T)FD2_W1, m2[1] + p2[1], s22);
-  s00 = fma((T)-FD2_W2, m0[2] + p0[2], s00); s11 = fma((T)-FD2_W2, m1[2] + p1[2], s11); s22 = fma((T)-FD2_W2, m2[2] + p2[2], s22);
-  s00 = fma((T)FD2_W3, m0[3] + p0[3], s00); s11 = fma((T)FD2_W3, m1[3] + p1[3], s11); s22 = fma((T)FD2_W3, m2[3] + p2[3], s22);
-  s00 = fma((T)-FD2_W4, m0[4] + p0[4], s00); s11 = fma((T)-FD2_W4, m1[4] + p1[4], s11); s22 = fma((T)-FD2_W4, m2[4] + p2[4], s22);
-  s00 = fma((T)FD2_W5, m0[5] + p0[5], s00); s11 = fma((T)FD2_W5, m1[5] + p1[5], s11); s22 = fma((T)FD2_W5, m2[5] + p2[5], s22);
-  T s0 = (T)FD1_W1 * (p0[1] - m0[1]), s1 = (T)FD1_W1 * (p1[1] - m1[1]);
-  s0 = fma((T)-FD1_W2, p0[2] - m0[2], s0); s1 = fma((T)-FD1_W2, p1[2] - m1[2], s1);
-  s0 = fma((T)FD1_W3, p0[3] - m0[3], s0); s1 = fma((T)FD1_W3, p1[3] - m1[3], s1);
-  s0 = fma((T)-FD1_W4, p0[4] - m0[4], s0); s1 = fma((T)-FD1_W4, p1[4] - m1[4], s1);
-  s0 = fma((T)FD1_W5, p0[5] - m0[5], s0); s1 = fma((T)FD1_W5, p1[5] - m1[5], s1);
-  const T X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
-  T e = (T)0.125 * (AA / X7);
-  e = fma(K[0], s00, e);
-  e = fma(K[1], s11, e);
-  e = fma(K[2], s22, e);
-  e = fma(K[3], s1, e);
-  e = fma(K[4], s0, e);
-  return e;
-}
-
-// Stage updates, MoL_step_forward_in_time.c:44-45 (k1), :88-89 (k2), :129-130 (k3), :170 (k4).
-template <typename T, int STAGE>
-__device__ inline void stage_update(T k, T dt, T yn, T &rt, T &out) {
-  if (STAGE == 1) {
-    rt = (T)(1.0 / 6.0) * (dt * k);
-    out = fma((T)0.5, dt * k, yn);
-  } else if (STAGE == 2) {
-    rt = fma((T)(1.0 / 3.0), dt * k, rt);
-    out = fma((T)0.5, dt * k, yn);
-  } else if (STAGE == 3) {
-    rt = fma((T)(1.0 / 3.0), dt * k, rt);
-    out = fma(dt, k, yn);
-  } else {
-    out = yn + fma((T)(1.0 / 6.0), dt * k, rt);
-  }
-}
-
-// 11-point first derivative with the stencil shifted `off` points (-5..-1, or 0 centred);
-// weights are the Fornberg rationals of apply_bcs_outerradiation_and_inner.c:11-141.
-__constant__ double c_fd1_shift[6][11] = {
-    {-1.0 / 1260, 5.0 / 504, -5.0 / 84, 5.0 / 21, -5.0 / 6, 0.0, 5.0 / 6, -5.0 / 21, 5.0 / 84, -5.0 / 504, 1.0 / 1260},
-    {1.0 / 1260, -1.0 / 105, 3.0 / 56, -4.0 / 21, 1.0 / 2, -6.0 / 5, 11.0 / 30, 4.0 / 7, -3.0 / 28, 1.0 / 63, -1.0 / 840},
-    {-1.0 / 840, 1.0 / 72, -3.0 / 40, 1.0 / 4, -7.0 / 12, 21.0 / 20, -7.0 / 4, 319.0 / 420, 3.0 / 8, -1.0 / 24, 1.0 / 360},
-    {1.0 / 360, -2.0 / 63, 1.0 / 6, -8.0 / 15, 7.0 / 6, -28.0 / 15, 7.0 / 3, -8.0 / 3, 341.0 / 280, 2.0 / 9, -1.0 / 90},
-    {-1.0 / 90, 1.0 / 8, -9.0 / 14, 2.0, -21.0 / 5, 63.0 / 10, -7.0, 6.0, -9.0 / 2, 4609.0 / 2520, 1.0 / 10},
-    {1.0 / 10, -10.0 / 9, 45.0 / 8, -120.0 / 7, 35.0, -252.0 / 5, 105.0 / 2, -40.0, 45.0 / 2, -10.0, 7381.0 / 2520}};
-
-
-template <typename T, int STAGE>
-__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
-  if (a.done && *a.done) return;
-  const GridDims &g = a.g;
-  const int tid = threadIdx.x;
-  const int ty_n = blockDim.x / a.tx;
-  const int tx = tid % a.tx, ty = tid / a.tx;
-  const int tile = blockIdx.x;
-  const int t0 = tile % a.ntile0, t1 = tile / a.ntile0;
-  const int i2 = NG + blockIdx.y;
-  const int i0 = NG + t0 * a.tx + tx;
-  const int i1 = NG + t1 * ty_n + ty;
-  const bool valid = (i0 < g.N0 + NG) && (i1 < g.N1 + NG);
-  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1, n2 = (long)g.G0 * g.G1;
-  const int ilast = g.N0 + NG - 1;  // last interior i0
-  const bool has_bc = (NG + t0 * a.tx <= ilast) && (ilast < NG + (t0 + 1) * a.tx);
-
-  __shared__ T sh_rhs[16][2][6];  // [row][gf][pt]: pt 0 = last interior, 1..5 = outer layers
-  __shared__ T sh_dr[16][2][6];
-
-  if (valid) {
-    const long p = idx3(g, i0, i1, i2);
-    const long k2 = i0 + (long)g.G0 * i1;
-    T K[5];
-#pragma unroll
-    for (int k = 0; k < 5; k++) K[k] = a.coef[k * n2 + k2];
-    const T psi = a.aux[PSI_BACKGROUNDGF * nt + p], AA = a.aux[ADD_TIMES_AUUGF * nt + p];
-    const T c = a.aux[WAVESPEEDGF * nt + p];
-    T uu;
-    const T L = fast_lap_source<T>(a.in, p, s1, s2, K, psi, AA, uu);
-    const T vv = a.in[nt + p];
-    const T ku = fma(-a.eta, uu, vv);  // rhs_eval.c:109 (NegFusedMulAdd)
-    const T kv = (c * c) * L;
-    if (has_bc && i0 == ilast) {
-      sh_rhs[ty][0][0] = ku;
-      sh_rhs[ty][1][0] = kv;
-    }
-    T rt_u = (STAGE == 1) ? (T)0 : a.rt[p], rt_v = (STAGE == 1) ? (T)0 : a.rt[nt + p];
-    T o_u, o_v;
-    stage_update<T, STAGE>(ku, a.dt, a.yn[p], rt_u, o_u);
-    stage_update<T, STAGE>(kv, a.dt, a.yn[nt + p], rt_v, o_v);
-    if (STAGE != 4) {
-      a.rt[p] = rt_u;
-      a.rt[nt + p] = rt_v;
-    }
-    a.out[p] = o_u;
-    a.out[nt + p] = o_v;
-    for_each_image(g, i0, i1, i2, [&](long d) {
-      a.out[d] = o_u;
-      a.out[nt + d] = o_v;
-    });
-  }
-  if (!has_bc) return;  // uniform per block
-
-  // ---- radiation BC on the x0-max face for this tile's rows (apply_bcs_outerradiation_and_inner.c:369-408)
-  // Phase B: d_r f at the last interior point (pt 0) and the 5 outer layers (pt 1..5).
-  const int rows = ty_n;
-  {
-    const int nwork = rows * 12;
-    for (int w = tid; w < nwork; w += blockDim.x) {
-      const int row = w / 12, rem = w % 12, gf = rem / 6, pt = rem % 6;
-      const int j1 = NG + t1 * ty_n + row;
-      if (j1 >= g.N1 + NG) continue;
-      const int j0 = ilast + pt;
-      const int off = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
-      const T *f = a.in + gf * nt;
-      const long q = idx3(g, j0, j1, i2);
-      T d0 = 0, d1 = 0;
-#pragma unroll
-      for (int j = 0; j < 11; j++) {
-        d0 = fma((T)c_fd1_shift[off][j], f[q + (j - 5 - off)], d0);
-        d1 = fma((T)c_fd1_shift[0][j], f[q + (long)(j - 5) * s1], d1);
-      }
-      d0 *= (T)g.invdx0;
-      d1 *= (T)g.invdx1;
-      const T *geo = a.geo + ((long)pt * g.G1 + j1) * 3;
-      sh_dr[row][gf][pt] = geo[1] * d0 + geo[2] * d1;
-    }
-  }
-  __syncthreads();
-  // Phase C: the innermost-first layer recurrence (serial in the layer, parallel over rows x gf).
-  if (tid < rows * 2) {
-    const int row = tid / 2, gf = tid % 2;
-    const int j1 = NG + t1 * ty_n + row;
-    if (j1 < g.N1 + NG) {
-      const T c = gf == 0 ? a.cOB_u : a.cOB_v;
-      const T *f = a.in + gf * nt;
-      T rhs_prev = sh_rhs[row][gf][0];
-      for (int pt = 1; pt <= 5; pt++) {
-        const T *gq = a.geo + ((long)(pt - 1) * g.G1 + j1) * 3;
-        const T *gp = a.geo + ((long)pt * g.G1 + j1) * 3;
-        const T fq = f[idx3(g, ilast + pt - 1, j1, i2)], fp = f[idx3(g, ilast + pt, j1, i2)];
-        const T rq = gq[0], rp = gp[0];
-        const T out_int = -c * (sh_dr[row][gf][pt - 1] + fq / rq);
-        const T kk = rq * rq * rq * (rhs_prev - out_int);
-        const T rinv = (T)1 / rp;
-        const T r = -c * (sh_dr[row][gf][pt] + fp * rinv) + kk * rinv * rinv * rinv;
-        sh_rhs[row][gf][pt] = r;
-        rhs_prev = r;
-      }
-    }
-  }
-  __syncthreads();
-  // Phase D: stage update at the outer points and their parity images.
-  {
-    const int nwork = rows * 10;
-    for (int w = tid; w < nwork; w += blockDim.x) {
-      const int row = w / 10, rem = w % 10, gf = rem / 5, pt = rem % 5 + 1;
-      const int j1 = NG + t1 * ty_n + row;
-      if (j1 >= g.N1 + NG) continue;
-      const int j0 = ilast + pt;
-      const long p = idx3(g, j0, j1, i2) + gf * nt;
-      T rt_ = (STAGE == 1) ? (T)0 : a.rt[p];
-      T o;
-      stage_update<T, STAGE>(sh_rhs[row][gf][pt], a.dt, a.yn[p], rt_, o);
-      if (STAGE != 4) a.rt[p] = rt_;
-      a.out[p] = o;
-      T *outg = a.out + gf * nt;
-      for_each_image(g, j0, j1, i2, [&](long d) { outg[d] = o; });
-    }
-  }
-}
-
-template <typename T>
-void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st) {
-  dim3 grd(a.ntile0 * a.ntile1, a.g.N2), blk(256);
-  switch (stage) {
-    case 1: hipLaunchKernelGGL((substep_kernel<T, 1>), grd, blk, 0, st, a); break;
-    case 2: hipLaunchKernelGGL((substep_kernel<T, 2>), grd, blk, 0, st, a); break;
-    case 3: hipLaunchKernelGGL((substep_kernel<T, 3>), grd, blk, 0, st, a); break;
-    default: hipLaunchKernelGGL((substep_kernel<T, 4>), grd, blk, 0, st, a); break;
-  }
-}
-template void launch_substep<double>(const SubstepArgs<double> &, int, hipStream_t);
-template void launch_substep<float>(const SubstepArgs<float> &, int, hipStream_t);
 
 // ---------------------------------------------------------------------------------------
 // Fused residual + L2 (residual_all_points.c + L2_norm_of_gf.c): per point

@@ -1,0 +1,306 @@
+// nrpyell_substep.hip -- the fused RK4 substep kernel (the hot path).
+//
+// One launch does, for one RK4 substep (MoL_step_forward_in_time.c:11-176):
+//   1. RHS at every interior point: 10th-order star stencil of u (rhs_eval.c:86-110) with the
+//      precombined (i0,i1) metric coefficients, v - eta u, and c^2 (lap u + AA^2/(8 (psi+u)^7));
+//   2. the radiation condition on the 5 outer ghost layers of the x0-max face, innermost first
+//      (apply_bcs_outerradiation_and_inner.c:369-408), done by the blocks that own the last
+//      interior i0 of each (i1,i2) column, entirely from LDS;
+//   3. the stage update (RK4 k_odd/k_even scheme) of every interior and outer point;
+//   4. the inner ghost zones: each updated point writes its own parity images (the closed form
+//      of the bcstruct map, verified against the table on the host), which is the inner-BC
+//      gather of apply_bcs_inner_only.c:18-26 applied to the stage output (parity +1 commutes
+//      with the linear stage update, so the values are those of the reference ordering);
+//   5. (stage 1, FP64) the L2 of the residual of the stage input y_n: the stencil of step 1 is
+//      c^2 times the residual, so a block reduction and a last-block finalize give the
+//      reference's every-step convergence check (main.c:146-151) for free.
+//
+// Tiles are TX (i0, across lanes) x TY (i1) points of one i2 plane, 256 threads; the u plane
+// with its i0/i1 halos is staged in LDS, the i2 arm is read from global memory (coalesced).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "nrpyell_common.h"
+#include "nrpyell_internal.h"
+
+#define NG NRPYELL_NG
+
+// Stage updates, MoL_step_forward_in_time.c:44-45 (k1), :88-89 (k2), :129-130 (k3), :170 (k4).
+template <typename T, int STAGE>
+__device__ __forceinline__ void stage_update(T k, T dt, T yn, T &rt, T &out) {
+  if (STAGE == 1) {
+    rt = (T)(1.0 / 6.0) * (dt * k);
+    out = fma((T)0.5, dt * k, yn);
+  } else if (STAGE == 2) {
+    rt = fma((T)(1.0 / 3.0), dt * k, rt);
+    out = fma((T)0.5, dt * k, yn);
+  } else if (STAGE == 3) {
+    rt = fma((T)(1.0 / 3.0), dt * k, rt);
+    out = fma(dt, k, yn);
+  } else {
+    out = yn + fma((T)(1.0 / 6.0), dt * k, rt);
+  }
+}
+
+// Raw 10th-order second-difference and first-difference sums of one direction
+// (weights rhs_eval.c:23-27; 1/dx factors are in the coefficient table).
+template <typename T>
+__device__ __forceinline__ void fd_sums(const T m[6], const T p[6], T uu, T &s2, T &s1) {
+  s2 = -(T)FD2_W0 * uu;
+  s2 = fma((T)FD2_W1, m[1] + p[1], s2);
+  s2 = fma((T)-FD2_W2, m[2] + p[2], s2);
+  s2 = fma((T)FD2_W3, m[3] + p[3], s2);
+  s2 = fma((T)-FD2_W4, m[4] + p[4], s2);
+  s2 = fma((T)FD2_W5, m[5] + p[5], s2);
+  s1 = (T)FD1_W1 * (p[1] - m[1]);
+  s1 = fma((T)-FD1_W2, p[2] - m[2], s1);
+  s1 = fma((T)FD1_W3, p[3] - m[3], s1);
+  s1 = fma((T)-FD1_W4, p[4] - m[4], s1);
+  s1 = fma((T)FD1_W5, p[5] - m[5], s1);
+}
+
+template <typename T, int STAGE, int TX>
+__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
+  constexpr int TY = 256 / TX;
+  constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG;
+  __shared__ T su[BY * BX];  // u on this i2 plane: tile + halos
+  __shared__ T sv[BY * BX];  // v likewise (BC tiles only)
+  __shared__ T s_rhs[TY][2][6], s_dr[TY][2][6], s_f[TY][2][6];
+  __shared__ T s_geo[6][TY][3];
+  __shared__ double s_red[4];
+
+  const GridDims &g = a.g;
+  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+  const int t0 = blockIdx.x % a.ntile0, t1 = blockIdx.x / a.ntile0;
+  const int i2 = NG + blockIdx.y;
+  const int i0s = NG + t0 * TX, i1s = NG + t1 * TY;
+  const int i0 = i0s + tx, i1 = i1s + ty;
+  const bool valid = i0 < g.N0 + NG && i1 < g.N1 + NG;
+  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
+  const long plane = (long)i2 * s2;
+  const int ilast = g.N0 + NG - 1;  // last interior i0; the radiation BC starts next to it
+  const bool has_bc = (i0s <= ilast) && (ilast < i0s + TX);
+  const bool fuse_l2 = (STAGE == 1) && (a.l2_partial != nullptr);
+  const int done = a.done ? *a.done : 0;
+
+  // ---- global loads: LDS staging of the plane, the i2 arm and the pointwise inputs
+  T stage_u[(BX * BY + 255) / 256], stage_v[(BX * BY + 255) / 256];
+#pragma unroll
+  for (int r = 0; r < (BX * BY + 255) / 256; r++) {
+    const int k = tid + 256 * r;
+    const int bx = k % BX, by = k / BX;
+    const int j0 = i0s - NG + bx, j1 = i1s - NG + by;
+    const bool corner = (bx < NG || bx >= TX + NG) && (by < NG || by >= TY + NG);
+    stage_u[r] = 0;
+    stage_v[r] = 0;
+    if (k < BX * BY && j0 < g.G0 && j1 < g.G1 && (!corner || has_bc)) {
+      const long q = plane + (long)j1 * s1 + j0;
+      stage_u[r] = a.in[q];
+      if (has_bc) stage_v[r] = a.in[nt + q];
+    }
+  }
+  const long p = valid ? plane + (long)i1 * s1 + i0 : 0;
+  const long k2 = valid ? (long)i1 * s1 + i0 : 0;
+  T zm[6], zp[6];
+  T vv = 0, psi = 1, AA = 0, c = 0, yn_u = 0, yn_v = 0, rt_u = 0, rt_v = 0, K[5] = {0, 0, 0, 0, 0};
+  double w2 = 0.0;
+  if (valid) {
+#pragma unroll
+    for (int k = 1; k <= 5; k++) {
+      zm[k] = a.in[p - k * s2];
+      zp[k] = a.in[p + k * s2];
+    }
+    vv = a.in[nt + p];
+    AA = a.aux[ADD_TIMES_AUUGF * nt + p];
+    psi = a.aux[PSI_BACKGROUNDGF * nt + p];
+    c = a.aux[WAVESPEEDGF * nt + p];
+#pragma unroll
+    for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
+    yn_u = a.yn[p];
+    yn_v = a.yn[nt + p];
+    if (STAGE != 1) {
+      rt_u = a.rt[p];
+      rt_v = a.rt[nt + p];
+    }
+    if (fuse_l2) w2 = a.w2d[k2];
+  }
+  // outer-layer state for the BC stage update (work item = (row, gf, layer 1..5))
+  T od_yn = 0, od_rt = 0;
+  long od_p = -1;
+  if (has_bc && tid < 10 * TY) {
+    const int row = tid / 10, rem = tid % 10, gf = rem / 5, pt = rem % 5 + 1;
+    const int j1 = i1s + row;
+    if (j1 < g.N1 + NG) {
+      od_p = plane + (long)j1 * s1 + (ilast + pt) + gf * nt;
+      od_yn = a.yn[od_p];
+      if (STAGE != 1) od_rt = a.rt[od_p];
+    }
+  }
+  if (has_bc)
+    for (int k = tid; k < 18 * TY; k += 256) {
+      const int pt = k / (3 * TY), row = (k / 3) % TY, comp = k % 3;
+      const int j1 = i1s + row;
+      s_geo[pt][row][comp] = j1 < g.G1 ? a.geo[((long)pt * g.G1 + j1) * 3 + comp] : (T)1;
+    }
+  if (done) return;  // converged: uniform across the grid
+#pragma unroll
+  for (int r = 0; r < (BX * BY + 255) / 256; r++) {
+    const int k = tid + 256 * r;
+    if (k < BX * BY) {
+      su[k] = stage_u[r];
+      if (has_bc) sv[k] = stage_v[r];
+    }
+  }
+  __syncthreads();
+
+  // ---- 1+3+4: interior RHS, stage update, parity images
+  double l2acc = 0.0;
+  if (valid) {
+    const int cb = (ty + NG) * BX + tx + NG;
+    const T uu = su[cb];
+    T xm[6], xp[6], ym[6], yp[6];
+#pragma unroll
+    for (int k = 1; k <= 5; k++) {
+      xm[k] = su[cb - k]; xp[k] = su[cb + k];
+      ym[k] = su[cb - k * BX]; yp[k] = su[cb + k * BX];
+    }
+    T s00, s0, s11, s1d, s22, s2d;
+    fd_sums<T>(xm, xp, uu, s00, s0);
+    fd_sums<T>(ym, yp, uu, s11, s1d);
+    fd_sums<T>(zm, zp, uu, s22, s2d);
+    const T X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
+    T L = (T)0.125 * (AA / X7);
+    L = fma(K[0], s00, L);
+    L = fma(K[1], s11, L);
+    L = fma(K[2], s22, L);
+    L = fma(K[3], s1d, L);
+    L = fma(K[4], s0, L);
+    const T ku = fma(-a.eta, uu, vv);  // rhs_eval.c:109
+    const T kv = (c * c) * L;          // rhs_eval.c:110
+    if (fuse_l2) l2acc = w2 * (double)L * (double)L;
+    if (has_bc && i0 == ilast) {
+      s_rhs[ty][0][0] = ku;
+      s_rhs[ty][1][0] = kv;
+    }
+    T o_u, o_v;
+    stage_update<T, STAGE>(ku, a.dt, yn_u, rt_u, o_u);
+    stage_update<T, STAGE>(kv, a.dt, yn_v, rt_v, o_v);
+    if (STAGE != 4) {
+      a.rt[p] = rt_u;
+      a.rt[nt + p] = rt_v;
+    }
+    a.out[p] = o_u;
+    a.out[nt + p] = o_v;
+    for_each_image(g, i0, i1, i2, [&](long d) {
+      a.out[d] = o_u;
+      a.out[nt + d] = o_v;
+    });
+  }
+
+  // ---- 2: radiation BC on the outer layers of this tile's rows
+  if (has_bc) {
+    const int bx0 = ilast - (i0s - NG);  // box column of the last interior point
+    if (tid < 12 * TY) {
+      const int row = tid / 12, rem = tid % 12, gf = rem / 6, pt = rem % 6;
+      if (i1s + row < g.N1 + NG) {
+        const T *box = gf == 0 ? su : sv;
+        const int bx = bx0 + pt, by = row + NG, sh = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
+        T d0 = 0, d1 = 0;
+#pragma unroll
+        for (int j = 0; j < 11; j++) {
+          d0 = fma((T)c_fd1_shift[sh][j], box[by * BX + bx + (j - 5 - sh)], d0);
+          d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * BX + bx], d1);
+        }
+        d0 *= (T)g.invdx0;
+        d1 *= (T)g.invdx1;
+        s_dr[row][gf][pt] = s_geo[pt][row][1] * d0 + s_geo[pt][row][2] * d1;
+        s_f[row][gf][pt] = box[by * BX + bx];
+      }
+    }
+    __syncthreads();
+    if (tid < 2 * TY) {
+      const int row = tid / 2, gf = tid % 2;
+      if (i1s + row < g.N1 + NG) {
+        const T cc = gf == 0 ? a.cOB_u : a.cOB_v;
+        T prev = s_rhs[row][gf][0];
+#pragma unroll
+        for (int pt = 1; pt <= 5; pt++) {
+          const T rq = s_geo[pt - 1][row][0], rp = s_geo[pt][row][0];
+          const T out_int = -cc * (s_dr[row][gf][pt - 1] + s_f[row][gf][pt - 1] / rq);
+          const T kk = rq * rq * rq * (prev - out_int);
+          const T rinv = (T)1 / rp;
+          prev = -cc * (s_dr[row][gf][pt] + s_f[row][gf][pt] * rinv) + kk * rinv * rinv * rinv;
+          s_rhs[row][gf][pt] = prev;
+        }
+      }
+    }
+    __syncthreads();
+    if (od_p >= 0) {
+      const int row = tid / 10, rem = tid % 10, gf = rem / 5, pt = rem % 5 + 1;
+      T o;
+      T rt_ = od_rt;
+      stage_update<T, STAGE>(s_rhs[row][gf][pt], a.dt, od_yn, rt_, o);
+      if (STAGE != 4) a.rt[od_p] = rt_;
+      a.out[od_p] = o;
+      T *outg = a.out + gf * nt;
+      for_each_image(g, ilast + pt, i1s + row, i2, [&](long d) { outg[d] = o; });
+    }
+  }
+
+  // ---- 5: residual L2 of the stage-1 input: one partial per block (plain store); the
+  // stage-2 launch that follows reduces them in block order, after the kernel boundary has
+  // made them visible (no atomics, no L2 flushes inside the kernel).
+  if (fuse_l2) {
+    for (int o = 32; o > 0; o >>= 1) l2acc += __shfl_down(l2acc, o, 64);
+    if ((tid & 63) == 0) s_red[tid >> 6] = l2acc;
+    __syncthreads();
+    if (tid == 0) a.l2_partial[blockIdx.x + gridDim.x * blockIdx.y] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+  }
+  if (STAGE == 2 && a.l2_partial != nullptr && blockIdx.x == 0 && blockIdx.y == 0) {
+    const unsigned nblk = gridDim.x * gridDim.y;
+    double acc = 0.0;
+    for (unsigned i = tid; i < nblk; i += 256) acc += a.l2_partial[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) s_red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      const double S = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+      const double l = log10(1e-16 + sqrt(S / a.l2_volume));
+      SolverDeviceState *st = a.state;
+      st->steps = st->nchecks;  // completed steps when stage 1 read y_n
+      st->nchecks++;
+      st->last_log10 = l;
+      if (st->nhist < NRPYELL_HIST) {
+        st->hist_step[st->nhist] = st->steps;
+        st->hist_log10[st->nhist] = l;
+        st->nhist++;
+      }
+      // stages 3 and 4 of this step then skip, leaving y_n as it was after st->steps steps
+      if (!(l == l)) st->done = 2;
+      else if (l <= a.l2_target) st->done = 1;
+    }
+  }
+}
+
+template <typename T, int STAGE>
+static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
+  dim3 grd(a.ntile0 * a.ntile1, a.g.N2), blk(256);
+  switch (a.tx) {
+    case 16: hipLaunchKernelGGL((substep_kernel<T, STAGE, 16>), grd, blk, 0, st, a); break;
+    case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((substep_kernel<T, STAGE, 32>), grd, blk, 0, st, a); break;
+  }
+}
+
+template <typename T>
+void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st) {
+  switch (stage) {
+    case 1: launch_stage<T, 1>(a, st); break;
+    case 2: launch_stage<T, 2>(a, st); break;
+    case 3: launch_stage<T, 3>(a, st); break;
+    default: launch_stage<T, 4>(a, st); break;
+  }
+}
+template void launch_substep<double>(const SubstepArgs<double> &, int, hipStream_t);
+template void launch_substep<float>(const SubstepArgs<float> &, int, hipStream_t);

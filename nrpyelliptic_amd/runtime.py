@@ -19,7 +19,7 @@ from . import abi
 from .config import PunctureConfig
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnrpyell_hip.so")
+LIB_PATH = os.environ.get("NRPYELL_LIB", os.path.join(HERE, "libnrpyell_hip.so"))
 
 NRPYELL_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "EGHOSTMAP", -4: "ENOTCONVERGED", -5: "ENAN"}
@@ -73,6 +73,8 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_steps.argtypes = [vp, C.c_int]
     L.nrpyell_solver_residual_l2.argtypes = [vp, C.POINTER(C.c_double)]
     L.nrpyell_solver_time_substep.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    L.nrpyell_solver_timed_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     _LIB = L
     return L
 
@@ -301,6 +303,18 @@ class Solver:
         v = C.c_double()
         check(lib().nrpyell_solver_residual_l2(self.h, C.byref(v)), "residual_l2")
         return v.value
+
+    def timed_steps(self, n: int):
+        """Exactly n production RK4 steps timed with HIP events: (milliseconds, final log10 L2)."""
+        ms, l2 = C.c_double(), C.c_double()
+        check(lib().nrpyell_solver_timed_steps(self.h, n, C.byref(ms), C.byref(l2)), "timed_steps")
+        return ms.value, l2.value
+
+    def history(self, max_entries: int = 4096):
+        steps = (C.c_int * max_entries)()
+        vals = (C.c_double * max_entries)()
+        n = lib().nrpyell_solver_history(self.h, max_entries, steps, vals)
+        return np.array(steps[:n]), np.array(vals[:n])
 
     def time_substep(self, stage: int, n: int) -> float:
         v = C.c_double()
