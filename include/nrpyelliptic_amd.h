@@ -210,6 +210,8 @@ void gridfunction_z_axis(const paramstruct *NRPYELL_RESTRICT params, const int g
                          const REAL *NRPYELL_RESTRICT in_gf, FILE *NRPYELL_RESTRICT outfile);
 void gridfunction_xz_plane(const paramstruct *NRPYELL_RESTRICT params, const int gf_index, REAL *NRPYELL_RESTRICT xx[3],
                            const REAL *NRPYELL_RESTRICT in_gf, FILE *NRPYELL_RESTRICT outfile);
+/* :33 / print_puncture_parameters.c:6 */
+void print_puncture_parameters(const paramstruct *NRPYELL_RESTRICT params);
 
 /* ---- Part 2: this library's API (no reference counterpart) ----------------------------- */
 
@@ -279,6 +281,48 @@ int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_
 int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms, double *log10_l2);
 /* Residual history of the last solve: (steps completed, log10 L2) pairs; returns the count. */
 int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2);
+
+/* Batched FP64 solver: n configurations, each with its own griddata (any grid shape), relaxed
+ * together -- one launch per RK4 substep per tile width covers every configuration, and each
+ * configuration stops on its own residual. tile_i0[k] (16/32/64, or 0 for the default) is the
+ * tile width chosen for configuration k (by the autotuner). */
+typedef struct nrpyell_batch nrpyell_batch;
+int nrpyell_batch_create(griddata_struct **griddatas, const double *dts, const int *tile_i0, int n,
+                         const nrpyell_solve_options *opt, nrpyell_batch **out);
+/* Relax every configuration to opt->log10_target; res has n entries. */
+int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, double *seconds);
+int nrpyell_batch_destroy(nrpyell_batch *b);
+
+/* Learned initial guess: coordinate MLP (64 padded input features -> 5 x [Linear 256 + SiLU]
+ * -> Linear 1) on BF16 MFMA kernels with FP32 accumulation; FP32 master weights and Adam.
+ * Parameter layout (FP32): for l = 1..5 W_l [256 x K_l] (K_1 = 64, else 256), b_l [256];
+ * then w_out [256], b_out [1]. Row counts must be multiples of 64. */
+typedef struct nrpyell_mlp nrpyell_mlp;
+int nrpyell_mlp_create(unsigned long long seed, nrpyell_mlp **out);
+int nrpyell_mlp_destroy(nrpyell_mlp *m);
+long nrpyell_mlp_num_params(nrpyell_mlp *m);
+int nrpyell_mlp_get_params(nrpyell_mlp *m, float *host);
+int nrpyell_mlp_set_params(nrpyell_mlp *m, const float *host);
+/* x: device BF16 [M x 64]; y: device FP32 [M] */
+int nrpyell_mlp_forward(nrpyell_mlp *m, const void *x, int M, float *y);
+/* mean squared error against device targets t [M], gradients (host, num_params) */
+int nrpyell_mlp_loss_grad(nrpyell_mlp *m, const void *x, const float *t, int M, double *loss, float *grads_host);
+int nrpyell_mlp_adam_step(nrpyell_mlp *m, float lr);
+/* nsteps Adam steps on uniform minibatches from a device dataset (x [N x 64], t [N]) */
+int nrpyell_mlp_train(nrpyell_mlp *m, const void *x, const float *t, long N, int batch, int nsteps, float lr,
+                      unsigned long long seed, double *losses_host);
+/* rows (interior + pure outer points, rounded up to 64) of a grid's feature matrix */
+long nrpyell_mlp_feature_rows(const griddata_struct *gd);
+/* device features x [rows x 64] (BF16), per-row u scale and grid index of a griddata */
+int nrpyell_mlp_features(const griddata_struct *gd, void *x, double *scale, long *idx);
+/* device training targets t = u/scale from griddata's y_n at the feature rows */
+int nrpyell_mlp_targets(const griddata_struct *gd, const double *scale, const long *idx, float *t);
+/* y_n <- network guess: u at interior and pure outer points, v = eta u, parity images */
+int nrpyell_mlp_seed(nrpyell_mlp *m, griddata_struct *gd);
+/* n_take uniform random rows of a device (x, t) set copied into (x_dst, t_dst) */
+int nrpyell_mlp_sample_rows(const void *x_src, const float *t_src, long n_src, long n_take, unsigned long long seed,
+                            void *x_dst, float *t_dst);
+int nrpyell_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -60,6 +60,11 @@ extern "C" int nrpyell_memcpy_d2h(void *dst, const void *src, size_t bytes) {
   if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
   return NRPYELL_OK;
 }
+extern "C" int nrpyell_memcpy_d2d(void *dst, const void *src, size_t bytes) {
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
+  return NRPYELL_OK;
+}
 extern "C" int nrpyell_memset_device(void *ptr, int value, size_t bytes) {
   hipError_t e = hipMemset(ptr, value, bytes);
   if (e != hipSuccess) { set_last_error(hipGetErrorString(e)); return NRPYELL_EHIP; }
@@ -430,11 +435,14 @@ extern "C" void gridfunction_z_axis(const paramstruct *params, const int gf_inde
   const GridDims g = dims_from_params(params);
   std::vector<double> h(g.ntot);
   HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
-  const double w = params->SINHWAA, A = params->AMAX, b = params->bScale;
-  const double D = exp(1.0 / w) - exp(-1.0 / w);
+  const double A = params->AMAX, b = params->bScale;
+  // z = sqrt(AMAX^2 sinh^2 / sinh^2 + b^2) cos x1, evaluated in the reference's operation order
+  // (its GCC build fuses the "+ b^2" into one fma), so the text output is identical
+  const double iw = 1.0 / params->SINHWAA;
+  const double D = exp(iw) - exp(-iw);
   auto zc = [&](double x0, double x1) {
-    const double s = exp(x0 / w) - exp(-x0 / w);
-    return sqrt(A * A * (s * s) / (D * D) + b * b) * cos(x1);
+    const double s = exp(iw * x0) - exp(-iw * x0);
+    return sqrt(fma(b, b, (A * A) * (s * s) / (D * D))) * cos(x1);
   };
   const int i2 = g.G2 / 2;
   for (int pass = 0; pass < 2; pass++) {
@@ -446,26 +454,45 @@ extern "C" void gridfunction_z_axis(const paramstruct *params, const int gf_inde
     fprintf(outfile, "%.17e %.17e\n", zc(xx[0][NRPYELL_NG], xx[1][i1]), h[idx3(g, NRPYELL_NG, i1, i2)]);
 }
 
-// gridfunction_xz_plane.c:6-49: x, z, gf on the xz plane (phi = 0 and phi = pi half planes).
+// gridfunction_xz_plane.c:6-49: x, z, gf on the xz plane: the x < 0 half at i2 = NGHOSTS,
+// then the x > 0 half at i2 = G2/2, "%.17e %.17e %.17e" per line.
 extern "C" void gridfunction_xz_plane(const paramstruct *params, const int gf_index, REAL *xx[3], const REAL *in_gf,
                                       FILE *outfile) {
   const GridDims g = dims_from_params(params);
   std::vector<double> h(g.ntot);
   HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
-  const double w = params->SINHWAA, A = params->AMAX, b = params->bScale;
-  const double D = exp(1.0 / w) - exp(-1.0 / w);
-  const int i2s[2] = {NRPYELL_NG + g.N2 / 4, NRPYELL_NG + (3 * g.N2) / 4};
+  const double A = params->AMAX, b = params->bScale;
+  const double iw = 1.0 / params->SINHWAA;
+  const double D = exp(iw) - exp(-iw);
+  const int i2s[2] = {NRPYELL_NG, g.G2 / 2};
   for (int k = 0; k < 2; k++) {
     const int i2 = i2s[k];
     for (int i1 = NRPYELL_NG; i1 < g.G1 - NRPYELL_NG; i1++)
       for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++) {
-        const double s = exp(xx[0][i0] / w) - exp(-xx[0][i0] / w);
-        const double f0 = A * s / D;
-        const double x = f0 * sin(xx[1][i1]) * cos(xx[2][i2]);
-        const double z = sqrt(f0 * f0 + b * b) * cos(xx[1][i1]);
+        const double s = exp(iw * xx[0][i0]) - exp(-iw * xx[0][i0]);
+        const double x = A * s * sin(xx[1][i1]) * cos(xx[2][i2]) / D;
+        const double z = sqrt(fma(b, b, (A * A) * (s * s) / (D * D))) * cos(xx[1][i1]);
         fprintf(outfile, "%.17e %.17e %.17e\n", x, z, h[idx3(g, i0, i1, i2)]);
       }
   }
+}
+
+// print_puncture_parameters.c:6-37: the same report on stdout.
+extern "C" void print_puncture_parameters(const paramstruct *p) {
+  const struct { const char *name; double v; } rows[] = {
+      {"bScale      ", p->bScale}, {"SINHWAA     ", p->SINHWAA}, {"AMAX        ", p->AMAX},
+      {"eta_damping ", p->eta_damping}, {"bare_mass_0 ", p->bare_mass_0}, {"bare_mass_1 ", p->bare_mass_1},
+      {"puncture_0_x", p->puncture_0_x}, {"puncture_0_y", p->puncture_0_y}, {"puncture_0_z", p->puncture_0_z},
+      {"puncture_1_x", p->puncture_1_x}, {"puncture_1_y", p->puncture_1_y}, {"puncture_1_z", p->puncture_1_z},
+      {"P0_x        ", p->P0_x}, {"P0_y        ", p->P0_y}, {"P0_z        ", p->P0_z},
+      {"P1_x        ", p->P1_x}, {"P1_y        ", p->P1_y}, {"P1_z        ", p->P1_z},
+      {"S0_x        ", p->S0_x}, {"S0_y        ", p->S0_y}, {"S0_z        ", p->S0_z},
+      {"S1_x        ", p->S1_x}, {"S1_y        ", p->S1_y}, {"S1_z        ", p->S1_z}};
+  for (const auto &r : rows) printf("%s = %lf\n", r.name, r.v);
+  printf("Nxx0         = %d\n", p->Nxx0);
+  printf("Nxx1         = %d\n", p->Nxx1);
+  printf("Nxx2         = %d\n", p->Nxx2);
+  printf("NGHOSTS      = %d\n", NGHOSTS);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -897,4 +924,166 @@ extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL d
   s->step64(s->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched solver: many configurations (each its own grid shape) relaxed together, one
+// launch per RK4 substep per tile width, each configuration stopping on its own residual.
+// ---------------------------------------------------------------------------------------
+__global__ void gather_state_kernel(SolverDeviceState *const *states, int n, double *out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const SolverDeviceState *s = states[k];
+  out[4 * k + 0] = (double)s->done;
+  out[4 * k + 1] = (double)s->steps;
+  out[4 * k + 2] = s->last_log10;
+  out[4 * k + 3] = (double)s->nchecks;
+}
+
+struct nrpyell_batch {
+  int n = 0;
+  std::vector<nrpyell_solver *> solvers;
+  nrpyell_solve_options opt{};
+  hipStream_t stream = nullptr;
+  struct Group {
+    int tx = 32;
+    std::vector<int> members;
+    SubstepArgs<double> *d_args[5] = {};
+    int *d_block_cfg = nullptr, *d_block_first = nullptr;
+    int nblocks = 0;
+  };
+  std::vector<Group> groups;
+  SolverDeviceState **d_states = nullptr;
+  double *d_heads = nullptr, *h_heads = nullptr;
+  hipGraphExec_t graph = nullptr;
+  ~nrpyell_batch() {
+    if (graph) hipGraphExecDestroy(graph);
+    for (auto &g : groups) {
+      for (auto *p : g.d_args) hipFree(p);
+      hipFree(g.d_block_cfg);
+      hipFree(g.d_block_first);
+    }
+    hipFree(d_states);
+    hipFree(d_heads);
+    if (h_heads) hipHostFree(h_heads);
+    for (auto *s : solvers) nrpyell_solver_destroy(s);
+    if (stream) hipStreamDestroy(stream);
+  }
+  void step(hipStream_t st) {
+    for (int stage = 1; stage <= 4; stage++)
+      for (auto &g : groups)
+        launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, stage, st);
+  }
+  void read_heads() {
+    hipLaunchKernelGGL(gather_state_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_states, n, d_heads);
+    HIPCHK(hipMemcpyAsync(h_heads, d_heads, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+};
+
+extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, const int *tile_i0, int n,
+                                    const nrpyell_solve_options *opt, nrpyell_batch **out) {
+  if (n <= 0) { set_last_error("empty batch"); return NRPYELL_EINVAL; }
+  auto b = std::make_unique<nrpyell_batch>();
+  b->n = n;
+  if (opt) b->opt = *opt; else nrpyell_default_options(&b->opt);
+  b->opt.precision = 64;
+  for (int k = 0; k < n; k++) {
+    nrpyell_solve_options o = b->opt;
+    o.tile_i0 = tile_i0 ? tile_i0[k] : 0;
+    nrpyell_solver *s = nullptr;
+    const int rc = nrpyell_solver_create(gds[k], dts[k], &o, &s);
+    if (rc != NRPYELL_OK) return rc;
+    b->solvers.push_back(s);
+  }
+  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  std::map<int, std::vector<int>> by_tx;
+  for (int k = 0; k < n; k++) by_tx[b->solvers[k]->tx].push_back(k);
+  for (auto &kv : by_tx) {
+    nrpyell_batch::Group g;
+    g.tx = kv.first;
+    g.members = kv.second;
+    const int m = (int)g.members.size();
+    std::vector<int> first(m), bcfg;
+    for (int j = 0; j < m; j++) {
+      nrpyell_solver *s = b->solvers[g.members[j]];
+      first[j] = (int)bcfg.size();
+      const int nb = s->ntile0 * s->ntile1 * s->g.N2;
+      for (int q = 0; q < nb; q++) bcfg.push_back(j);
+    }
+    g.nblocks = (int)bcfg.size();
+    g.d_block_cfg = upload(bcfg.data(), bcfg.size());
+    g.d_block_first = upload(first.data(), first.size());
+    for (int stage = 1; stage <= 4; stage++) {
+      std::vector<SubstepArgs<double>> args(m);
+      for (int j = 0; j < m; j++) {
+        nrpyell_solver *s = b->solvers[g.members[j]];
+        MoL_gridfunctions_struct &f = s->gd->gridfuncs;
+        SubstepArgs<double> a = s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs,
+                                                f.y_nplus1_running_total_gfs, f.auxevol_gfs, s->coef64, s->geo64);
+        if (stage <= 2) {
+          a.w2d = s->w2d;
+          a.l2_partial = s->partial;
+          a.l2_count = s->l2_count;
+          a.state = s->dstate;
+          a.l2_volume = s->volume;
+          a.l2_target = b->opt.log10_target;
+        }
+        args[j] = a;
+      }
+      g.d_args[stage] = upload(args.data(), args.size());
+    }
+    b->groups.push_back(g);
+  }
+  std::vector<SolverDeviceState *> st(n);
+  for (int k = 0; k < n; k++) st[k] = b->solvers[k]->dstate;
+  b->d_states = upload(st.data(), st.size());
+  b->d_heads = dalloc<double>(4 * n);
+  HIPCHK(hipHostMalloc((void **)&b->h_heads, sizeof(double) * 4 * n, hipHostMallocDefault));
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (int k = 0; k < b->opt.steps_per_graph; k++) b->step(b->stream);
+  HIPCHK(hipStreamEndCapture(b->stream, &graph));
+  HIPCHK(hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0));
+  HIPCHK(hipGraphDestroy(graph));
+  *out = b.release();
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, double *seconds) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (auto *s : b->solvers) reset_state(s);
+  int launched = 0;
+  while (true) {
+    HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    launched += b->opt.steps_per_graph;
+    b->read_heads();
+    bool all = true;
+    for (int k = 0; k < b->n; k++)
+      if (b->h_heads[4 * k] == 0.0) all = false;
+    if (all || launched >= b->opt.max_steps) break;
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  *seconds = std::chrono::duration<double>(t1 - t0).count();
+  int rc = NRPYELL_OK;
+  for (int k = 0; k < b->n; k++) {
+    nrpyell_solve_result &r = res[k];
+    memset(&r, 0, sizeof(r));
+    r.converged = b->h_heads[4 * k] == 1.0;
+    r.steps = r.converged ? (int)b->h_heads[4 * k + 1] : launched;
+    r.log10_l2 = b->h_heads[4 * k + 2];
+    r.seconds = *seconds;
+    if (b->h_heads[4 * k] == 2.0) rc = NRPYELL_ENAN;
+    else if (!r.converged && rc == NRPYELL_OK) rc = NRPYELL_ENOTCONVERGED;
+  }
+  if (rc == NRPYELL_ENAN) set_last_error("a configuration's residual became NaN");
+  if (rc == NRPYELL_ENOTCONVERGED) set_last_error("max_steps reached before every configuration converged");
+  return rc;
+}
+
+extern "C" int nrpyell_batch_destroy(nrpyell_batch *b) {
+  if (!b) return NRPYELL_OK;
+  hipStreamSynchronize(b->stream);
+  delete b;
+  return NRPYELL_OK;
 }

@@ -69,6 +69,9 @@ void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const doubl
 void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st);
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
+template <typename T>
+void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
+                            int tx, int stage, hipStream_t st);
 long launch_residual_l2(const GridDims &g, const void *u, bool u_is_float, const double *aux, const double *coef,
                         const double *w2d, double *partial, const int *done, double *res_out, hipStream_t st);
 long launch_l2_partial(const GridDims &g, const double *gf, const double *w2d, double *partial, hipStream_t st);

@@ -60,7 +60,10 @@ __device__ __forceinline__ void fd_sums(const T m[6], const T p[6], T uu, T &s2,
 }
 
 template <typename T, int STAGE, int TX>
-__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
+// One block's work: tile `tile` (t0 + ntile0*t1) of i2 plane `plane`; `bid` is the block's
+// index among the nblk blocks of this configuration (for the L2 partial sums).
+__device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int tile, const int plane_idx,
+                                             const unsigned bid, const unsigned nblk) {
   constexpr int TY = 256 / TX;
   constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG;
   __shared__ T su[BY * BX];  // u on this i2 plane: tile + halos
@@ -71,8 +74,8 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-  const int t0 = blockIdx.x % a.ntile0, t1 = blockIdx.x / a.ntile0;
-  const int i2 = NG + blockIdx.y;
+  const int t0 = tile % a.ntile0, t1 = tile / a.ntile0;
+  const int i2 = NG + plane_idx;
   const int i0s = NG + t0 * TX, i1s = NG + t1 * TY;
   const int i0 = i0s + tx, i1 = i1s + ty;
   const bool valid = i0 < g.N0 + NG && i1 < g.N1 + NG;
@@ -254,10 +257,9 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
     for (int o = 32; o > 0; o >>= 1) l2acc += __shfl_down(l2acc, o, 64);
     if ((tid & 63) == 0) s_red[tid >> 6] = l2acc;
     __syncthreads();
-    if (tid == 0) a.l2_partial[blockIdx.x + gridDim.x * blockIdx.y] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    if (tid == 0) a.l2_partial[bid] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
   }
-  if (STAGE == 2 && a.l2_partial != nullptr && blockIdx.x == 0 && blockIdx.y == 0) {
-    const unsigned nblk = gridDim.x * gridDim.y;
+  if (STAGE == 2 && a.l2_partial != nullptr && bid == 0) {
     double acc = 0.0;
     for (unsigned i = tid; i < nblk; i += 256) acc += a.l2_partial[i];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
@@ -283,6 +285,26 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
   }
 }
 
+// One configuration: grid (tiles, i2 planes).
+template <typename T, int STAGE, int TX>
+__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
+  substep_body<T, STAGE, TX>(a, blockIdx.x, blockIdx.y, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+}
+
+// Many configurations (uneven grids) in one launch: block b works for configuration
+// block_cfg[b], as that configuration's block b - block_first[cfg]. Configurations that have
+// converged return at once (their own done flag).
+template <typename T, int STAGE, int TX>
+__global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<T> *__restrict__ cfgs,
+                                                              const int *__restrict__ block_cfg,
+                                                              const int *__restrict__ block_first) {
+  const int c = block_cfg[blockIdx.x];
+  const SubstepArgs<T> a = cfgs[c];
+  const int local = blockIdx.x - block_first[c];
+  const int ntile = a.ntile0 * a.ntile1;
+  substep_body<T, STAGE, TX>(a, local % ntile, local / ntile, local, ntile * a.g.N2);
+}
+
 template <typename T, int STAGE>
 static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
   dim3 grd(a.ntile0 * a.ntile1, a.g.N2), blk(256);
@@ -292,6 +314,32 @@ static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
     default: hipLaunchKernelGGL((substep_kernel<T, STAGE, 32>), grd, blk, 0, st, a); break;
   }
 }
+
+template <typename T, int STAGE>
+static void launch_stage_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first,
+                                 int nblocks, int tx, hipStream_t st) {
+  dim3 grd(nblocks), blk(256);
+  switch (tx) {
+    case 16: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 16>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    case 64: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 64>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    default: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 32>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+  }
+}
+
+template <typename T>
+void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
+                            int tx, int stage, hipStream_t st) {
+  switch (stage) {
+    case 1: launch_stage_batched<T, 1>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
+    case 2: launch_stage_batched<T, 2>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
+    case 3: launch_stage_batched<T, 3>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
+    default: launch_stage_batched<T, 4>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
+  }
+}
+template void launch_substep_batched<double>(const SubstepArgs<double> *, const int *, const int *, int, int, int,
+                                             hipStream_t);
+template void launch_substep_batched<float>(const SubstepArgs<float> *, const int *, const int *, int, int, int,
+                                            hipStream_t);
 
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st) {

@@ -1,0 +1,132 @@
+"""Learned initial guess, end to end: generate training solutions with the batched FP64
+solver, train the MLP on the GPU, and evaluate NN-seeded solves against zero-guess solves
+with the fallback rule (zero guess whenever the NN guess has the larger initial residual).
+
+    python -m nrpyelliptic_amd.learned --train-configs 1024 --eval-configs 64 --out weights.npz
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+from typing import List
+
+import numpy as np
+
+from . import config, runtime
+from .mlp import K0, Features, InitialGuessMLP
+from .runtime import DeviceArray
+
+
+def solve_batch(problems, target=-10.0, max_steps=40000, tiles=None):
+    B = runtime.BatchSolver(problems, tiles=tiles, log10_target=target, max_steps=max_steps)
+    res, sec = B.solve()
+    B.close()
+    return res, sec
+
+
+def generate_dataset(configs: List[config.PunctureConfig], rows_per_config=8192, batch=64, target=-10.0, seed=0,
+                     log=print):
+    """Solve every configuration (batched, FP64, to `target`) and keep rows_per_config random
+    (features, t) rows of each solution in a device dataset."""
+    N = len(configs) * rows_per_config
+    X = DeviceArray(N * K0, np.uint16)
+    T = DeviceArray(N, np.float32)
+    row = 0
+    t0 = time.time()
+    for b0 in range(0, len(configs), batch):
+        chunk = configs[b0:b0 + batch]
+        probs = [runtime.Problem(c) for c in chunk]
+        res, sec = solve_batch(probs, target=target)
+        for k, (P, r) in enumerate(zip(probs, res)):
+            F = Features(P)
+            t = F.targets(P)
+            F.sample_into(t, rows_per_config, seed * 1000003 + b0 + k, X, T, row)
+            row += rows_per_config
+            P.close()
+        conv = sum(r.converged for r in res)
+        log(f"dataset: solved {b0 + len(chunk)}/{len(configs)} ({conv}/{len(chunk)} converged, "
+            f"{sec:.2f}s batch, mean steps {np.mean([r.steps for r in res]):.0f}, {time.time() - t0:.1f}s total)")
+    return X, T, N
+
+
+def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem):
+    """Zero-guess residual vs NN-guess residual; keep the NN guess only if its residual is lower."""
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    r0 = P.log10_residual()
+    net.seed(P)
+    r1 = P.log10_residual()
+    used = r1 < r0
+    if not used:
+        P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    return used, r0, r1
+
+
+def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, log=print):
+    """Per configuration: steps and seconds to `target` from the zero guess and from the NN
+    guess (with the fallback), single-configuration solver, FP64 or mixed precision."""
+    rows = []
+    for k, cfg in enumerate(configs):
+        P = runtime.Problem(cfg)
+        S = P.solver(log10_target=target)
+        P.set("y_n_gfs", np.zeros(2 * P.ntot))
+        rz = S.solve()
+        used, r0, r1 = seed_with_fallback(net, P)
+        S2 = P.solver(log10_target=target, precision=precision)
+        rn = S2.solve()
+        rows.append({"config": cfg.name, "zero_steps": rz.steps, "zero_s": rz.seconds, "nn_steps": rn.steps,
+                     "nn_fp32_steps": rn.fp32_steps, "nn_s": rn.seconds, "nn_used": bool(used),
+                     "log10_res0_zero": r0, "log10_res0_nn": r1, "converged": bool(rz.converged and rn.converged)})
+        S.close()
+        S2.close()
+        P.close()
+        if k < 8 or k % 16 == 0:
+            log(json.dumps(rows[-1]))
+    return rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train-configs", type=int, default=1024)
+    ap.add_argument("--rows-per-config", type=int, default=8192)
+    ap.add_argument("--eval-configs", type=int, default=64)
+    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--target", type=float, default=-10.0)
+    ap.add_argument("--out", default="mlp_initial_guess.npz")
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args()
+    log = lambda s: print(s, flush=True)  # noqa: E731
+    train_cfgs = config.random_batch(args.train_configs, seed=1000)
+    X, T, N = generate_dataset(train_cfgs, rows_per_config=args.rows_per_config, target=args.target, log=log)
+    net = InitialGuessMLP(seed=7)
+    t0 = time.time()
+    losses = net.train(X, T, N, batch=args.batch, steps=args.steps, lr=args.lr, seed=11)
+    log(f"train: {args.steps} steps in {time.time() - t0:.1f}s, loss {losses[:10].mean():.3e} -> "
+        f"{losses[-50:].mean():.3e}")
+    net.save(args.out)
+    eval_cfgs = config.random_batch(args.eval_configs, seed=900, grids=False, Nxx=(128, 128, 16))
+    rows64 = evaluate(net, eval_cfgs, precision=64, target=args.target, log=log)
+    rows32 = evaluate(net, eval_cfgs, precision=32, target=args.target, log=log)
+
+    def summary(rows):
+        z = np.array([r["zero_s"] for r in rows])
+        n = np.array([r["nn_s"] for r in rows])
+        zs = np.array([r["zero_steps"] for r in rows])
+        ns = np.array([r["nn_steps"] for r in rows])
+        return {"mean_zero_steps": float(zs.mean()), "mean_nn_steps": float(ns.mean()),
+                "speedup_time_mean": float(z.sum() / n.sum()), "speedup_steps_mean": float(zs.sum() / ns.sum()),
+                "nn_used_frac": float(np.mean([r["nn_used"] for r in rows])),
+                "all_converged": bool(all(r["converged"] for r in rows))}
+
+    rep = {"train_configs": args.train_configs, "rows": N, "final_loss": float(losses[-50:].mean()),
+           "fp64": summary(rows64), "mixed": summary(rows32)}
+    log(json.dumps(rep))
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump({"summary": rep, "fp64": rows64, "mixed": rows32, "losses": losses[::10].tolist()}, f)
+
+
+if __name__ == "__main__":
+    main()

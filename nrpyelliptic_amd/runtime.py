@@ -75,6 +75,10 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_time_substep.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.nrpyell_solver_timed_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.nrpyell_batch_create.argtypes = [C.POINTER(C.POINTER(abi.griddata_struct)), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options), C.POINTER(vp)]
+    L.nrpyell_batch_solve.argtypes = [vp, C.POINTER(solve_result), C.POINTER(C.c_double)]
+    L.nrpyell_batch_destroy.argtypes = [vp]
     _LIB = L
     return L
 
@@ -324,6 +328,42 @@ class Solver:
     def close(self):
         if self.h:
             lib().nrpyell_solver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchSolver:
+    """Relaxes many configurations (uneven grids) together on one GPU: one launch per RK4
+    substep per tile width covers the whole batch; each configuration stops at its own
+    residual target. `tiles` gives each configuration's tile width (from the autotuner)."""
+
+    def __init__(self, problems, tiles=None, **opts):
+        self.problems = list(problems)
+        n = len(self.problems)
+        self.opts = default_options(**opts)
+        gds = (C.POINTER(abi.griddata_struct) * n)(*[C.pointer(p.gd) for p in self.problems])
+        dts = (C.c_double * n)(*[p.dt for p in self.problems])
+        tl = (C.c_int * n)(*[int(t) for t in (tiles or [0] * n)])
+        self.h = C.c_void_p()
+        check(lib().nrpyell_batch_create(gds, dts, tl, n, C.byref(self.opts), C.byref(self.h)), "batch_create")
+
+    def solve(self):
+        n = len(self.problems)
+        res = (solve_result * n)()
+        sec = C.c_double()
+        rc = lib().nrpyell_batch_solve(self.h, res, C.byref(sec))
+        if rc not in (NRPYELL_OK, -4):
+            check(rc, "batch_solve")
+        return list(res), sec.value
+
+    def close(self):
+        if self.h:
+            lib().nrpyell_batch_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

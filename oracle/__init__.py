@@ -52,6 +52,8 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
     if os.path.isdir("/root/reference/NRPyElliptic_Ccodes"):
         subprocess.run(["make", "-s", "-j8", "-C", HERE, "ref"], check=True)
+        if os.path.exists(os.path.join(HERE, "..", "nrpyelliptic_amd", "libnrpyell_hip.so")):
+            subprocess.run(["make", "-s", "-C", HERE, "drivers"], check=True)
 
 
 def lib():
