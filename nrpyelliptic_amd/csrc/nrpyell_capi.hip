@@ -584,6 +584,9 @@ struct nrpyell_solver {
       launch_substep<double>(a, s, st);
     }
   }
+  // The fused kernels keep only the interior phi planes current; this fills the ghost planes
+  // of y_n from them before the state is handed back.
+  void fill_ghosts(hipStream_t st) { launch_fill_ghost_planes<double>(g, gd->gridfuncs.y_n_gfs, 2, st); }
   void step32(hipStream_t st) {
     for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, aux32, coef32, geo32), s, st);
   }
@@ -655,7 +658,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   s->coef64 = dalloc<double>(5 * n2);
   s->coef32 = dalloc<float>(5 * n2);
   launch_coef(g, &gd->rfmstruct, s->coef64, s->coef32, s->stream);
-  std::vector<double> geo((size_t)6 * g.G1 * 3);
+  std::vector<double> geo((size_t)6 * g.G1 * 4);
   host_outer_geometry(&gd->params, gd->xx, geo.data());
   s->geo64 = upload(geo.data(), geo.size());
   std::vector<float> geof(geo.begin(), geo.end());
@@ -719,6 +722,7 @@ static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double t
 extern "C" int nrpyell_solver_steps(nrpyell_solver *s, int n) {
   reset_state(s);
   for (int k = 0; k < n; k++) s->step64(s->stream);
+  s->fill_ghosts(s->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
   return NRPYELL_OK;
@@ -788,6 +792,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
     }
     steps += s->hstate->steps;
   }
+  s->fill_ghosts(s->stream);
   HIPCHK(hipStreamSynchronize(s->stream));
   const auto t1 = std::chrono::steady_clock::now();
   res->steps = steps;
@@ -825,6 +830,7 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  s->fill_ghosts(s->stream);
   read_state(s);
   *ms_out = ms;
   if (log10_out) *log10_out = s->hstate->last_log10;
@@ -922,6 +928,7 @@ extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL d
   s->dt = dt;
   HIPCHK(hipMemsetAsync(&s->dstate->done, 0, sizeof(int), s->stream));
   s->step64(s->stream);
+  s->fill_ghosts(s->stream);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->stream));
 }
@@ -1063,6 +1070,8 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
       if (b->h_heads[4 * k] == 0.0) all = false;
     if (all || launched >= b->opt.max_steps) break;
   }
+  for (auto *s : b->solvers) s->fill_ghosts(b->stream);
+  HIPCHK(hipStreamSynchronize(b->stream));
   const auto t1 = std::chrono::steady_clock::now();
   *seconds = std::chrono::duration<double>(t1 - t0).count();
   int rc = NRPYELL_OK;

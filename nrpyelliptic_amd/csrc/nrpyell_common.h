@@ -83,6 +83,26 @@ __host__ __device__ inline void for_each_image(const GridDims &g, int i0, int i1
     }
 }
 
+// The images of (i0,i1,i2) that lie in the interior phi planes [NG, N2+NG) only: the i0/i1
+// reflection ghosts, each in the one interior plane its half turn selects. Kernels that read
+// phi neighbours through wrap2 never read the ghost planes, so the solver keeps just these
+// and fills the ghost planes once, when the state is handed back (fill_ghost_planes).
+template <class F>
+__host__ __device__ inline void for_each_image_in_planes(const GridDims &g, int i0, int i1, int i2, F &&f) {
+  const bool ra = i0 >= NRPYELL_NG && i0 < 2 * NRPYELL_NG;
+  int b_c[2], nb = 0;
+  if (i1 >= NRPYELL_NG && i1 < 2 * NRPYELL_NG) b_c[nb++] = 2 * NRPYELL_NG - 1 - i1;
+  if (i1 >= g.N1 && i1 < g.N1 + NRPYELL_NG && i1 >= NRPYELL_NG) b_c[nb++] = 2 * (g.N1 + NRPYELL_NG) - 1 - i1;
+  if (!ra && nb == 0) return;
+  const int j2h = wrap2(g, i2 + g.N2 / 2);  // one reflection: half turn
+  const int a = 2 * NRPYELL_NG - 1 - i0;
+  if (ra) f(idx3(g, a, i1, j2h));
+  for (int ib = 0; ib < nb; ib++) {
+    f(idx3(g, i0, b_c[ib], j2h));
+    if (ra) f(idx3(g, a, b_c[ib], i2));  // two reflections: no turn
+  }
+}
+
 // 10th-order centred FD weights (rhs_eval.c:23-27; Fornberg).
 #define FD1_W1 (5.0 / 6.0)
 #define FD1_W2 (5.0 / 21.0)

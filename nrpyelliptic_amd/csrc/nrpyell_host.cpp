@@ -7,6 +7,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -259,6 +260,28 @@ int ghostmap_verify(const GridDims &g, const bc_struct *bc, std::string *err) {
                     " images for " + std::to_string(n) + " inner points)";
     return NRPYELL_EGHOSTMAP;
   }
+  // the interior-plane enumeration of the solver kernels is exactly the interior-plane part
+  // of the full one
+  std::vector<long> full, part;
+  for (int i2 = NG; i2 < g.N2 + NG; i2++)
+    for (int i1 = NG; i1 < g.N1 + NG; i1++)
+      for (int i0 = NG; i0 < g.G0; i0++) {
+        if (i0 >= g.N0 + NG && seen[idx3(g, i0, i1, i2)]) continue;
+        full.clear();
+        part.clear();
+        for_each_image(g, i0, i1, i2, [&](long d) {
+          const long j2 = d / ((long)g.G0 * g.G1);
+          if (j2 >= NG && j2 < g.N2 + NG) full.push_back(d);
+        });
+        for_each_image_in_planes(g, i0, i1, i2, [&](long d) { part.push_back(d); });
+        std::sort(full.begin(), full.end());
+        std::sort(part.begin(), part.end());
+        if (full != part) {
+          if (err) *err = "interior-plane image enumeration differs from the full one at (" + std::to_string(i0) + "," +
+                          std::to_string(i1) + "," + std::to_string(i2) + ")";
+          return NRPYELL_EGHOSTMAP;
+        }
+      }
   return NRPYELL_OK;
 }
 
@@ -322,14 +345,24 @@ void host_radial_geometry(const paramstruct *p, double x0, double x1, double out
   out[3] = 0.0;
 }
 
+// Per outer layer pt = 0..5 (pt = 0: last interior i0) and i1:
+//   [1/r, dxx0/dr, dxx1/dr, (r_0/r_pt)^3].
 void host_outer_geometry(const paramstruct *p, REAL *const xx[3], double *geo) {
   const int G1 = p->Nxx_plus_2NGHOSTS1, ilast = p->Nxx0 + NG - 1;
-  for (int pt = 0; pt < 6; pt++)
-    for (int i1 = 0; i1 < G1; i1++) {
+  for (int i1 = 0; i1 < G1; i1++) {
+    double o0[4];
+    host_radial_geometry(p, xx[0][ilast], xx[1][i1], o0);
+    for (int pt = 0; pt < 6; pt++) {
       double o[4];
       host_radial_geometry(p, xx[0][ilast + pt], xx[1][i1], o);
-      for (int k = 0; k < 3; k++) geo[((long)pt * G1 + i1) * 3 + k] = o[k];
+      const double q = o0[0] / o[0];
+      double *e = geo + ((long)pt * G1 + i1) * 4;
+      e[0] = 1.0 / o[0];
+      e[1] = o[1];
+      e[2] = o[2];
+      e[3] = q * q * q;
     }
+  }
 }
 
 // L2_norm_of_gf.c:14-35: dV = sqrt(det gamma_hat) dxx0 dxx1 dxx2 inside r < radius, 0 outside;

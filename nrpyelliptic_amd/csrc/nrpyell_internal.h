@@ -45,7 +45,7 @@ struct SubstepArgs {
   T *rt;          // running total
   const T *aux;   // AA^2 | psi | wavespeed
   const T *coef;  // 5 x (G0*G1)
-  const T *geo;   // outer-face geometry: [pt 0..5][i1][r, dxx0/dr, dxx1/dr], pt 0 = last interior i0
+  const T *geo;   // outer-face geometry: [pt 0..5][i1][1/r, dxx0/dr, dxx1/dr, (r_0/r_pt)^3], pt 0 = last interior i0
   T cOB_u, cOB_v;
   T eta, dt;
   const int *done;  // nonzero: converged, every launch returns at once
@@ -67,6 +67,8 @@ void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *au
 void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in,
                            double *out, hipStream_t st);
 void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st);
+template <typename T>
+void launch_fill_ghost_planes(const GridDims &g, T *y, int ngf, hipStream_t st);
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 template <typename T>
@@ -93,7 +95,7 @@ void launch_fill(long n, double v, double *out, hipStream_t st);
 // host setup (nrpyell_host.cpp)
 GridDims dims_from_params(const paramstruct *p);
 void host_min_ds_2d(const paramstruct *p, REAL *const xx[3], double *ds2d);  // G0*G1, interior entries
-void host_outer_geometry(const paramstruct *p, REAL *const xx[3], double *geo);  // 6*G1*3
+void host_outer_geometry(const paramstruct *p, REAL *const xx[3], double *geo);  // 6*G1*4
 void host_l2_weights(const paramstruct *p, REAL *const xx[3], double radius, double *w2d, double *volume);
 void host_radial_geometry(const paramstruct *p, double x0, double x1, double out[4]);
 int host_bcstruct(const paramstruct *params, REAL *xx[3], bc_struct *bc, std::string *err);

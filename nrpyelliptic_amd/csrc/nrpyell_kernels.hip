@@ -194,7 +194,8 @@ __global__ void __launch_bounds__(256) residual_l2_partial_kernel(GridDims g, co
     for (int k = 1; k <= 5; k++) {
       m0[k] = (double)u[p - k]; p0[k] = (double)u[p + k];
       m1[k] = (double)u[p - k * s1]; p1[k] = (double)u[p + k * s1];
-      m2[k] = (double)u[p - k * s2]; p2[k] = (double)u[p + k * s2];
+      // phi neighbours through the periodic map (the solver leaves the ghost planes stale)
+      m2[k] = (double)u[(long)wrap2(g, i2 - k) * s2 + k2]; p2[k] = (double)u[(long)wrap2(g, i2 + k) * s2 + k2];
     }
     const double uu = (double)u[p];
     double s00 = -FD2_W0 * uu, s11 = s00, s22 = s00;
@@ -483,6 +484,27 @@ void launch_convert_d2f(long n, const double *in, float *out, hipStream_t st) {
 void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st) {
   hipLaunchKernelGGL((convert_kernel<double, float>), dim3((n + 255) / 256), dim3(256), 0, st, n, in, out);
 }
+
+// Ghost phi planes (i2 < NG, i2 >= N2+NG) of the first ngf gridfunctions of y from the
+// interior planes: every point takes its source's value (ghost_source; parity +1 for u and v).
+// Sources lie in interior planes, so no thread reads what another writes.
+template <typename T>
+__global__ void fill_ghost_planes_kernel(GridDims g, T *y, int ngf) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = blockIdx.y, k = blockIdx.z;
+  if (i0 >= g.G0) return;
+  const int i2 = k < NG ? k : g.N2 + k;
+  int s0, s1, s2;
+  ghost_source(g, i0, i1, i2, s0, s1, s2);
+  const long d = idx3(g, i0, i1, i2), src = idx3(g, s0, s1, s2);
+  for (int f = 0; f < ngf; f++) y[f * g.ntot + d] = y[f * g.ntot + src];
+}
+template <typename T>
+void launch_fill_ghost_planes(const GridDims &g, T *y, int ngf, hipStream_t st) {
+  hipLaunchKernelGGL(fill_ghost_planes_kernel<T>, dim3((g.G0 + 63) / 64, g.G1, 2 * NG), dim3(64), 0, st, g, y, ngf);
+}
+template void launch_fill_ghost_planes<double>(const GridDims &, double *, int, hipStream_t);
+template void launch_fill_ghost_planes<float>(const GridDims &, float *, int, hipStream_t);
 
 __global__ void fill_kernel(long n, double v, double *out) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -3,14 +3,17 @@
 // One launch does, for one RK4 substep (MoL_step_forward_in_time.c:11-176):
 //   1. RHS at every interior point: 10th-order star stencil of u (rhs_eval.c:86-110) with the
 //      precombined (i0,i1) metric coefficients, v - eta u, and c^2 (lap u + AA^2/(8 (psi+u)^7));
-//   2. the radiation condition on the 5 outer ghost layers of the x0-max face, innermost first
-//      (apply_bcs_outerradiation_and_inner.c:369-408), done by the blocks that own the last
-//      interior i0 of each (i1,i2) column, entirely from LDS;
+//   2. the radiation condition on the 5 outer ghost layers of the x0-max face (the reference's
+//      innermost-first recurrence, apply_bcs_outerradiation_and_inner.c:369-408, in closed
+//      form), done by the blocks that own the last interior i0 of each (i1,i2) row, from LDS;
 //   3. the stage update (RK4 k_odd/k_even scheme) of every interior and outer point;
 //   4. the inner ghost zones: each updated point writes its own parity images (the closed form
 //      of the bcstruct map, verified against the table on the host), which is the inner-BC
 //      gather of apply_bcs_inner_only.c:18-26 applied to the stage output (parity +1 commutes
-//      with the linear stage update, so the values are those of the reference ordering);
+//      with the linear stage update, so the values are those of the reference ordering).
+//      Only the images inside the interior phi planes are written: phi neighbours are read
+//      through the periodic map, so the ghost planes are dead during a solve and are filled
+//      once when the state is handed back (fill_ghost_planes_kernel);
 //   5. (stage 1, FP64) the L2 of the residual of the stage input y_n: the stencil of step 1 is
 //      c^2 times the residual, so a block reduction and a last-block finalize give the
 //      reference's every-step convergence check (main.c:146-151) for free.
@@ -68,8 +71,8 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG;
   __shared__ T su[BY * BX];  // u on this i2 plane: tile + halos
   __shared__ T sv[BY * BX];  // v likewise (BC tiles only)
-  __shared__ T s_rhs[TY][2][6], s_dr[TY][2][6], s_f[TY][2][6];
-  __shared__ T s_geo[6][TY][3];
+  __shared__ T s_rhs0[TY][2], s_A[TY][2][6];
+  __shared__ T s_geo[6][TY][4];
   __shared__ double s_red[4];
 
   const GridDims &g = a.g;
@@ -83,23 +86,31 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   const long plane = (long)i2 * s2;
   const int ilast = g.N0 + NG - 1;  // last interior i0; the radiation BC starts next to it
   const bool has_bc = (i0s <= ilast) && (ilast < i0s + TX);
+  const int bx0 = ilast - (i0s - NG);  // box column of the last interior point (BC tiles)
   const bool fuse_l2 = (STAGE == 1) && (a.l2_partial != nullptr);
   const int done = a.done ? *a.done : 0;
 
-  // ---- global loads: LDS staging of the plane, the i2 arm and the pointwise inputs
+  // ---- global loads: LDS staging of the plane, the i2 arm and the pointwise inputs.
+  // u: the tile with its i0/i1 halos; BC tiles also need the halo rows (corners included) of
+  // the six columns ilast..ilast+5 (i1 derivative at the outer layers). v (BC tiles only):
+  // columns ilast-6..ilast+5 of the tile rows (shifted i0 derivatives) and the halo rows of
+  // columns ilast..ilast+5.
   T stage_u[(BX * BY + 255) / 256], stage_v[(BX * BY + 255) / 256];
 #pragma unroll
   for (int r = 0; r < (BX * BY + 255) / 256; r++) {
     const int k = tid + 256 * r;
     const int bx = k % BX, by = k / BX;
     const int j0 = i0s - NG + bx, j1 = i1s - NG + by;
-    const bool corner = (bx < NG || bx >= TX + NG) && (by < NG || by >= TY + NG);
+    const bool halo_row = by < NG || by >= TY + NG;
+    const bool corner = (bx < NG || bx >= TX + NG) && halo_row;
+    const bool bc_col = has_bc && bx >= bx0 && bx <= bx0 + NG;
+    const bool need_v = has_bc && bx >= bx0 - NG - 1 && bx <= bx0 + NG && (!halo_row || bx >= bx0);
     stage_u[r] = 0;
     stage_v[r] = 0;
-    if (k < BX * BY && j0 < g.G0 && j1 < g.G1 && (!corner || has_bc)) {
+    if (k < BX * BY && j0 < g.G0 && j1 < g.G1) {
       const long q = plane + (long)j1 * s1 + j0;
-      stage_u[r] = a.in[q];
-      if (has_bc) stage_v[r] = a.in[nt + q];
+      if (!corner || bc_col) stage_u[r] = a.in[q];
+      if (need_v) stage_v[r] = a.in[nt + q];
     }
   }
   const long p = valid ? plane + (long)i1 * s1 + i0 : 0;
@@ -108,10 +119,11 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   T vv = 0, psi = 1, AA = 0, c = 0, yn_u = 0, yn_v = 0, rt_u = 0, rt_v = 0, K[5] = {0, 0, 0, 0, 0};
   double w2 = 0.0;
   if (valid) {
+    // the i2 arm through the periodic map: the ghost planes are never read
 #pragma unroll
     for (int k = 1; k <= 5; k++) {
-      zm[k] = a.in[p - k * s2];
-      zp[k] = a.in[p + k * s2];
+      zm[k] = a.in[(long)wrap2(g, i2 - k) * s2 + k2];
+      zp[k] = a.in[(long)wrap2(g, i2 + k) * s2 + k2];
     }
     vv = a.in[nt + p];
     AA = a.aux[ADD_TIMES_AUUGF * nt + p];
@@ -140,10 +152,10 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
     }
   }
   if (has_bc)
-    for (int k = tid; k < 18 * TY; k += 256) {
-      const int pt = k / (3 * TY), row = (k / 3) % TY, comp = k % 3;
+    for (int k = tid; k < 24 * TY; k += 256) {
+      const int pt = k / (4 * TY), row = (k / 4) % TY, comp = k % 4;
       const int j1 = i1s + row;
-      s_geo[pt][row][comp] = j1 < g.G1 ? a.geo[((long)pt * g.G1 + j1) * 3 + comp] : (T)1;
+      s_geo[pt][row][comp] = j1 < g.G1 ? a.geo[((long)pt * g.G1 + j1) * 4 + comp] : (T)1;
     }
   if (done) return;  // converged: uniform across the grid
 #pragma unroll
@@ -182,8 +194,8 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
     const T kv = (c * c) * L;          // rhs_eval.c:110
     if (fuse_l2) l2acc = w2 * (double)L * (double)L;
     if (has_bc && i0 == ilast) {
-      s_rhs[ty][0][0] = ku;
-      s_rhs[ty][1][0] = kv;
+      s_rhs0[ty][0] = ku;
+      s_rhs0[ty][1] = kv;
     }
     T o_u, o_v;
     stage_update<T, STAGE>(ku, a.dt, yn_u, rt_u, o_u);
@@ -194,15 +206,19 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
     }
     a.out[p] = o_u;
     a.out[nt + p] = o_v;
-    for_each_image(g, i0, i1, i2, [&](long d) {
+    for_each_image_in_planes(g, i0, i1, i2, [&](long d) {
       a.out[d] = o_u;
       a.out[nt + d] = o_v;
     });
   }
 
-  // ---- 2: radiation BC on the outer layers of this tile's rows
+  // ---- 2: radiation BC on the outer layers of this tile's rows. With the radiation term
+  // A_pt = -c (df/dr + f/r) at layer pt (pt = 0: the last interior point), the reference's
+  // innermost-first recurrence (apply_bcs_outerradiation_and_inner.c:369-408)
+  //   rhs_pt = A_pt + (r_{pt-1}/r_pt)^3 (rhs_{pt-1} - A_{pt-1})
+  // telescopes to rhs_pt = A_pt + (r_0/r_pt)^3 (rhs_0 - A_0), so the layers are independent
+  // (1/r and (r_0/r_pt)^3 come precomputed in the geometry table).
   if (has_bc) {
-    const int bx0 = ilast - (i0s - NG);  // box column of the last interior point
     if (tid < 12 * TY) {
       const int row = tid / 12, rem = tid % 12, gf = rem / 6, pt = rem % 6;
       if (i1s + row < g.N1 + NG) {
@@ -211,42 +227,31 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
         T d0 = 0, d1 = 0;
 #pragma unroll
         for (int j = 0; j < 11; j++) {
-          d0 = fma((T)c_fd1_shift[sh][j], box[by * BX + bx + (j - 5 - sh)], d0);
+          // column ilast-6 (pt = 0, j = 0) lies left of the box when the tile holds only
+          // the last interior column: read it from global memory
+          const int col = bx + (j - 5 - sh);
+          const T f0v = col >= 0 ? box[by * BX + col] : a.in[gf * nt + plane + (long)(i1s + row) * s1 + (i0s - NG + col)];
+          d0 = fma((T)c_fd1_shift[sh][j], f0v, d0);
           d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * BX + bx], d1);
         }
         d0 *= (T)g.invdx0;
         d1 *= (T)g.invdx1;
-        s_dr[row][gf][pt] = s_geo[pt][row][1] * d0 + s_geo[pt][row][2] * d1;
-        s_f[row][gf][pt] = box[by * BX + bx];
-      }
-    }
-    __syncthreads();
-    if (tid < 2 * TY) {
-      const int row = tid / 2, gf = tid % 2;
-      if (i1s + row < g.N1 + NG) {
+        const T dr = s_geo[pt][row][1] * d0 + s_geo[pt][row][2] * d1;
         const T cc = gf == 0 ? a.cOB_u : a.cOB_v;
-        T prev = s_rhs[row][gf][0];
-#pragma unroll
-        for (int pt = 1; pt <= 5; pt++) {
-          const T rq = s_geo[pt - 1][row][0], rp = s_geo[pt][row][0];
-          const T out_int = -cc * (s_dr[row][gf][pt - 1] + s_f[row][gf][pt - 1] / rq);
-          const T kk = rq * rq * rq * (prev - out_int);
-          const T rinv = (T)1 / rp;
-          prev = -cc * (s_dr[row][gf][pt] + s_f[row][gf][pt] * rinv) + kk * rinv * rinv * rinv;
-          s_rhs[row][gf][pt] = prev;
-        }
+        s_A[row][gf][pt] = -cc * (dr + box[by * BX + bx] * s_geo[pt][row][0]);
       }
     }
     __syncthreads();
     if (od_p >= 0) {
       const int row = tid / 10, rem = tid % 10, gf = rem / 5, pt = rem % 5 + 1;
+      const T rhs = fma(s_geo[pt][row][3], s_rhs0[row][gf] - s_A[row][gf][0], s_A[row][gf][pt]);
       T o;
       T rt_ = od_rt;
-      stage_update<T, STAGE>(s_rhs[row][gf][pt], a.dt, od_yn, rt_, o);
+      stage_update<T, STAGE>(rhs, a.dt, od_yn, rt_, o);
       if (STAGE != 4) a.rt[od_p] = rt_;
       a.out[od_p] = o;
       T *outg = a.out + gf * nt;
-      for_each_image(g, ilast + pt, i1s + row, i2, [&](long d) { outg[d] = o; });
+      for_each_image_in_planes(g, ilast + pt, i1s + row, i2, [&](long d) { outg[d] = o; });
     }
   }
 
@@ -285,29 +290,41 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   }
 }
 
-// One configuration: grid (tiles, i2 planes).
-template <typename T, int STAGE, int TX>
-__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
-  substep_body<T, STAGE, TX>(a, blockIdx.x, blockIdx.y, blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+// XCD-aware block order. Workgroups are dispatched to the 8 XCDs round robin (workgroup L to
+// XCD L % 8), and each XCD has its own L2. Logical block q = (L % 8) * (n / 8) + L / 8 gives
+// every XCD one contiguous run of logical blocks; with the i2 plane fastest in q, the blocks
+// that read a tile column's i2 arm, and the neighbouring tiles that read its i1 halos, share
+// one L2. Blocks past the last multiple of 8 keep their index.
+__device__ __forceinline__ unsigned xcd_logical(unsigned L, unsigned n) {
+  const unsigned per = n / 8;
+  return L < 8 * per ? (L % 8) * per + L / 8 : L;
 }
 
-// Many configurations (uneven grids) in one launch: block b works for configuration
-// block_cfg[b], as that configuration's block b - block_first[cfg]. Configurations that have
+// One configuration: a 1-D grid of ntile0*ntile1*N2 blocks.
+template <typename T, int STAGE, int TX>
+__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
+  const unsigned n = gridDim.x, q = xcd_logical(blockIdx.x, n);
+  const unsigned N2 = (unsigned)a.g.N2;
+  substep_body<T, STAGE, TX>(a, q / N2, q % N2, q, n);
+}
+
+// Many configurations (uneven grids) in one launch: logical block q works for configuration
+// block_cfg[q], as that configuration's block q - block_first[cfg]. Configurations that have
 // converged return at once (their own done flag).
 template <typename T, int STAGE, int TX>
 __global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<T> *__restrict__ cfgs,
                                                               const int *__restrict__ block_cfg,
                                                               const int *__restrict__ block_first) {
-  const int c = block_cfg[blockIdx.x];
+  const unsigned q = xcd_logical(blockIdx.x, gridDim.x);
+  const int c = block_cfg[q];
   const SubstepArgs<T> a = cfgs[c];
-  const int local = blockIdx.x - block_first[c];
-  const int ntile = a.ntile0 * a.ntile1;
-  substep_body<T, STAGE, TX>(a, local % ntile, local / ntile, local, ntile * a.g.N2);
+  const unsigned local = q - block_first[c], N2 = (unsigned)a.g.N2;
+  substep_body<T, STAGE, TX>(a, local / N2, local % N2, local, a.ntile0 * a.ntile1 * N2);
 }
 
 template <typename T, int STAGE>
 static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
-  dim3 grd(a.ntile0 * a.ntile1, a.g.N2), blk(256);
+  dim3 grd(a.ntile0 * a.ntile1 * a.g.N2), blk(256);
   switch (a.tx) {
     case 16: hipLaunchKernelGGL((substep_kernel<T, STAGE, 16>), grd, blk, 0, st, a); break;
     case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64>), grd, blk, 0, st, a); break;

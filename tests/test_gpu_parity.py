@@ -13,8 +13,16 @@ from nrpyelliptic_amd import config, runtime
 
 pytestmark = pytest.mark.gpu
 
-SMALL = [config.shipped_free_parameters((16, 16, 6)), config.unequal_mass_q4((32, 32, 8))]
-IDS = ["shipped_16x16x6", "q4_32x32x8"]
+# Square and non-square grids (Nxx0 != Nxx1 catches any i0/i1 mix-up), ragged and full tiles,
+# generic spins and momenta.
+SMALL = [config.shipped_free_parameters((16, 16, 6)), config.unequal_mass_q4((32, 32, 8)),
+         config.random_batch(1, seed=42, grids=False, Nxx=(24, 16, 6))[0],
+         config.random_batch(1, seed=5, grids=False, Nxx=(16, 40, 8))[0],
+         config.random_batch(1, seed=9, grids=False, Nxx=(33, 16, 6))[0],
+         config.random_batch(1, seed=42, grids=False, Nxx=(96, 128, 16))[0]]
+# 33: the tile holding the last interior i0 holds only that column (the one-sided BC stencil
+# reaches into the previous tile)
+IDS = ["shipped_16x16x6", "q4_32x32x8", "rand_24x16x6", "rand_16x40x8", "rand_33x16x6", "rand_96x128x16"]
 
 
 def interior_mask(Ng):
