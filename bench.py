@@ -87,6 +87,54 @@ def run_cpu_baseline_subprocess(steps=150, warmup=5):
     return out
 
 
+def dropin_call_us(P, n=50):
+    """Wall time per call of the reference-surface entry points on the problem's device
+    arrays (each call ends with a device synchronize, as a C caller sees it)."""
+    import ctypes as C
+    from nrpyelliptic_amd import runtime
+    L = runtime.lib()
+    gd, p = P.gd, P.gd.params
+    f = gd.gridfuncs
+    ws = (C.c_double * 2)(p.uu_wavespeed_at_OB, p.vv_wavespeed_at_OB)
+    finf = (C.c_double * 2)(0.0, 0.0)
+    calls = {
+        "rhs_eval": lambda: L.rhs_eval(C.byref(p), C.byref(gd.rfmstruct), f.auxevol_gfs, f.y_n_gfs, f.k_odd_gfs),
+        "apply_bcs_outerradiation_and_inner": lambda: L.apply_bcs_outerradiation_and_inner(
+            C.byref(p), C.byref(gd.bcstruct), gd.xx, ws, finf, f.y_n_gfs, f.k_odd_gfs),
+        "residual_all_points": lambda: L.residual_all_points(C.byref(p), C.byref(gd.rfmstruct), f.auxevol_gfs,
+                                                             f.y_n_gfs, f.diagnostic_output_gfs),
+        "L2_norm_of_gf": lambda: L.L2_norm_of_gf(C.byref(p), 0, 100.0, gd.xx, f.diagnostic_output_gfs),
+        "MoL_step_forward_in_time": lambda: L.MoL_step_forward_in_time(C.byref(gd), P.dt),
+    }
+    out = {}
+    for name, fn in calls.items():
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        out[name] = round(1e6 * (time.perf_counter() - t0) / n, 2)
+    return out
+
+
+def pmc_probe():
+    """A short run for rocprofv3 counter passes: 20 production steps of the 128x128x16 solve,
+    then 5 plain device copies of 64 MiB (torch), whose FETCH_SIZE / WRITE_SIZE per dispatch
+    calibrate the counters against a known byte count."""
+    import numpy as np
+    import torch
+    from nrpyelliptic_amd import config, runtime
+    P = runtime.Problem(config.equal_mass_boosted())
+    S = P.solver(steps_per_graph=10)
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    S.timed_steps(20)
+    a = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("pmc probe: 20 steps (80 substep launches) + 5 copies of 64 MiB", flush=True)
+
+
 def load_traffic():
     """HBM bytes per substep launch from the committed rocprofv3 PMC summary, scaled by the
     copy-kernel calibration recorded with it (null when absent)."""
@@ -106,7 +154,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--cpu-baseline-only", nargs=2, type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pmc-probe", action="store_true")
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe()
+        return
 
     if args.cpu_baseline_only:
         print(json.dumps(cpu_baseline(*args.cpu_baseline_only)))
@@ -141,6 +193,7 @@ def main():
     r = S.solve()
     # per-stage kernel durations (back-to-back launches, HIP events on the solver stream)
     stage_us = {f"stage{k}": round(S.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)}
+    dropin_us = dropin_call_us(P)
 
     cpu = None if args.no_cpu_baseline else run_cpu_baseline_subprocess()
     line = {
@@ -164,6 +217,7 @@ def main():
                      "kernel": "substep_kernel (fused RHS + radiation BC + RK4 stage update + ghost images)",
                      "us_per_launch": us_per_launch, "bytes_per_launch": BYTES_PER_GP_SUBSTEP * npts,
                      "stage_us": stage_us},
+        "dropin_call_us": dropin_us,
         "cpu_baseline": cpu,
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
                              "log10_l2": r.log10_l2, "converged": bool(r.converged),

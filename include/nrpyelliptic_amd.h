@@ -247,7 +247,8 @@ typedef struct {
   double log10_target;    /* stop when log10 L2 <= this (e.g. -10)                           */
   int max_steps;          /* give up after this many steps (NRPYELL_ENOTCONVERGED)          */
   double fp32_switch_log10;   /* precision 32: switch to FP64 at/below this log10 L2         */
-  double fp32_plateau_ratio;  /* precision 32: or when L2 shrinks by less than this factor  */
+  int fp32_plateau_steps;     /* precision 32: or after this many FP32 steps without a new
+                                 lowest residual (the FP32 state has reached its floor)      */
   double integration_radius;  /* L2 mask radius, 100 in the reference driver (main.c:150) */
   int tile_i0, tile_i1;   /* thread-block tile of the fused substep kernel (0 = default)    */
 } nrpyell_solve_options;

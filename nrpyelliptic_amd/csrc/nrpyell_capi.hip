@@ -505,8 +505,11 @@ struct nrpyell_solver {
   nrpyell_solve_options opt{};
   hipStream_t stream = nullptr;
   int tx = 32, ntile0 = 0, ntile1 = 0;
-  double *coef64 = nullptr;
-  float *coef32 = nullptr;
+  double *coef64 = nullptr;   // metric coefficients (residual kernel)
+  double *kc64 = nullptr;     // c^2 x coefficients (substep kernel)
+  float *kc32 = nullptr;
+  double *src64 = nullptr;    // psi | 0.125 c^2 AA^2 (substep kernel)
+  double *w2c = nullptr;      // L2 weights / c^4 (fused stage-1 check)
   double *geo64 = nullptr;
   float *geo32 = nullptr;
   double *w2d = nullptr;
@@ -517,7 +520,7 @@ struct nrpyell_solver {
   SolverDeviceState *dstate = nullptr;
   SolverDeviceState *hstate = nullptr;  // pinned
   // FP32 shadow state
-  float *y32 = nullptr, *ko32 = nullptr, *ke32 = nullptr, *rt32 = nullptr, *aux32 = nullptr;
+  float *y32 = nullptr, *ko32 = nullptr, *ke32 = nullptr, *rt32 = nullptr, *src32 = nullptr;
   hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
   int graph64_steps = 0, graph32_steps = 0;
   double graph32_target = 0.0;
@@ -525,15 +528,16 @@ struct nrpyell_solver {
   ~nrpyell_solver() {
     if (graph64) hipGraphExecDestroy(graph64);
     if (graph32) hipGraphExecDestroy(graph32);
-    hipFree(coef64); hipFree(coef32); hipFree(geo64); hipFree(geo32); hipFree(w2d); hipFree(partial);
+    hipFree(coef64); hipFree(kc64); hipFree(kc32); hipFree(src64); hipFree(w2c);
+    hipFree(geo64); hipFree(geo32); hipFree(w2d); hipFree(partial);
     hipFree(l2_out); hipFree(l2_count); hipFree(dstate);
     if (hstate) hipHostFree(hstate);
-    hipFree(y32); hipFree(ko32); hipFree(ke32); hipFree(rt32); hipFree(aux32);
+    hipFree(y32); hipFree(ko32); hipFree(ke32); hipFree(rt32); hipFree(src32);
     if (stream) hipStreamDestroy(stream);
   }
 
   template <typename T>
-  SubstepArgs<T> args(int stage, T *yn, T *ko, T *ke, T *rt, const T *aux, const T *coef, const T *geo) const {
+  SubstepArgs<T> args(int stage, T *yn, T *ko, T *ke, T *rt, const T *src, const T *coef, const T *geo) const {
     SubstepArgs<T> a;
     a.g = g;
     const MoL_gridfunctions_struct &gf = gd->gridfuncs;
@@ -546,7 +550,7 @@ struct nrpyell_solver {
     }
     a.yn = yn;
     a.rt = rt;
-    a.aux = aux;
+    a.src = src;
     a.coef = coef;
     a.geo = geo;
     a.cOB_u = (T)gd->params.uu_wavespeed_at_OB;
@@ -572,9 +576,9 @@ struct nrpyell_solver {
     MoL_gridfunctions_struct &f = gd->gridfuncs;
     for (int s = 1; s <= 4; s++) {
       SubstepArgs<double> a = args<double>(s, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs,
-                                           f.auxevol_gfs, coef64, geo64);
+                                           src64, kc64, geo64);
       if (s <= 2 && fuse_check) {  // stage 1 writes the partials, stage 2 reduces them
-        a.w2d = w2d;
+        a.w2d = w2c;
         a.l2_partial = partial;
         a.l2_count = l2_count;
         a.state = dstate;
@@ -588,7 +592,7 @@ struct nrpyell_solver {
   // of y_n from them before the state is handed back.
   void fill_ghosts(hipStream_t st) { launch_fill_ghost_planes<double>(g, gd->gridfuncs.y_n_gfs, 2, st); }
   void step32(hipStream_t st) {
-    for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, aux32, coef32, geo32), s, st);
+    for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, src32, kc32, geo32), s, st);
   }
   // residual of the current y_n (FP64 or the FP32 shadow) and a convergence-state update
   void check(hipStream_t st, bool from32, int steps_per_check, double target) {
@@ -617,7 +621,7 @@ extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
   o->log10_target = -10.0;
   o->max_steps = 200000;
   o->fp32_switch_log10 = -6.0;
-  o->fp32_plateau_ratio = 0.7;
+  o->fp32_plateau_steps = 200;
   o->integration_radius = 100.0;  // main.c:150
   o->tile_i0 = 0;
   o->tile_i1 = 0;
@@ -656,8 +660,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   const long n2 = (long)g.G0 * g.G1;
   s->coef64 = dalloc<double>(5 * n2);
-  s->coef32 = dalloc<float>(5 * n2);
-  launch_coef(g, &gd->rfmstruct, s->coef64, s->coef32, s->stream);
+  launch_coef(g, &gd->rfmstruct, s->coef64, nullptr, s->stream);
   std::vector<double> geo((size_t)6 * g.G1 * 4);
   host_outer_geometry(&gd->params, gd->xx, geo.data());
   s->geo64 = upload(geo.data(), geo.size());
@@ -666,6 +669,23 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   std::vector<double> w(n2);
   host_l2_weights(&gd->params, gd->xx, s->opt.integration_radius, w.data(), &s->volume);
   s->w2d = upload(w.data(), w.size());
+  // tables with the wave speed folded in (auxevol and rfm are read here, once per solver)
+  s->kc64 = dalloc<double>(5 * n2);
+  s->kc32 = dalloc<float>(5 * n2);
+  s->w2c = dalloc<double>(n2);
+  s->src64 = dalloc<double>(2 * g.ntot);
+  int *bad = dalloc<int>(1);
+  HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
+  launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->w2c, s->src64, bad, s->stream);
+  int hbad = 0;
+  HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  hipFree(bad);
+  if (hbad) {
+    set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
+                   "wavespeed_gf_all_points sets it");
+    return NRPYELL_EINVAL;
+  }
   const long nblk_res = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
   const long nblk_sub = (long)s->ntile0 * s->ntile1 * g.N2;
   s->partial = dalloc<double>(nblk_res > nblk_sub ? nblk_res : nblk_sub);
@@ -695,7 +715,7 @@ static void ensure_fp32(nrpyell_solver *s) {
   s->ko32 = dalloc<float>(2 * nt);
   s->ke32 = dalloc<float>(2 * nt);
   s->rt32 = dalloc<float>(2 * nt);
-  s->aux32 = dalloc<float>(3 * nt);
+  s->src32 = dalloc<float>(2 * nt);
 }
 
 // Capture `nsteps` RK4 steps with a residual check every check_every steps as one graph.
@@ -752,24 +772,41 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
   if (s->hstate->done == 0 && s->opt.precision == 32) {
     ensure_fp32(s);
     launch_convert_d2f(2 * nt, f.y_n_gfs, s->y32, s->stream);
-    launch_convert_d2f(3 * nt, f.auxevol_gfs, s->aux32, s->stream);
+    launch_convert_d2f(2 * nt, s->src64, s->src32, s->stream);
     if (!s->graph32 || s->graph32_target != s->opt.fp32_switch_log10) {
       if (s->graph32) hipGraphExecDestroy(s->graph32);
       s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
       s->graph32_target = s->opt.fp32_switch_log10;
     }
     reset_state(s);
-    double prev = res->log10_l2_initial;
-    const double plateau = log10(s->opt.fp32_plateau_ratio);
+    // The relaxation residual oscillates, so the FP32 phase ends on the lowest value seen so
+    // far: at the switch target, after fp32_plateau_steps steps without a new lowest value
+    // (FP32 rounding floor), or when it climbs 2 decades above the lowest (diverging).
+    double best = res->log10_l2_initial;
+    int since_best = 0, seen = 0;
     while (true) {
       HIPCHK(hipGraphLaunch(s->graph32, s->stream));
       read_state(s);
       const SolverDeviceState &h = *s->hstate;
       if (h.done || h.steps >= s->opt.max_steps) break;
-      // plateau: the last check improved by less than the ratio
-      if (h.nhist >= 2 && h.hist_log10[h.nhist - 1] - h.hist_log10[h.nhist - 2] > plateau) break;
-      if (h.last_log10 > prev + 3.0) break;  // FP32 diverging: hand over to FP64
-      prev = h.last_log10;
+      if (h.nhist > seen) {
+        for (int i = seen; i < h.nhist; i++) {
+          if (h.hist_log10[i] < best - 0.01) {
+            best = h.hist_log10[i];
+            since_best = 0;
+          } else {
+            since_best += s->opt.check_every;
+          }
+        }
+        seen = h.nhist;
+      } else if (h.last_log10 < best - 0.01) {  // history full: the graph's last check
+        best = h.last_log10;
+        since_best = 0;
+      } else {
+        since_best += s->opt.steps_per_graph;
+      }
+      if (since_best >= s->opt.fp32_plateau_steps) break;
+      if (h.last_log10 > best + 2.0) break;
     }
     steps = s->hstate->steps;
     res->fp32_steps = steps;
@@ -859,10 +896,10 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
   if (fuse) stage = 1;
   // the kernels write y_n only in stage 4; time on scratch copies so the state is untouched
   SubstepArgs<double> a =
-      s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, f.auxevol_gfs,
-                      s->coef64, s->geo64);
+      s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, s->src64,
+                      s->kc64, s->geo64);
   if (fuse) {
-    a.w2d = s->w2d;
+    a.w2d = s->w2c;
     a.l2_partial = s->partial;
     a.l2_count = s->l2_count;
     a.state = s->dstate;
@@ -1027,9 +1064,9 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
         nrpyell_solver *s = b->solvers[g.members[j]];
         MoL_gridfunctions_struct &f = s->gd->gridfuncs;
         SubstepArgs<double> a = s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs,
-                                                f.y_nplus1_running_total_gfs, f.auxevol_gfs, s->coef64, s->geo64);
+                                                f.y_nplus1_running_total_gfs, s->src64, s->kc64, s->geo64);
         if (stage <= 2) {
-          a.w2d = s->w2d;
+          a.w2d = s->w2c;
           a.l2_partial = s->partial;
           a.l2_count = s->l2_count;
           a.state = s->dstate;

@@ -485,6 +485,42 @@ void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st) {
   hipLaunchKernelGGL((convert_kernel<double, float>), dim3((n + 255) / 256), dim3(256), 0, st, n, in, out);
 }
 
+// Solver tables with the wave speed folded in. The wave speed of the reference depends on
+// (xx0,xx1) only (wavespeed_gf_all_points.c:19-41), so the substep kernel takes
+//   c^2 K_k          (the metric coefficients, per (i0,i1)),
+//   w / c^4          (L2 weights: the stage-1 stencil then gives c^2 R, R the residual),
+//   0.125 c^2 AA     (per point, next to psi),
+// and evaluates kv = c^2 (lap u + AA/(8 (psi+u)^7)) without reading c. A point whose wave speed
+// differs from that of its (i0,i1) column on plane NG raises *bad.
+__global__ void fold_coef_kernel(GridDims g, const double *aux, const double *coef, const double *w2d, double *kc64,
+                                 float *kc32, double *w2c) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, i1 = blockIdx.y;
+  if (i0 >= g.G0) return;
+  const long k2 = i0 + (long)g.G0 * i1, n2 = (long)g.G0 * g.G1;
+  const double c = aux[WAVESPEEDGF * g.ntot + idx3(g, i0, i1, NG)], c2 = c * c;
+  for (int k = 0; k < 5; k++) {
+    const double v = c2 * coef[k * n2 + k2];
+    kc64[k * n2 + k2] = v;
+    kc32[k * n2 + k2] = (float)v;
+  }
+  w2c[k2] = c2 > 0.0 ? w2d[k2] / (c2 * c2) : 0.0;
+}
+__global__ void fold_source_kernel(GridDims g, const double *aux, double *src, int *bad) {
+  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, i1 = blockIdx.y, i2 = blockIdx.z;
+  if (i0 >= g.G0) return;
+  const long p = idx3(g, i0, i1, i2), nt = g.ntot;
+  const double c = aux[WAVESPEEDGF * nt + p];
+  src[p] = aux[PSI_BACKGROUNDGF * nt + p];
+  src[nt + p] = 0.125 * (c * c) * aux[ADD_TIMES_AUUGF * nt + p];
+  const bool inside = i0 >= NG && i0 < g.N0 + NG && i1 >= NG && i1 < g.N1 + NG && i2 >= NG && i2 < g.N2 + NG;
+  if (inside && c != aux[WAVESPEEDGF * nt + idx3(g, i0, i1, NG)]) *bad = 1;
+}
+void launch_fold(const GridDims &g, const double *aux, const double *coef, const double *w2d, double *kc64,
+                 float *kc32, double *w2c, double *src, int *bad, hipStream_t st) {
+  hipLaunchKernelGGL(fold_coef_kernel, dim3((g.G0 + 63) / 64, g.G1), dim3(64), 0, st, g, aux, coef, w2d, kc64, kc32, w2c);
+  hipLaunchKernelGGL(fold_source_kernel, dim3((g.G0 + 63) / 64, g.G1, g.G2), dim3(64), 0, st, g, aux, src, bad);
+}
+
 // Ghost phi planes (i2 < NG, i2 >= N2+NG) of the first ngf gridfunctions of y from the
 // interior planes: every point takes its source's value (ghost_source; parity +1 for u and v).
 // Sources lie in interior planes, so no thread reads what another writes.

@@ -116,7 +116,7 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   const long p = valid ? plane + (long)i1 * s1 + i0 : 0;
   const long k2 = valid ? (long)i1 * s1 + i0 : 0;
   T zm[6], zp[6];
-  T vv = 0, psi = 1, AA = 0, c = 0, yn_u = 0, yn_v = 0, rt_u = 0, rt_v = 0, K[5] = {0, 0, 0, 0, 0};
+  T vv = 0, psi = 1, AAc = 0, yn_u = 0, yn_v = 0, rt_u = 0, rt_v = 0, K[5] = {0, 0, 0, 0, 0};
   double w2 = 0.0;
   if (valid) {
     // the i2 arm through the periodic map: the ghost planes are never read
@@ -126,14 +126,13 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
       zp[k] = a.in[(long)wrap2(g, i2 + k) * s2 + k2];
     }
     vv = a.in[nt + p];
-    AA = a.aux[ADD_TIMES_AUUGF * nt + p];
-    psi = a.aux[PSI_BACKGROUNDGF * nt + p];
-    c = a.aux[WAVESPEEDGF * nt + p];
+    psi = a.src[p];
+    AAc = a.src[nt + p];
 #pragma unroll
     for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
-    yn_u = a.yn[p];
-    yn_v = a.yn[nt + p];
-    if (STAGE != 1) {
+    if (STAGE != 1) {  // stage 1 updates its own input (y_n)
+      yn_u = a.yn[p];
+      yn_v = a.yn[nt + p];
       rt_u = a.rt[p];
       rt_v = a.rt[nt + p];
     }
@@ -183,16 +182,20 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
     fd_sums<T>(xm, xp, uu, s00, s0);
     fd_sums<T>(ym, yp, uu, s11, s1d);
     fd_sums<T>(zm, zp, uu, s22, s2d);
+    if (STAGE == 1) {
+      yn_u = uu;
+      yn_v = vv;
+    }
+    // kv = c^2 (lap u + AA^2/(8 (psi+u)^7)) (rhs_eval.c:110) with c^2 in the tables
     const T X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
-    T L = (T)0.125 * (AA / X7);
-    L = fma(K[0], s00, L);
-    L = fma(K[1], s11, L);
-    L = fma(K[2], s22, L);
-    L = fma(K[3], s1d, L);
-    L = fma(K[4], s0, L);
+    T kv = AAc / X7;
+    kv = fma(K[0], s00, kv);
+    kv = fma(K[1], s11, kv);
+    kv = fma(K[2], s22, kv);
+    kv = fma(K[3], s1d, kv);
+    kv = fma(K[4], s0, kv);
     const T ku = fma(-a.eta, uu, vv);  // rhs_eval.c:109
-    const T kv = (c * c) * L;          // rhs_eval.c:110
-    if (fuse_l2) l2acc = w2 * (double)L * (double)L;
+    if (fuse_l2) l2acc = w2 * (double)kv * (double)kv;  // w / c^4 x (c^2 R)^2
     if (has_bc && i0 == ilast) {
       s_rhs0[ty][0] = ku;
       s_rhs0[ty][1] = kv;

@@ -18,7 +18,7 @@ from .mlp import K0, Features, InitialGuessMLP
 from .runtime import DeviceArray
 
 
-def solve_batch(problems, target=-10.0, max_steps=40000, tiles=None):
+def solve_batch(problems, target=-10.0, max_steps=10000, tiles=None):
     B = runtime.BatchSolver(problems, tiles=tiles, log10_target=target, max_steps=max_steps)
     res, sec = B.solve()
     B.close()
@@ -26,28 +26,39 @@ def solve_batch(problems, target=-10.0, max_steps=40000, tiles=None):
 
 
 def generate_dataset(configs: List[config.PunctureConfig], rows_per_config=8192, batch=64, target=-10.0, seed=0,
-                     log=print):
+                     max_steps=10000, tuner=None, log=print):
     """Solve every configuration (batched, FP64, to `target`) and keep rows_per_config random
-    (features, t) rows of each solution in a device dataset."""
+    (features, t) rows of each converged solution in a device dataset. Configurations the
+    relaxation does not bring to `target` within max_steps (some corners of the parameter
+    space diverge, in the reference's C as well) are left out. Returns (X, T, rows, stats)."""
     N = len(configs) * rows_per_config
     X = DeviceArray(N * K0, np.uint16)
     T = DeviceArray(N, np.float32)
     row = 0
+    stats = {"converged": 0, "skipped": [], "steps": 0, "points": 0.0, "seconds": 0.0}
     t0 = time.time()
     for b0 in range(0, len(configs), batch):
         chunk = configs[b0:b0 + batch]
         probs = [runtime.Problem(c) for c in chunk]
-        res, sec = solve_batch(probs, target=target)
+        tiles = tuner.choose_all(probs) if tuner is not None else None
+        res, sec = solve_batch(probs, target=target, max_steps=max_steps, tiles=tiles)
         for k, (P, r) in enumerate(zip(probs, res)):
-            F = Features(P)
-            t = F.targets(P)
-            F.sample_into(t, rows_per_config, seed * 1000003 + b0 + k, X, T, row)
-            row += rows_per_config
+            if r.converged:
+                F = Features(P)
+                t = F.targets(P)
+                F.sample_into(t, rows_per_config, seed * 1000003 + b0 + k, X, T, row)
+                row += rows_per_config
+                stats["converged"] += 1
+                stats["steps"] += r.steps
+                stats["points"] += 4.0 * float(np.prod(P.cfg.Nxx)) * r.steps
+            else:
+                stats["skipped"].append(P.cfg.name)
             P.close()
+        stats["seconds"] += sec
         conv = sum(r.converged for r in res)
         log(f"dataset: solved {b0 + len(chunk)}/{len(configs)} ({conv}/{len(chunk)} converged, "
             f"{sec:.2f}s batch, mean steps {np.mean([r.steps for r in res]):.0f}, {time.time() - t0:.1f}s total)")
-    return X, T, N
+    return X, T, row, stats
 
 
 def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem):
@@ -62,21 +73,33 @@ def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem):
     return used, r0, r1
 
 
-def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, log=print):
-    """Per configuration: steps and seconds to `target` from the zero guess and from the NN
-    guess (with the fallback), single-configuration solver, FP64 or mixed precision."""
+def _solve(S):
+    try:
+        return S.solve()
+    except runtime.NrpyellError:  # NaN: reported as not converged
+        return None
+
+
+def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_steps=10000, log=print):
+    """Per configuration: steps and seconds to `target` from the zero guess (FP64) and from the
+    NN guess with the fallback (FP64 or mixed precision), single-configuration solver, and the
+    max |du| between the two converged states."""
     rows = []
     for k, cfg in enumerate(configs):
         P = runtime.Problem(cfg)
-        S = P.solver(log10_target=target)
+        S = P.solver(log10_target=target, max_steps=max_steps)
         P.set("y_n_gfs", np.zeros(2 * P.ntot))
-        rz = S.solve()
+        rz = _solve(S)
+        u_zero = P.get("y_n_gfs")[: P.ntot]
         used, r0, r1 = seed_with_fallback(net, P)
-        S2 = P.solver(log10_target=target, precision=precision)
-        rn = S2.solve()
-        rows.append({"config": cfg.name, "zero_steps": rz.steps, "zero_s": rz.seconds, "nn_steps": rn.steps,
-                     "nn_fp32_steps": rn.fp32_steps, "nn_s": rn.seconds, "nn_used": bool(used),
-                     "log10_res0_zero": r0, "log10_res0_nn": r1, "converged": bool(rz.converged and rn.converged)})
+        S2 = P.solver(log10_target=target, precision=precision, max_steps=max_steps)
+        rn = _solve(S2)
+        ok = rz is not None and rn is not None and bool(rz.converged and rn.converged)
+        du = float(np.max(np.abs(P.get("y_n_gfs")[: P.ntot] - u_zero))) if ok else None
+        rows.append({"config": cfg.name, "zero_steps": rz.steps if rz else None, "zero_s": rz.seconds if rz else None,
+                     "nn_steps": rn.steps if rn else None, "nn_fp32_steps": rn.fp32_steps if rn else None,
+                     "nn_s": rn.seconds if rn else None, "nn_used": bool(used), "log10_res0_zero": r0,
+                     "log10_res0_nn": r1, "max_du": du, "converged": ok})
         S.close()
         S2.close()
         P.close()
@@ -94,12 +117,18 @@ def main():
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--target", type=float, default=-10.0)
+    ap.add_argument("--max-steps", type=int, default=10000)
     ap.add_argument("--out", default="mlp_initial_guess.npz")
     ap.add_argument("--report", default=None)
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)  # noqa: E731
     train_cfgs = config.random_batch(args.train_configs, seed=1000)
-    X, T, N = generate_dataset(train_cfgs, rows_per_config=args.rows_per_config, target=args.target, log=log)
+    from .autotune import TileTuner
+    tuner = TileTuner(cache_path=None)
+    X, T, N, stats = generate_dataset(train_cfgs, rows_per_config=args.rows_per_config, target=args.target,
+                                      max_steps=args.max_steps, tuner=tuner, log=log)
+    log(f"dataset: {stats['converged']} converged solutions, {N} rows; batched solver "
+        f"{stats['points'] / max(stats['seconds'], 1e-9):.3e} gp-substeps/s; tiles {tuner.table}")
     net = InitialGuessMLP(seed=7)
     t0 = time.time()
     losses = net.train(X, T, N, batch=args.batch, steps=args.steps, lr=args.lr, seed=11)
@@ -111,6 +140,10 @@ def main():
     rows32 = evaluate(net, eval_cfgs, precision=32, target=args.target, log=log)
 
     def summary(rows):
+        n_all = len(rows)
+        rows = [r for r in rows if r["converged"]]
+        if not rows:
+            return {"n": 0, "n_all": n_all}
         z = np.array([r["zero_s"] for r in rows])
         n = np.array([r["nn_s"] for r in rows])
         zs = np.array([r["zero_steps"] for r in rows])
@@ -118,9 +151,10 @@ def main():
         return {"mean_zero_steps": float(zs.mean()), "mean_nn_steps": float(ns.mean()),
                 "speedup_time_mean": float(z.sum() / n.sum()), "speedup_steps_mean": float(zs.sum() / ns.sum()),
                 "nn_used_frac": float(np.mean([r["nn_used"] for r in rows])),
-                "all_converged": bool(all(r["converged"] for r in rows))}
+                "max_du": float(max(r["max_du"] for r in rows)), "n": len(rows), "n_all": n_all}
 
-    rep = {"train_configs": args.train_configs, "rows": N, "final_loss": float(losses[-50:].mean()),
+    rep = {"train_configs": args.train_configs, "converged_train": stats["converged"], "rows": N,
+           "final_loss": float(losses[-50:].mean()),
            "fp64": summary(rows64), "mixed": summary(rows32)}
     log(json.dumps(rep))
     if args.report:

@@ -28,7 +28,7 @@ ERRORS = {-1: "EINVAL", -2: "EHIP", -3: "EGHOSTMAP", -4: "ENOTCONVERGED", -5: "E
 class solve_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("check_every", C.c_int), ("steps_per_graph", C.c_int),
                 ("log10_target", C.c_double), ("max_steps", C.c_int), ("fp32_switch_log10", C.c_double),
-                ("fp32_plateau_ratio", C.c_double), ("integration_radius", C.c_double),
+                ("fp32_plateau_steps", C.c_int), ("integration_radius", C.c_double),
                 ("tile_i0", C.c_int), ("tile_i1", C.c_int)]
 
 

@@ -51,19 +51,27 @@ def test_fp64_solve_matches_reference_64x64x4():
 
 
 def test_batched_equals_single_solves():
+    """Four configurations on uneven grids (96x128x16, 128x128x16, 128x96x16, 128x128x8) with
+    three tile widths in one batch. seed42_1 does not converge (the reference's C diverges on it
+    too), so every solve is capped at 3000 steps: converged configurations must stop at the
+    same step with the same state, the others must hold the same state after 3000 steps."""
     cfgs = config.random_batch(4, seed=42)
+    tiles = [16, 32, 64, 32]
     probs = [runtime.Problem(c) for c in cfgs]
     singles = []
-    for P in probs:
-        S = P.solver(log10_target=-8.0)
+    for P, tx in zip(probs, tiles):
+        S = P.solver(log10_target=-8.0, max_steps=3000, tile_i0=tx)
         r = S.solve()
-        singles.append((r.steps, P.get("y_n_gfs")))
+        singles.append((r.converged, r.steps, P.get("y_n_gfs")))
         S.close()
         P.set("y_n_gfs", np.zeros(2 * P.ntot))
-    B = runtime.BatchSolver(probs, log10_target=-8.0)
+    assert sum(c for c, _, _ in singles) >= 2
+    B = runtime.BatchSolver(probs, tiles=tiles, log10_target=-8.0, max_steps=3000)
     res, _ = B.solve()
-    for P, r, (steps, y) in zip(probs, res, singles):
-        assert r.converged and r.steps == steps
+    for P, r, (conv, steps, y) in zip(probs, res, singles):
+        assert bool(r.converged) == bool(conv)
+        if conv:
+            assert r.steps == steps
         assert np.array_equal(P.get("y_n_gfs"), y)  # same kernels, same order: bit-identical
     B.close()
     for P in probs:
@@ -80,8 +88,12 @@ def test_mixed_precision_reaches_threshold_and_matches_fp64():
     r = S32.solve()
     assert r.converged and r.log10_l2 <= -10.0 and r.fp32_steps > 0
     u = P.get("y_n_gfs")[: P.ntot]
-    assert np.max(np.abs(u - u64)) <= 1e-10 * 10  # both at residual 1e-10; see DESIGN.md
-    print(f"fp64 {r64.steps} steps {r64.seconds:.3f}s; mixed {r.steps} steps ({r.fp32_steps} fp32) {r.seconds:.3f}s")
+    du = float(np.max(np.abs(u - u64)))
+    print(f"fp64 {r64.steps} steps {r64.seconds:.3f}s; mixed {r.steps} steps ({r.fp32_steps} fp32) "
+          f"{r.seconds:.3f}s; max|du| {du:.2e}")
+    # two solves stopped at L2 <= 1e-10 on different trajectories differ by their remaining
+    # error, which the every-step FP64 finish keeps at the 1e-10 level
+    assert du <= 1e-9
     S.close()
     S32.close()
     P.close()
