@@ -250,7 +250,9 @@ typedef struct {
   int fp32_plateau_steps;     /* precision 32: or after this many FP32 steps without a new
                                  lowest residual (the FP32 state has reached its floor)      */
   double integration_radius;  /* L2 mask radius, 100 in the reference driver (main.c:150) */
-  int tile_i0, tile_i1;   /* thread-block tile of the fused substep kernel (0 = default)    */
+  int tile_i0;            /* i0 width of the fused substep kernel's 256-thread tile: 16, 32
+                             or 64 (0 = default); the i1 height is 256 / tile_i0            */
+  int planes;             /* i2 planes per thread: 1, or 2 with tile_i0 32/64 (0 = 1)     */
 } nrpyell_solve_options;
 
 typedef struct {
@@ -285,11 +287,12 @@ int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10
 
 /* Batched FP64 solver: n configurations, each with its own griddata (any grid shape), relaxed
  * together -- one launch per RK4 substep per tile width covers every configuration, and each
- * configuration stops on its own residual. tile_i0[k] (16/32/64, or 0 for the default) is the
- * tile width chosen for configuration k (by the autotuner). */
+ * configuration stops on its own residual. tile_i0[k] and planes[k] (see
+ * nrpyell_solve_options; 0 or a NULL array for the default) are the tile chosen for
+ * configuration k (by the autotuner); configurations with the same tile share a launch. */
 typedef struct nrpyell_batch nrpyell_batch;
-int nrpyell_batch_create(griddata_struct **griddatas, const double *dts, const int *tile_i0, int n,
-                         const nrpyell_solve_options *opt, nrpyell_batch **out);
+int nrpyell_batch_create(griddata_struct **griddatas, const double *dts, const int *tile_i0, const int *planes,
+                         int n, const nrpyell_solve_options *opt, nrpyell_batch **out);
 /* Relax every configuration to opt->log10_target; res has n entries. */
 int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, double *seconds);
 int nrpyell_batch_destroy(nrpyell_batch *b);

@@ -1,11 +1,12 @@
 """Tile-width autotuner for the fused RK4 substep kernel.
 
-The fused kernel puts i0 across the lanes of a 256-thread block (tile TX × 256/TX); which
-width keeps HBM busiest depends on N0 (ragged last tile, BC-tile share) and on the number
-of blocks per launch. This module times one fused substep (all four stages, the production
-launch) for each candidate width on the configuration's own grid and caches the winner per
-(N0, N1, N2, precision) in a JSON file, so a batch of configurations is grouped by width
-without re-timing shapes it has already seen.
+The fused kernel puts i0 across the lanes of a 256-thread block (tile TX x 256/TX) and gives
+each thread one or two i2 planes; which shape keeps HBM busiest depends on N0 (ragged last
+tile, BC-tile share) and on the number of blocks per launch. This module times the fused
+substep (stage 1 with the residual check, the production launch) for each candidate shape on
+the configuration's own grid and caches the winner per (N0, N1, N2, precision) in a JSON
+file, so a batch of configurations is grouped by shape without re-timing shapes it has
+already seen.
 """
 from __future__ import annotations
 
@@ -15,7 +16,7 @@ from typing import Dict, Iterable, Optional, Sequence
 
 from .runtime import Problem, Solver
 
-CANDIDATES = (16, 32, 64)
+CANDIDATES = ((16, 1), (32, 1), (64, 1), (32, 2), (64, 2))  # (tile_i0, planes)
 _DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_cache.json")
 
 
@@ -39,25 +40,27 @@ class TileTuner:
             with open(cache_path) as f:
                 self.table = json.load(f)
 
-    def measure(self, prob: Problem, precision: int = 64) -> Dict[int, float]:
-        """Microseconds per fused substep launch for every candidate width."""
+    def measure(self, prob: Problem, precision: int = 64) -> Dict[tuple, float]:
+        """Microseconds per fused substep launch for every candidate shape."""
         out = {}
-        for tx in self.candidates:
-            s = Solver(prob, precision=precision, tile_i0=tx)
+        for tx, pl in self.candidates:
+            s = Solver(prob, precision=precision, tile_i0=tx, planes=pl)
             try:
-                out[tx] = s.time_substep(0, self.launches)
+                out[(tx, pl)] = s.time_substep(0, self.launches)
             finally:
                 s.close()
         return out
 
-    def choose(self, prob: Problem, precision: int = 64) -> int:
+    def choose(self, prob: Problem, precision: int = 64) -> tuple:
+        """(tile_i0, planes) of the fastest shape for this grid."""
         k = _key(prob, precision)
         if k not in self.table:
             us = self.measure(prob, precision)
             best = min(us, key=us.get)
-            self.table[k] = {"tile_i0": best, "us_per_launch": {str(t): v for t, v in us.items()}}
+            self.table[k] = {"tile_i0": best[0], "planes": best[1],
+                             "us_per_launch": {f"{t[0]}x{t[1]}": v for t, v in us.items()}}
             self._save()
-        return int(self.table[k]["tile_i0"])
+        return int(self.table[k]["tile_i0"]), int(self.table[k]["planes"])
 
     def choose_all(self, problems: Iterable[Problem], precision: int = 64):
         return [self.choose(p, precision) for p in problems]

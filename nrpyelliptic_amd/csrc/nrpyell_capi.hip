@@ -504,7 +504,7 @@ struct nrpyell_solver {
   double dt = 0.0;
   nrpyell_solve_options opt{};
   hipStream_t stream = nullptr;
-  int tx = 32, ntile0 = 0, ntile1 = 0;
+  int tx = 32, pl = 1, ntile0 = 0, ntile1 = 0;
   double *coef64 = nullptr;   // metric coefficients (residual kernel)
   double *kc64 = nullptr;     // c^2 x coefficients (substep kernel)
   float *kc32 = nullptr;
@@ -559,6 +559,7 @@ struct nrpyell_solver {
     a.dt = (T)dt;
     a.done = &dstate->done;
     a.tx = tx;
+    a.pl = pl;
     a.ntile0 = ntile0;
     a.ntile1 = ntile1;
     a.w2d = nullptr;
@@ -624,7 +625,7 @@ extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
   o->fp32_plateau_steps = 200;
   o->integration_radius = 100.0;  // main.c:150
   o->tile_i0 = 0;
-  o->tile_i1 = 0;
+  o->planes = 0;
 }
 
 extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell_solve_options *opt,
@@ -649,11 +650,17 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     return NRPYELL_EGHOSTMAP;
   }
   int tx = s->opt.tile_i0 > 0 ? s->opt.tile_i0 : 32;
-  if (tx != 16 && tx != 32 && tx != 64 && tx != 128 && tx != 256) {
-    set_last_error("tile_i0 must be 16, 32, 64, 128 or 256");
+  if (tx != 16 && tx != 32 && tx != 64) {
+    set_last_error("tile_i0 must be 16, 32 or 64");
+    return NRPYELL_EINVAL;
+  }
+  const int pl = s->opt.planes > 0 ? s->opt.planes : 1;
+  if (pl != 1 && !(pl == 2 && tx >= 32)) {
+    set_last_error("planes must be 1, or 2 with tile_i0 32 or 64");
     return NRPYELL_EINVAL;
   }
   s->tx = tx;
+  s->pl = pl;
   const int ty = 256 / tx;
   s->ntile0 = (g.N0 + tx - 1) / tx;
   s->ntile1 = (g.N1 + ty - 1) / ty;
@@ -687,7 +694,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     return NRPYELL_EINVAL;
   }
   const long nblk_res = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
-  const long nblk_sub = (long)s->ntile0 * s->ntile1 * g.N2;
+  const long nblk_sub = (long)s->ntile0 * s->ntile1 * (g.N2 / pl);
   s->partial = dalloc<double>(nblk_res > nblk_sub ? nblk_res : nblk_sub);
   s->l2_out = dalloc<double>(1);
   s->l2_count = dalloc<unsigned int>(1);
@@ -822,12 +829,14 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       s->graph64_steps = s->opt.steps_per_graph;
     }
     reset_state(s);
+    // every step's stage 1 checks the state it starts from: a converged solve took `steps`
+    // steps (the count when its check passed), an unconverged one all `nchecks` launched
     while (true) {
       HIPCHK(hipGraphLaunch(s->graph64, s->stream));
       read_state(s);
-      if (s->hstate->done || steps + s->hstate->steps >= s->opt.max_steps) break;
+      if (s->hstate->done || steps + s->hstate->nchecks >= s->opt.max_steps) break;
     }
-    steps += s->hstate->steps;
+    steps += s->hstate->done ? s->hstate->steps : s->hstate->nchecks;
   }
   s->fill_ghosts(s->stream);
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -990,7 +999,7 @@ struct nrpyell_batch {
   nrpyell_solve_options opt{};
   hipStream_t stream = nullptr;
   struct Group {
-    int tx = 32;
+    int tx = 32, pl = 1;
     std::vector<int> members;
     SubstepArgs<double> *d_args[5] = {};
     int *d_block_cfg = nullptr, *d_block_first = nullptr;
@@ -1016,7 +1025,8 @@ struct nrpyell_batch {
   void step(hipStream_t st) {
     for (int stage = 1; stage <= 4; stage++)
       for (auto &g : groups)
-        launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, stage, st);
+        launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl, stage,
+                                       st);
   }
   void read_heads() {
     hipLaunchKernelGGL(gather_state_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_states, n, d_heads);
@@ -1025,8 +1035,8 @@ struct nrpyell_batch {
   }
 };
 
-extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, const int *tile_i0, int n,
-                                    const nrpyell_solve_options *opt, nrpyell_batch **out) {
+extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, const int *tile_i0, const int *planes,
+                                    int n, const nrpyell_solve_options *opt, nrpyell_batch **out) {
   if (n <= 0) { set_last_error("empty batch"); return NRPYELL_EINVAL; }
   auto b = std::make_unique<nrpyell_batch>();
   b->n = n;
@@ -1034,25 +1044,27 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
   b->opt.precision = 64;
   for (int k = 0; k < n; k++) {
     nrpyell_solve_options o = b->opt;
-    o.tile_i0 = tile_i0 ? tile_i0[k] : 0;
+    o.tile_i0 = tile_i0 ? tile_i0[k] : b->opt.tile_i0;
+    o.planes = planes ? planes[k] : b->opt.planes;
     nrpyell_solver *s = nullptr;
     const int rc = nrpyell_solver_create(gds[k], dts[k], &o, &s);
     if (rc != NRPYELL_OK) return rc;
     b->solvers.push_back(s);
   }
   HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  std::map<int, std::vector<int>> by_tx;
-  for (int k = 0; k < n; k++) by_tx[b->solvers[k]->tx].push_back(k);
-  for (auto &kv : by_tx) {
+  std::map<std::pair<int, int>, std::vector<int>> by_tile;
+  for (int k = 0; k < n; k++) by_tile[{b->solvers[k]->tx, b->solvers[k]->pl}].push_back(k);
+  for (auto &kv : by_tile) {
     nrpyell_batch::Group g;
-    g.tx = kv.first;
+    g.tx = kv.first.first;
+    g.pl = kv.first.second;
     g.members = kv.second;
     const int m = (int)g.members.size();
     std::vector<int> first(m), bcfg;
     for (int j = 0; j < m; j++) {
       nrpyell_solver *s = b->solvers[g.members[j]];
       first[j] = (int)bcfg.size();
-      const int nb = s->ntile0 * s->ntile1 * s->g.N2;
+      const int nb = s->ntile0 * s->ntile1 * (s->g.N2 / s->pl);
       for (int q = 0; q < nb; q++) bcfg.push_back(j);
     }
     g.nblocks = (int)bcfg.size();

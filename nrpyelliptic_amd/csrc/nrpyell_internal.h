@@ -50,6 +50,7 @@ struct SubstepArgs {
   T eta, dt;
   const int *done;  // nonzero: converged, every launch returns at once
   int tx;           // tile width along i0 (threads), tile height along i1 = blockDim/tx
+  int pl;           // i2 planes per thread (1 or 2)
   int ntile0, ntile1;
   // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
   // is c^2 times the residual. Non-null l2_partial enables it.
@@ -75,7 +76,7 @@ template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 template <typename T>
 void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
-                            int tx, int stage, hipStream_t st);
+                            int tx, int pl, int stage, hipStream_t st);
 long launch_residual_l2(const GridDims &g, const void *u, bool u_is_float, const double *aux, const double *coef,
                         const double *w2d, double *partial, const int *done, double *res_out, hipStream_t st);
 long launch_l2_partial(const GridDims &g, const double *gf, const double *w2d, double *partial, hipStream_t st);

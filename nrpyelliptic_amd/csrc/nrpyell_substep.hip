@@ -62,178 +62,208 @@ __device__ __forceinline__ void fd_sums(const T m[6], const T p[6], T uu, T &s2,
   s1 = fma((T)FD1_W5, p[5] - m[5], s1);
 }
 
-template <typename T, int STAGE, int TX>
-// One block's work: tile `tile` (t0 + ntile0*t1) of i2 plane `plane`; `bid` is the block's
-// index among the nblk blocks of this configuration (for the L2 partial sums).
-__device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int tile, const int plane_idx,
+// One block's work: tile `tile` (t0 + ntile0*t1) of the PL consecutive i2 planes
+// NG + PL*pgroup ...; `bid` is the block's index among the nblk blocks of this configuration
+// (for the L2 partial sums). With PL = 2 every thread updates the same (i0,i1) on two planes:
+// the metric coefficients and weights are read once, and the i2 arms share 12 planes instead
+// of 2 x 11.
+template <typename T, int STAGE, int TX, int PL>
+__device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int tile, const int pgroup,
                                              const unsigned bid, const unsigned nblk) {
   constexpr int TY = 256 / TX;
-  constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG;
-  __shared__ T su[BY * BX];  // u on this i2 plane: tile + halos
-  __shared__ T sv[BY * BX];  // v likewise (BC tiles only)
-  __shared__ T s_rhs0[TY][2], s_A[TY][2][6];
+  constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG, BOX = BX * BY, NR = (BOX + 255) / 256;
+  constexpr int NW = PL + 2 * NG;        // i2 window: planes i2a-NG .. i2a+PL-1+NG
+  constexpr int NV = (12 * BY + 255) / 256;  // v columns ilast-6..ilast+5, all box rows
+  static_assert(12 * TY <= 256 && PL * 10 * TY <= 256, "one BC work item per thread");
+  __shared__ T su[PL][NR * 256];  // u on this block's planes: tile + halos (padded: no store guards)
+  __shared__ T sv[PL][BOX];       // v, the 12 columns the BC reads (BC tiles only)
+  __shared__ T s_rhs0[PL][TY][2], s_A[PL][TY][2][6];
   __shared__ T s_geo[6][TY][4];
   __shared__ double s_red[4];
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
   const int t0 = tile % a.ntile0, t1 = tile / a.ntile0;
-  const int i2 = NG + plane_idx;
+  const int i2a = NG + PL * pgroup;
   const int i0s = NG + t0 * TX, i1s = NG + t1 * TY;
   const int i0 = i0s + tx, i1 = i1s + ty;
   const bool valid = i0 < g.N0 + NG && i1 < g.N1 + NG;
   const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
-  const long plane = (long)i2 * s2;
   const int ilast = g.N0 + NG - 1;  // last interior i0; the radiation BC starts next to it
   const bool has_bc = (i0s <= ilast) && (ilast < i0s + TX);
   const int bx0 = ilast - (i0s - NG);  // box column of the last interior point (BC tiles)
   const bool fuse_l2 = (STAGE == 1) && (a.l2_partial != nullptr);
   const int done = a.done ? *a.done : 0;
 
-  // ---- global loads: LDS staging of the plane, the i2 arm and the pointwise inputs.
-  // u: the tile with its i0/i1 halos; BC tiles also need the halo rows (corners included) of
-  // the six columns ilast..ilast+5 (i1 derivative at the outer layers). v (BC tiles only):
-  // columns ilast-6..ilast+5 of the tile rows (shifted i0 derivatives) and the halo rows of
-  // columns ilast..ilast+5.
-  T stage_u[(BX * BY + 255) / 256], stage_v[(BX * BY + 255) / 256];
-#pragma unroll
-  for (int r = 0; r < (BX * BY + 255) / 256; r++) {
-    const int k = tid + 256 * r;
-    const int bx = k % BX, by = k / BX;
-    const int j0 = i0s - NG + bx, j1 = i1s - NG + by;
-    const bool halo_row = by < NG || by >= TY + NG;
-    const bool corner = (bx < NG || bx >= TX + NG) && halo_row;
-    const bool bc_col = has_bc && bx >= bx0 && bx <= bx0 + NG;
-    const bool need_v = has_bc && bx >= bx0 - NG - 1 && bx <= bx0 + NG && (!halo_row || bx >= bx0);
-    stage_u[r] = 0;
-    stage_v[r] = 0;
-    if (k < BX * BY && j0 < g.G0 && j1 < g.G1) {
-      const long q = plane + (long)j1 * s1 + j0;
-      if (!corner || bc_col) stage_u[r] = a.in[q];
-      if (need_v) stage_v[r] = a.in[nt + q];
-    }
-  }
-  const long p = valid ? plane + (long)i1 * s1 + i0 : 0;
-  const long k2 = valid ? (long)i1 * s1 + i0 : 0;
-  T zm[6], zp[6];
-  T vv = 0, psi = 1, AAc = 0, yn_u = 0, yn_v = 0, rt_u = 0, rt_v = 0, K[5] = {0, 0, 0, 0, 0};
-  double w2 = 0.0;
-  if (valid) {
-    // the i2 arm through the periodic map: the ghost planes are never read
-#pragma unroll
-    for (int k = 1; k <= 5; k++) {
-      zm[k] = a.in[(long)wrap2(g, i2 - k) * s2 + k2];
-      zp[k] = a.in[(long)wrap2(g, i2 + k) * s2 + k2];
-    }
-    vv = a.in[nt + p];
-    psi = a.src[p];
-    AAc = a.src[nt + p];
-#pragma unroll
-    for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
-    if (STAGE != 1) {  // stage 1 updates its own input (y_n)
-      yn_u = a.yn[p];
-      yn_v = a.yn[nt + p];
-      rt_u = a.rt[p];
-      rt_v = a.rt[nt + p];
-    }
-    if (fuse_l2) w2 = a.w2d[k2];
-  }
-  // outer-layer state for the BC stage update (work item = (row, gf, layer 1..5))
-  T od_yn = 0, od_rt = 0;
-  long od_p = -1;
-  if (has_bc && tid < 10 * TY) {
-    const int row = tid / 10, rem = tid % 10, gf = rem / 5, pt = rem % 5 + 1;
-    const int j1 = i1s + row;
-    if (j1 < g.N1 + NG) {
-      od_p = plane + (long)j1 * s1 + (ilast + pt) + gf * nt;
-      od_yn = a.yn[od_p];
-      if (STAGE != 1) od_rt = a.rt[od_p];
-    }
-  }
-  if (has_bc)
-    for (int k = tid; k < 24 * TY; k += 256) {
-      const int pt = k / (4 * TY), row = (k / 4) % TY, comp = k % 4;
-      const int j1 = i1s + row;
-      s_geo[pt][row][comp] = j1 < g.G1 ? a.geo[((long)pt * g.G1 + j1) * 4 + comp] : (T)1;
-    }
-  if (done) return;  // converged: uniform across the grid
-#pragma unroll
-  for (int r = 0; r < (BX * BY + 255) / 256; r++) {
-    const int k = tid + 256 * r;
-    if (k < BX * BY) {
-      su[k] = stage_u[r];
-      if (has_bc) sv[k] = stage_v[r];
-    }
-  }
-  __syncthreads();
-
-  // ---- 1+3+4: interior RHS, stage update, parity images
-  double l2acc = 0.0;
-  if (valid) {
-    const int cb = (ty + NG) * BX + tx + NG;
-    const T uu = su[cb];
-    T xm[6], xp[6], ym[6], yp[6];
-#pragma unroll
-    for (int k = 1; k <= 5; k++) {
-      xm[k] = su[cb - k]; xp[k] = su[cb + k];
-      ym[k] = su[cb - k * BX]; yp[k] = su[cb + k * BX];
-    }
-    T s00, s0, s11, s1d, s22, s2d;
-    fd_sums<T>(xm, xp, uu, s00, s0);
-    fd_sums<T>(ym, yp, uu, s11, s1d);
-    fd_sums<T>(zm, zp, uu, s22, s2d);
-    if (STAGE == 1) {
-      yn_u = uu;
-      yn_v = vv;
-    }
-    // kv = c^2 (lap u + AA^2/(8 (psi+u)^7)) (rhs_eval.c:110) with c^2 in the tables
-    const T X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
-    T kv = AAc / X7;
-    kv = fma(K[0], s00, kv);
-    kv = fma(K[1], s11, kv);
-    kv = fma(K[2], s22, kv);
-    kv = fma(K[3], s1d, kv);
-    kv = fma(K[4], s0, kv);
-    const T ku = fma(-a.eta, uu, vv);  // rhs_eval.c:109
-    if (fuse_l2) l2acc = w2 * (double)kv * (double)kv;  // w / c^4 x (c^2 R)^2
-    if (has_bc && i0 == ilast) {
-      s_rhs0[ty][0] = ku;
-      s_rhs0[ty][1] = kv;
-    }
-    T o_u, o_v;
-    stage_update<T, STAGE>(ku, a.dt, yn_u, rt_u, o_u);
-    stage_update<T, STAGE>(kv, a.dt, yn_v, rt_v, o_v);
-    if (STAGE != 4) {
-      a.rt[p] = rt_u;
-      a.rt[nt + p] = rt_v;
-    }
-    a.out[p] = o_u;
-    a.out[nt + p] = o_v;
-    for_each_image_in_planes(g, i0, i1, i2, [&](long d) {
-      a.out[d] = o_u;
-      a.out[nt + d] = o_v;
-    });
-  }
-
-  // ---- 2: radiation BC on the outer layers of this tile's rows. With the radiation term
-  // A_pt = -c (df/dr + f/r) at layer pt (pt = 0: the last interior point), the reference's
-  // innermost-first recurrence (apply_bcs_outerradiation_and_inner.c:369-408)
-  //   rhs_pt = A_pt + (r_{pt-1}/r_pt)^3 (rhs_{pt-1} - A_{pt-1})
-  // telescopes to rhs_pt = A_pt + (r_0/r_pt)^3 (rhs_0 - A_0), so the layers are independent
-  // (1/r and (r_0/r_pt)^3 come precomputed in the geometry table).
+  // ---- global loads: LDS staging of the planes, the i2 window and the pointwise inputs.
+  // Every load is unconditional, from an index clamped into the grid: values the stencils do
+  // not reach (box corners of non-BC tiles, columns or rows past a ragged edge) are loaded
+  // but never read, and the loads compile to straight-line code the compiler can keep in
+  // flight across the per-plane barriers.
+  // Issue order is plane-major (vmcnt counts in order): the BC geometry and edge values, then
+  // everything plane 0 needs, then plane 1's box and pointwise values, so plane 0 computes and
+  // stores while plane 1's loads are in flight.
+  constexpr int NGEO = (24 * TY + 255) / 256;
+  T geo_r[NGEO];
+  // BC phase A work item of this thread: (row, gf, layer 0..5); the pt = 0 stencil reaches
+  // column ilast-6, left of the box when the tile holds only the last interior column
+  const int a_row = min(tid / 12, TY - 1), a_gf = (tid % 12) / 6;
+  T edge[PL];
   if (has_bc) {
-    if (tid < 12 * TY) {
-      const int row = tid / 12, rem = tid % 12, gf = rem / 6, pt = rem % 6;
+#pragma unroll
+    for (int r = 0; r < NGEO; r++) {
+      const int k = min(tid + 256 * r, 24 * TY - 1);
+      const int pt = k / (4 * TY), row = (k / 4) % TY, comp = k % 4;
+      geo_r[r] = a.geo[((long)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp];
+    }
+#pragma unroll
+    for (int q = 0; q < PL; q++)
+      edge[q] = a.in[a_gf * nt + (long)(i2a + q) * s2 + (long)min(i1s + a_row, g.G1 - 1) * s1 + max(ilast - NG - 1, 0)];
+  }
+  const int i0c = min(i0, g.N0 + NG - 1), i1c = min(i1, g.N1 + NG - 1);
+  const long k2 = (long)i1c * s1 + i0c;
+  T stage_u[PL][NR], stage_v[PL][NV];
+  int v_slot[NV];
+  T zw[NW];  // u on the i2 window at (i0,i1), through the periodic map
+  T vv[PL], psi[PL], AAc[PL], yn_u[PL], yn_v[PL], rt_u[PL], rt_v[PL], K[5];
+  double w2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < PL; q++) {
+    // plane q's box: the tile with its i0/i1 halos
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int k = tid + 256 * r;
+      const int bx = k % BX, by = k / BX;
+      const int j0 = min(i0s - NG + bx, g.G0 - 1), j1 = min(i1s - NG + by, g.G1 - 1);
+      stage_u[q][r] = a.in[(long)(i2a + q) * s2 + (long)j1 * s1 + j0];
+    }
+    // v for the BC: columns ilast-6..ilast+5 (box columns bx0-6..bx0+5) of every box row
+    if (has_bc) {
+#pragma unroll
+      for (int r = 0; r < NV; r++) {
+        const int k = tid + 256 * r, c = k % 12, by = min(k / 12, BY - 1);
+        const int bx = bx0 - NG - 1 + c;
+        const int j0 = max(ilast - NG - 1 + c, 0), j1 = min(i1s - NG + by, g.G1 - 1);
+        v_slot[r] = (k < 12 * BY && bx >= 0) ? by * BX + bx : -1;
+        stage_v[q][r] = a.in[nt + (long)(i2a + q) * s2 + (long)j1 * s1 + j0];
+      }
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int jj = 0; jj < NW; jj++) zw[jj] = a.in[(long)wrap2(g, i2a - NG + jj) * s2 + k2];
+#pragma unroll
+      for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
+      if (fuse_l2) w2 = a.w2d[k2];
+    }
+    const long p = (long)(i2a + q) * s2 + k2;
+    vv[q] = a.in[nt + p];
+    psi[q] = a.src[p];
+    AAc[q] = a.src[nt + p];
+    yn_u[q] = 0; yn_v[q] = 0; rt_u[q] = 0; rt_v[q] = 0;
+    if (STAGE != 1) {  // stage 1 updates its own input (y_n)
+      yn_u[q] = a.yn[p];
+      yn_v[q] = a.yn[nt + p];
+      rt_u[q] = a.rt[p];
+      rt_v[q] = a.rt[nt + p];
+    }
+  }
+  // outer-layer state for the BC stage update (work item = (plane, row, gf, layer 1..5))
+  const int od_q = min(tid / (10 * TY), PL - 1), od_rem = tid % (10 * TY);
+  const int od_row = od_rem / 10, od_gf = (od_rem % 10) / 5, od_pt = od_rem % 5 + 1;
+  const int od_j1 = min(i1s + od_row, g.G1 - 1);
+  const long od_p = (long)(i2a + od_q) * s2 + (long)od_j1 * s1 + (ilast + od_pt) + od_gf * nt;
+  const bool od_on = has_bc && tid < PL * 10 * TY && i1s + od_row < g.N1 + NG;
+  T od_yn = 0, od_rt = 0;
+  if (has_bc) {
+    od_yn = a.yn[od_p];
+    if (STAGE != 1) od_rt = a.rt[od_p];
+  }
+  if (done) return;  // converged: uniform across the grid
+
+  // ---- per plane: stage the box in LDS, then 1+3+4 (interior RHS, stage update, parity
+  // images) and the BC radiation terms of that plane. The planes are pipelined: plane q's
+  // barrier waits only for plane q's box loads, so plane q+1's loads are still in flight while
+  // plane q computes and stores.
+  if (has_bc) {
+#pragma unroll
+    for (int r = 0; r < NGEO; r++)
+      if (tid + 256 * r < 24 * TY) (&s_geo[0][0][0])[tid + 256 * r] = geo_r[r];
+  }
+  double l2acc = 0.0;
+  const int cb = (ty + NG) * BX + tx + NG;
+#pragma unroll
+  for (int q = 0; q < PL; q++) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) su[q][tid + 256 * r] = stage_u[q][r];
+    if (has_bc) {
+#pragma unroll
+      for (int r = 0; r < NV; r++)
+        if (v_slot[r] >= 0) sv[q][v_slot[r]] = stage_v[q][r];
+    }
+    __syncthreads();
+    if (valid) {
+      const int i2 = i2a + q;
+      const long p = (long)i2 * s2 + k2;
+      const T uu = zw[NG + q];
+      T xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+#pragma unroll
+      for (int k = 1; k <= 5; k++) {
+        xm[k] = su[q][cb - k]; xp[k] = su[q][cb + k];
+        ym[k] = su[q][cb - k * BX]; yp[k] = su[q][cb + k * BX];
+        zm[k] = zw[NG + q - k]; zp[k] = zw[NG + q + k];
+      }
+      T s00, s0, s11, s1d, s22, s2d;
+      fd_sums<T>(xm, xp, uu, s00, s0);
+      fd_sums<T>(ym, yp, uu, s11, s1d);
+      fd_sums<T>(zm, zp, uu, s22, s2d);
+      if (STAGE == 1) {
+        yn_u[q] = uu;
+        yn_v[q] = vv[q];
+      }
+      // kv = c^2 (lap u + AA^2/(8 (psi+u)^7)) (rhs_eval.c:110) with c^2 in the tables
+      const T X = psi[q] + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
+      T kv = AAc[q] / X7;
+      kv = fma(K[0], s00, kv);
+      kv = fma(K[1], s11, kv);
+      kv = fma(K[2], s22, kv);
+      kv = fma(K[3], s1d, kv);
+      kv = fma(K[4], s0, kv);
+      const T ku = fma(-a.eta, uu, vv[q]);  // rhs_eval.c:109
+      if (fuse_l2) l2acc += w2 * (double)kv * (double)kv;  // w / c^4 x (c^2 R)^2
+      if (has_bc && i0 == ilast) {
+        s_rhs0[q][ty][0] = ku;
+        s_rhs0[q][ty][1] = kv;
+      }
+      T o_u, o_v;
+      stage_update<T, STAGE>(ku, a.dt, yn_u[q], rt_u[q], o_u);
+      stage_update<T, STAGE>(kv, a.dt, yn_v[q], rt_v[q], o_v);
+      if (STAGE != 4) {
+        a.rt[p] = rt_u[q];
+        a.rt[nt + p] = rt_v[q];
+      }
+      a.out[p] = o_u;
+      a.out[nt + p] = o_v;
+      for_each_image_in_planes(g, i0, i1, i2, [&](long d) {
+        a.out[d] = o_u;
+        a.out[nt + d] = o_v;
+      });
+    }
+    // ---- 2a: BC radiation terms of plane q. With A_pt = -c (df/dr + f/r) at layer pt
+    // (pt = 0: the last interior point), the reference's innermost-first recurrence
+    // (apply_bcs_outerradiation_and_inner.c:369-408)
+    //   rhs_pt = A_pt + (r_{pt-1}/r_pt)^3 (rhs_{pt-1} - A_{pt-1})
+    // telescopes to rhs_pt = A_pt + (r_0/r_pt)^3 (rhs_0 - A_0), so the layers are independent
+    // (1/r and (r_0/r_pt)^3 come precomputed in the geometry table).
+    if (has_bc && tid < 12 * TY) {
+      const int row = a_row, gf = a_gf, pt = tid % 6;
       if (i1s + row < g.N1 + NG) {
-        const T *box = gf == 0 ? su : sv;
+        const T *box = gf == 0 ? su[q] : sv[q];
         const int bx = bx0 + pt, by = row + NG, sh = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
         T d0 = 0, d1 = 0;
 #pragma unroll
         for (int j = 0; j < 11; j++) {
-          // column ilast-6 (pt = 0, j = 0) lies left of the box when the tile holds only
-          // the last interior column: read it from global memory
           const int col = bx + (j - 5 - sh);
-          const T f0v = col >= 0 ? box[by * BX + col] : a.in[gf * nt + plane + (long)(i1s + row) * s1 + (i0s - NG + col)];
+          const T f0v = col >= 0 ? box[by * BX + col] : edge[q];  // col < 0: column ilast-6
           d0 = fma((T)c_fd1_shift[sh][j], f0v, d0);
           d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * BX + bx], d1);
         }
@@ -241,20 +271,25 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
         d1 *= (T)g.invdx1;
         const T dr = s_geo[pt][row][1] * d0 + s_geo[pt][row][2] * d1;
         const T cc = gf == 0 ? a.cOB_u : a.cOB_v;
-        s_A[row][gf][pt] = -cc * (dr + box[by * BX + bx] * s_geo[pt][row][0]);
+        s_A[q][row][gf][pt] = -cc * (dr + box[by * BX + bx] * s_geo[pt][row][0]);
       }
     }
+  }
+
+  // ---- 2b: stage update of the outer layers (all planes) once their radiation terms and
+  // the last interior point's RHS are in LDS
+  if (has_bc) {
     __syncthreads();
-    if (od_p >= 0) {
-      const int row = tid / 10, rem = tid % 10, gf = rem / 5, pt = rem % 5 + 1;
-      const T rhs = fma(s_geo[pt][row][3], s_rhs0[row][gf] - s_A[row][gf][0], s_A[row][gf][pt]);
+    if (od_on) {
+      const T rhs = fma(s_geo[od_pt][od_row][3], s_rhs0[od_q][od_row][od_gf] - s_A[od_q][od_row][od_gf][0],
+                        s_A[od_q][od_row][od_gf][od_pt]);
       T o;
       T rt_ = od_rt;
       stage_update<T, STAGE>(rhs, a.dt, od_yn, rt_, o);
       if (STAGE != 4) a.rt[od_p] = rt_;
       a.out[od_p] = o;
-      T *outg = a.out + gf * nt;
-      for_each_image_in_planes(g, ilast + pt, i1s + row, i2, [&](long d) { outg[d] = o; });
+      T *outg = a.out + od_gf * nt;
+      for_each_image_in_planes(g, ilast + od_pt, i1s + od_row, i2a + od_q, [&](long d) { outg[d] = o; });
     }
   }
 
@@ -295,71 +330,85 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
 
 // XCD-aware block order. Workgroups are dispatched to the 8 XCDs round robin (workgroup L to
 // XCD L % 8), and each XCD has its own L2. Logical block q = (L % 8) * (n / 8) + L / 8 gives
-// every XCD one contiguous run of logical blocks; with the i2 plane fastest in q, the blocks
-// that read a tile column's i2 arm, and the neighbouring tiles that read its i1 halos, share
-// one L2. Blocks past the last multiple of 8 keep their index.
+// every XCD one contiguous run of logical blocks; with the i2 plane group fastest in q, the
+// blocks that read a tile column's i2 arm, and the neighbouring tiles that read its i1 halos,
+// share one L2. Blocks past the last multiple of 8 keep their index.
 __device__ __forceinline__ unsigned xcd_logical(unsigned L, unsigned n) {
   const unsigned per = n / 8;
   return L < 8 * per ? (L % 8) * per + L / 8 : L;
 }
 
-// One configuration: a 1-D grid of ntile0*ntile1*N2 blocks.
-template <typename T, int STAGE, int TX>
+// One configuration: a 1-D grid of ntile0*ntile1*(N2/PL) blocks.
+template <typename T, int STAGE, int TX, int PL>
 __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
   const unsigned n = gridDim.x, q = xcd_logical(blockIdx.x, n);
-  const unsigned N2 = (unsigned)a.g.N2;
-  substep_body<T, STAGE, TX>(a, q / N2, q % N2, q, n);
+  const unsigned ng = (unsigned)(a.g.N2 / PL);
+  substep_body<T, STAGE, TX, PL>(a, q / ng, q % ng, q, n);
 }
 
 // Many configurations (uneven grids) in one launch: logical block q works for configuration
 // block_cfg[q], as that configuration's block q - block_first[cfg]. Configurations that have
 // converged return at once (their own done flag).
-template <typename T, int STAGE, int TX>
+template <typename T, int STAGE, int TX, int PL>
 __global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<T> *__restrict__ cfgs,
                                                               const int *__restrict__ block_cfg,
                                                               const int *__restrict__ block_first) {
   const unsigned q = xcd_logical(blockIdx.x, gridDim.x);
   const int c = block_cfg[q];
   const SubstepArgs<T> a = cfgs[c];
-  const unsigned local = q - block_first[c], N2 = (unsigned)a.g.N2;
-  substep_body<T, STAGE, TX>(a, local / N2, local % N2, local, a.ntile0 * a.ntile1 * N2);
+  const unsigned local = q - block_first[c], ng = (unsigned)(a.g.N2 / PL);
+  substep_body<T, STAGE, TX, PL>(a, local / ng, local % ng, local, a.ntile0 * a.ntile1 * ng);
 }
 
 template <typename T, int STAGE>
 static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
-  dim3 grd(a.ntile0 * a.ntile1 * a.g.N2), blk(256);
+  dim3 grd(a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(256);
+  if (a.pl == 2) {
+    switch (a.tx) {
+      case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64, 2>), grd, blk, 0, st, a); break;
+      default: hipLaunchKernelGGL((substep_kernel<T, STAGE, 32, 2>), grd, blk, 0, st, a); break;
+    }
+    return;
+  }
   switch (a.tx) {
-    case 16: hipLaunchKernelGGL((substep_kernel<T, STAGE, 16>), grd, blk, 0, st, a); break;
-    case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64>), grd, blk, 0, st, a); break;
-    default: hipLaunchKernelGGL((substep_kernel<T, STAGE, 32>), grd, blk, 0, st, a); break;
+    case 16: hipLaunchKernelGGL((substep_kernel<T, STAGE, 16, 1>), grd, blk, 0, st, a); break;
+    case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64, 1>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((substep_kernel<T, STAGE, 32, 1>), grd, blk, 0, st, a); break;
   }
 }
 
 template <typename T, int STAGE>
 static void launch_stage_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first,
-                                 int nblocks, int tx, hipStream_t st) {
+                                 int nblocks, int tx, int pl, hipStream_t st) {
   dim3 grd(nblocks), blk(256);
+  if (pl == 2) {
+    switch (tx) {
+      case 64: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 64, 2>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+      default: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 32, 2>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    }
+    return;
+  }
   switch (tx) {
-    case 16: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 16>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
-    case 64: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 64>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
-    default: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 32>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    case 16: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 16, 1>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    case 64: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 64, 1>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    default: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 32, 1>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
   }
 }
 
 template <typename T>
 void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
-                            int tx, int stage, hipStream_t st) {
+                            int tx, int pl, int stage, hipStream_t st) {
   switch (stage) {
-    case 1: launch_stage_batched<T, 1>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
-    case 2: launch_stage_batched<T, 2>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
-    case 3: launch_stage_batched<T, 3>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
-    default: launch_stage_batched<T, 4>(cfgs, block_cfg, block_first, nblocks, tx, st); break;
+    case 1: launch_stage_batched<T, 1>(cfgs, block_cfg, block_first, nblocks, tx, pl, st); break;
+    case 2: launch_stage_batched<T, 2>(cfgs, block_cfg, block_first, nblocks, tx, pl, st); break;
+    case 3: launch_stage_batched<T, 3>(cfgs, block_cfg, block_first, nblocks, tx, pl, st); break;
+    default: launch_stage_batched<T, 4>(cfgs, block_cfg, block_first, nblocks, tx, pl, st); break;
   }
 }
 template void launch_substep_batched<double>(const SubstepArgs<double> *, const int *, const int *, int, int, int,
-                                             hipStream_t);
+                                             int, hipStream_t);
 template void launch_substep_batched<float>(const SubstepArgs<float> *, const int *, const int *, int, int, int,
-                                            hipStream_t);
+                                            int, hipStream_t);
 
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st) {

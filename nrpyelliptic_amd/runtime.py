@@ -29,7 +29,7 @@ class solve_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("check_every", C.c_int), ("steps_per_graph", C.c_int),
                 ("log10_target", C.c_double), ("max_steps", C.c_int), ("fp32_switch_log10", C.c_double),
                 ("fp32_plateau_steps", C.c_int), ("integration_radius", C.c_double),
-                ("tile_i0", C.c_int), ("tile_i1", C.c_int)]
+                ("tile_i0", C.c_int), ("planes", C.c_int)]
 
 
 class solve_result(C.Structure):
@@ -76,7 +76,8 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_timed_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.nrpyell_batch_create.argtypes = [C.POINTER(C.POINTER(abi.griddata_struct)), C.POINTER(C.c_double),
-                                       C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options), C.POINTER(vp)]
+                                       C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options),
+                                       C.POINTER(vp)]
     L.nrpyell_batch_solve.argtypes = [vp, C.POINTER(solve_result), C.POINTER(C.c_double)]
     L.nrpyell_batch_destroy.argtypes = [vp]
     _LIB = L
@@ -340,7 +341,8 @@ class Solver:
 class BatchSolver:
     """Relaxes many configurations (uneven grids) together on one GPU: one launch per RK4
     substep per tile width covers the whole batch; each configuration stops at its own
-    residual target. `tiles` gives each configuration's tile width (from the autotuner)."""
+    residual target. `tiles` gives each configuration's tile as an i0 width or a
+    (width, planes) pair (from the autotuner)."""
 
     def __init__(self, problems, tiles=None, **opts):
         self.problems = list(problems)
@@ -348,9 +350,11 @@ class BatchSolver:
         self.opts = default_options(**opts)
         gds = (C.POINTER(abi.griddata_struct) * n)(*[C.pointer(p.gd) for p in self.problems])
         dts = (C.c_double * n)(*[p.dt for p in self.problems])
-        tl = (C.c_int * n)(*[int(t) for t in (tiles or [0] * n)])
+        pairs = [t if isinstance(t, (tuple, list)) else (t, 0) for t in (tiles or [0] * n)]
+        tl = (C.c_int * n)(*[int(t[0]) for t in pairs])
+        pl = (C.c_int * n)(*[int(t[1]) for t in pairs])
         self.h = C.c_void_p()
-        check(lib().nrpyell_batch_create(gds, dts, tl, n, C.byref(self.opts), C.byref(self.h)), "batch_create")
+        check(lib().nrpyell_batch_create(gds, dts, tl, pl, n, C.byref(self.opts), C.byref(self.h)), "batch_create")
 
     def solve(self):
         n = len(self.problems)

@@ -134,12 +134,14 @@ def test_mol_step_matches_oracle(pair, nsteps):
     assert rel_maxnorm(got, st["y_n"]) <= 1e-12
 
 
-def test_solver_matches_oracle_trajectory(pair):
-    """The graph-captured solver for 200 steps equals the per-step drop-in path and the oracle."""
+@pytest.mark.parametrize("tile", [(32, 1), (16, 1), (64, 2), (32, 2)], ids=lambda t: f"tx{t[0]}_pl{t[1]}")
+def test_solver_matches_oracle_trajectory(pair, tile):
+    """The graph-captured solver for 200 steps (every tile shape the kernel has, one and two
+    i2 planes per thread) equals the oracle's reference-order RK4 at every point."""
     P, O = pair
     use_oracle_inputs(P, O)
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
-    S = P.solver(check_every=10, steps_per_graph=50)
+    S = P.solver(check_every=10, steps_per_graph=50, tile_i0=tile[0], planes=tile[1])
     S.steps(200)
     got = P.get("y_n_gfs")
     st = O.new_state()

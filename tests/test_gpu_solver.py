@@ -52,15 +52,15 @@ def test_fp64_solve_matches_reference_64x64x4():
 
 def test_batched_equals_single_solves():
     """Four configurations on uneven grids (96x128x16, 128x128x16, 128x96x16, 128x128x8) with
-    three tile widths in one batch. seed42_1 does not converge (the reference's C diverges on it
+    four tile shapes in one batch. seed42_1 does not converge (the reference's C diverges on it
     too), so every solve is capped at 3000 steps: converged configurations must stop at the
     same step with the same state, the others must hold the same state after 3000 steps."""
     cfgs = config.random_batch(4, seed=42)
-    tiles = [16, 32, 64, 32]
+    tiles = [(16, 1), (32, 2), (64, 2), (32, 1)]
     probs = [runtime.Problem(c) for c in cfgs]
     singles = []
     for P, tx in zip(probs, tiles):
-        S = P.solver(log10_target=-8.0, max_steps=3000, tile_i0=tx)
+        S = P.solver(log10_target=-8.0, max_steps=3000, tile_i0=tx[0], planes=tx[1])
         r = S.solve()
         singles.append((r.converged, r.steps, P.get("y_n_gfs")))
         S.close()
