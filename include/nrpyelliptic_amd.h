@@ -319,10 +319,11 @@ int nrpyell_mlp_train(nrpyell_mlp *m, const void *x, const float *t, long N, int
 long nrpyell_mlp_feature_rows(const griddata_struct *gd);
 /* device features x [rows x 64] (BF16), per-row u scale and grid index of a griddata */
 int nrpyell_mlp_features(const griddata_struct *gd, void *x, double *scale, long *idx);
-/* device training targets t = u/scale from griddata's y_n at the feature rows */
+/* device training targets t = log(u/scale) from griddata's y_n at the feature rows */
 int nrpyell_mlp_targets(const griddata_struct *gd, const double *scale, const long *idx, float *t);
-/* y_n <- network guess: u at interior and pure outer points, v = eta u, parity images */
-int nrpyell_mlp_seed(nrpyell_mlp *m, griddata_struct *gd);
+/* y_n <- network guess: u at interior and pure outer points, smoothed by smooth_passes
+ * binomial passes along i0, i1, i2 (0..64; k_odd_gfs is scratch), v = eta u, parity images */
+int nrpyell_mlp_seed(nrpyell_mlp *m, griddata_struct *gd, int smooth_passes);
 /* n_take uniform random rows of a device (x, t) set copied into (x_dst, t_dst) */
 int nrpyell_mlp_sample_rows(const void *x_src, const float *t_src, long n_src, long n_take, unsigned long long seed,
                             void *x_dst, float *t_dst);

@@ -846,6 +846,10 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
   res->seconds = std::chrono::duration<double>(t1 - t0).count();
   res->converged = s->hstate->done == 1;
   if (s->hstate->done == 2) { set_last_error("residual became NaN"); return NRPYELL_ENAN; }
+  if (s->hstate->done == 3) {
+    set_last_error("residual diverged (4 decades above its lowest value)");
+    return NRPYELL_ENOTCONVERGED;
+  }
   if (!res->converged) { set_last_error("max_steps reached before the residual target"); return NRPYELL_ENOTCONVERGED; }
   return NRPYELL_OK;
 }
@@ -1002,13 +1006,14 @@ struct nrpyell_batch {
     int tx = 32, pl = 1;
     std::vector<int> members;
     SubstepArgs<double> *d_args[5] = {};
-    int *d_block_cfg = nullptr, *d_block_first = nullptr;
-    int nblocks = 0;
+    int *d_block_cfg = nullptr, *d_block_first = nullptr;  // sized for every member
+    int nblocks = 0;                                       // blocks of the members in the graph
   };
   std::vector<Group> groups;
   SolverDeviceState **d_states = nullptr;
   double *d_heads = nullptr, *h_heads = nullptr;
   hipGraphExec_t graph = nullptr;
+  int in_graph = 0;  // configurations the current graph launches blocks for
   ~nrpyell_batch() {
     if (graph) hipGraphExecDestroy(graph);
     for (auto &g : groups) {
@@ -1025,8 +1030,43 @@ struct nrpyell_batch {
   void step(hipStream_t st) {
     for (int stage = 1; stage <= 4; stage++)
       for (auto &g : groups)
-        launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl, stage,
-                                       st);
+        if (g.nblocks > 0)
+          launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl,
+                                         stage, st);
+  }
+  // Block maps over the configurations with active[k] set, and a graph of steps_per_graph
+  // steps over them: converged configurations drop out of the launches.
+  void rebuild(const std::vector<char> &active) {
+    in_graph = 0;
+    for (auto &g : groups) {
+      const int m = (int)g.members.size();
+      std::vector<int> first(m, 0), bcfg;
+      for (int j = 0; j < m; j++) {
+        const int k = g.members[j];
+        if (!active[k]) continue;
+        in_graph++;
+        nrpyell_solver *s = solvers[k];
+        first[j] = (int)bcfg.size();
+        const int nb = s->ntile0 * s->ntile1 * (s->g.N2 / s->pl);
+        for (int q = 0; q < nb; q++) bcfg.push_back(j);
+      }
+      if (!g.d_block_cfg) {  // first build: every member is active, the buffers get their full size
+        g.d_block_cfg = dalloc<int>(bcfg.size() > 0 ? bcfg.size() : 1);
+        g.d_block_first = dalloc<int>(m);
+      }
+      g.nblocks = (int)bcfg.size();
+      if (g.nblocks > 0)
+        HIPCHK(hipMemcpyAsync(g.d_block_cfg, bcfg.data(), sizeof(int) * bcfg.size(), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(g.d_block_first, first.data(), sizeof(int) * m, hipMemcpyHostToDevice, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    if (graph) HIPCHK(hipGraphExecDestroy(graph));
+    hipGraph_t gr;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < opt.steps_per_graph; k++) step(stream);
+    HIPCHK(hipStreamEndCapture(stream, &gr));
+    HIPCHK(hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(gr));
   }
   void read_heads() {
     hipLaunchKernelGGL(gather_state_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, d_states, n, d_heads);
@@ -1060,16 +1100,6 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
     g.pl = kv.first.second;
     g.members = kv.second;
     const int m = (int)g.members.size();
-    std::vector<int> first(m), bcfg;
-    for (int j = 0; j < m; j++) {
-      nrpyell_solver *s = b->solvers[g.members[j]];
-      first[j] = (int)bcfg.size();
-      const int nb = s->ntile0 * s->ntile1 * (s->g.N2 / s->pl);
-      for (int q = 0; q < nb; q++) bcfg.push_back(j);
-    }
-    g.nblocks = (int)bcfg.size();
-    g.d_block_cfg = upload(bcfg.data(), bcfg.size());
-    g.d_block_first = upload(first.data(), first.size());
     for (int stage = 1; stage <= 4; stage++) {
       std::vector<SubstepArgs<double>> args(m);
       for (int j = 0; j < m; j++) {
@@ -1096,12 +1126,7 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
   b->d_states = upload(st.data(), st.size());
   b->d_heads = dalloc<double>(4 * n);
   HIPCHK(hipHostMalloc((void **)&b->h_heads, sizeof(double) * 4 * n, hipHostMallocDefault));
-  hipGraph_t graph;
-  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  for (int k = 0; k < b->opt.steps_per_graph; k++) b->step(b->stream);
-  HIPCHK(hipStreamEndCapture(b->stream, &graph));
-  HIPCHK(hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0));
-  HIPCHK(hipGraphDestroy(graph));
+  b->rebuild(std::vector<char>(n, 1));
   *out = b.release();
   return NRPYELL_OK;
 }
@@ -1109,15 +1134,21 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
 extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, double *seconds) {
   const auto t0 = std::chrono::steady_clock::now();
   for (auto *s : b->solvers) reset_state(s);
+  if (b->in_graph != b->n) b->rebuild(std::vector<char>(b->n, 1));
   int launched = 0;
+  std::vector<char> active(b->n, 1);
   while (true) {
     HIPCHK(hipGraphLaunch(b->graph, b->stream));
     launched += b->opt.steps_per_graph;
     b->read_heads();
-    bool all = true;
-    for (int k = 0; k < b->n; k++)
-      if (b->h_heads[4 * k] == 0.0) all = false;
-    if (all || launched >= b->opt.max_steps) break;
+    int n_active = 0;
+    for (int k = 0; k < b->n; k++) {
+      active[k] = b->h_heads[4 * k] == 0.0;
+      n_active += active[k];
+    }
+    if (n_active == 0 || launched >= b->opt.max_steps) break;
+    // compact when a quarter of the launched configurations have finished
+    if (4 * n_active <= 3 * b->in_graph) b->rebuild(active);
   }
   for (auto *s : b->solvers) s->fill_ghosts(b->stream);
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -1128,14 +1159,15 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
     nrpyell_solve_result &r = res[k];
     memset(&r, 0, sizeof(r));
     r.converged = b->h_heads[4 * k] == 1.0;
-    r.steps = r.converged ? (int)b->h_heads[4 * k + 1] : launched;
+    // converged, NaN or diverged: the steps when its check decided; running: all launched
+    r.steps = b->h_heads[4 * k] != 0.0 ? (int)b->h_heads[4 * k + 1] : launched;
     r.log10_l2 = b->h_heads[4 * k + 2];
     r.seconds = *seconds;
     if (b->h_heads[4 * k] == 2.0) rc = NRPYELL_ENAN;
     else if (!r.converged && rc == NRPYELL_OK) rc = NRPYELL_ENOTCONVERGED;
   }
   if (rc == NRPYELL_ENAN) set_last_error("a configuration's residual became NaN");
-  if (rc == NRPYELL_ENOTCONVERGED) set_last_error("max_steps reached before every configuration converged");
+  if (rc == NRPYELL_ENOTCONVERGED) set_last_error("some configurations diverged or reached max_steps");
   return rc;
 }
 

@@ -11,14 +11,43 @@
 
 // Device-resident convergence state of one solve.
 struct SolverDeviceState {
-  int done;          // 0 running, 1 converged, 2 NaN
+  int done;          // 0 running, 1 converged, 2 NaN, 3 diverged
   int steps;         // RK4 steps completed at the last residual check
   int nhist;
   int nchecks;       // residual checks made by fused stage-1 kernels
   double last_log10;
+  int have_best;
+  double best_log10;  // lowest residual seen
   int hist_step[NRPYELL_HIST];
   double hist_log10[NRPYELL_HIST];
 };
+
+// A residual this many decades above the lowest one seen ends the solve as diverged: the
+// relaxation diverges for some puncture parameters (the reference's C as well), and a batch
+// should not spend max_steps on them. Transient rises of converging solves stay below one
+// decade.
+#define NRPYELL_DIVERGE_DECADES 4.0
+
+// Bookkeeping of one residual check (log10 L2 = l) taken after st->steps steps: history,
+// done = 1 at the target, 2 on NaN, 3 on divergence.
+__device__ inline void record_check(SolverDeviceState *st, double l, double target) {
+  st->last_log10 = l;
+  if (st->nhist < NRPYELL_HIST) {
+    st->hist_step[st->nhist] = st->steps;
+    st->hist_log10[st->nhist] = l;
+    st->nhist++;
+  }
+  if (!(l == l)) {
+    st->done = 2;
+    return;
+  }
+  if (!st->have_best || l < st->best_log10) {
+    st->best_log10 = l;
+    st->have_best = 1;
+  }
+  if (l <= target) st->done = 1;
+  else if (l > st->best_log10 + NRPYELL_DIVERGE_DECADES) st->done = 3;
+}
 
 struct Punctures {
   double m[2];

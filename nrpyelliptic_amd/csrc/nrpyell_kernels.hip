@@ -270,14 +270,7 @@ __global__ void __launch_bounds__(256) l2_finalize_kernel(const double *partial,
     if (log10_out) *log10_out = l;
     if (st) {
       st->steps += steps_per_check;
-      st->last_log10 = l;
-      if (st->nhist < NRPYELL_HIST) {
-        st->hist_step[st->nhist] = st->steps;
-        st->hist_log10[st->nhist] = l;
-        st->nhist++;
-      }
-      if (!(l == l)) st->done = 2;  // NaN: stop, reported as NRPYELL_ENAN
-      else if (l <= target) st->done = 1;
+      record_check(st, l, target);
     }
   }
 }

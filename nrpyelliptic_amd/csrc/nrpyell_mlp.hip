@@ -203,7 +203,9 @@ __global__ void zero_kernel(long n, float *x) {
 // With b the SinhSymTP focus (bScale), r = |X|, r_n = |X - x_n|:
 //   point: X/(r+b), Y/(r+b), Z/(r+b), r/(r+b), log(1+r/b)/10, b/(r_0+b), b/(r_1+b)
 //   configuration: m_0, m_1, D/20, 10 P_0, 10 P_1, S_0/m_0^2, S_1/m_1^2, b/10
-// (25 numbers, zero-padded to 32). The network predicts t = 50 u (r+b), so u = t/(50 (r+b)).
+// (23 numbers, zero-padded to 64). The network predicts t = log(50 u (r+b)): u > 0 everywhere
+// (lap u = -AA^2/(8 psi^7) <= 0 with u -> 0 far away), and u (r+b) stays within a few
+// decades over the grid, so t is O(1) at every point; u = exp(t)/(50 (r+b)).
 // ---------------------------------------------------------------------------------------
 struct FeatureParams {
   double AMAX, w, b, D;
@@ -267,7 +269,7 @@ __global__ void seed_state_kernel(GridDims g, long npts, const float *__restrict
                                   const long *__restrict__ idx, double eta, double *__restrict__ yn) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
-  const double u = (double)t[k] * scale[k];
+  const double u = exp((double)t[k]) * scale[k];
   const long p = idx[k];
   yn[p] = u;
   yn[g.ntot + p] = eta * u;
@@ -278,11 +280,41 @@ __global__ void seed_state_kernel(GridDims g, long npts, const float *__restrict
   });
 }
 
-// u / scale at the listed points: the network's training target from a solved state.
+// One binomial smoothing sub-pass of u along direction DIR over the interior points,
+// out = (u[-1] + 2 u + u[+1]) / 4 (phi neighbours through the periodic map), pure outer points
+// copied; every point writes its interior-plane parity images, so the next sub-pass reads
+// current ghosts. It removes the grid-scale part of the network's pointwise error (BF16
+// activations round independently at neighbouring points), which the Laplacian in the
+// residual would otherwise amplify by 1/dx^2.
+template <int DIR>
+__global__ void smooth_kernel(GridDims g, const double *__restrict__ in, double *__restrict__ out) {
+  const int i0 = NRPYELL_NG + blockIdx.x * blockDim.x + threadIdx.x;
+  const int i1 = NRPYELL_NG + blockIdx.y, i2 = NRPYELL_NG + blockIdx.z;
+  if (i0 >= g.G0) return;
+  const long p = idx3(g, i0, i1, i2);
+  double v = in[p];
+  if (i0 < g.N0 + NRPYELL_NG) {
+    long pm, pp;
+    if (DIR == 0) { pm = p - 1; pp = p + 1; }
+    else if (DIR == 1) { pm = p - g.G0; pp = p + g.G0; }
+    else { pm = idx3(g, i0, i1, wrap2(g, i2 - 1)); pp = idx3(g, i0, i1, wrap2(g, i2 + 1)); }
+    v = 0.25 * (in[pm] + in[pp]) + 0.5 * v;
+  }
+  out[p] = v;
+  for_each_image_in_planes(g, i0, i1, i2, [&](long d) { out[d] = v; });
+}
+
+__global__ void v_from_u_kernel(long n, double eta, double *y) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[n + k] = eta * y[k];
+}
+
+// log(u / scale) at the listed points: the network's training target from a solved state
+// (u / scale floored at 1e-8).
 __global__ void target_kernel(long npts, const double *__restrict__ yn, const double *__restrict__ scale,
                               const long *__restrict__ idx, float *__restrict__ t) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < npts) t[k] = (float)(yn[idx[k]] / scale[k]);
+  if (k < npts) t[k] = (float)log(fmax(yn[idx[k]] / scale[k], 1e-8));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -601,9 +633,15 @@ extern "C" int nrpyell_mlp_targets(const griddata_struct *gd, const double *scal
   return NRPYELL_OK;
 }
 
-// Seed griddata's y_n from the network: u = t scale at interior and pure outer points,
-// v = eta u (the steady state of du/dt = v - eta u), parity images written in the ghosts.
-extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd) {
+// Seed griddata's y_n from the network: u = e^t scale at interior and pure outer points,
+// smoothed by smooth_passes binomial passes (each along i0, i1, i2), v = eta u (the steady
+// state of du/dt = v - eta u), parity images written in every ghost point. k_odd_gfs is the
+// scratch of the smoothing.
+extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd, int smooth_passes) {
+  if (smooth_passes < 0 || smooth_passes > 64) {
+    set_last_error("mlp_seed: smooth_passes must be in [0, 64]");
+    return NRPYELL_EINVAL;
+  }
   const GridDims g = dims_from_params(&gd->params);
   const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
   bf16_t *X = mlp_alloc<bf16_t>(rows * MLP_K0);
@@ -616,8 +654,21 @@ extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd) {
     const int M = (int)((rows - r0) < chunk ? (rows - r0) : chunk);
     nrpyell_mlp_forward(s, X + r0 * MLP_K0, M, y + r0);
   }
+  double *yn = gd->gridfuncs.y_n_gfs, *tmp = gd->gridfuncs.k_odd_gfs;
   hipLaunchKernelGGL(seed_state_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, g, n, y, scale, idx,
-                     gd->params.eta_damping, gd->gridfuncs.y_n_gfs);
+                     gd->params.eta_damping, yn);
+  const dim3 sg((g.G0 - NRPYELL_NG + 63) / 64, g.N1, g.N2);
+  for (int k = 0; k < smooth_passes; k++) {
+    hipLaunchKernelGGL(smooth_kernel<0>, sg, dim3(64), 0, 0, g, yn, tmp);
+    hipLaunchKernelGGL(smooth_kernel<1>, sg, dim3(64), 0, 0, g, tmp, yn);
+    hipLaunchKernelGGL(smooth_kernel<2>, sg, dim3(64), 0, 0, g, yn, tmp);
+    HIPCHK(hipMemcpyAsync(yn, tmp, sizeof(double) * g.ntot, hipMemcpyDeviceToDevice, 0));
+  }
+  if (smooth_passes > 0) {
+    launch_fill_ghost_planes<double>(g, yn, 1, 0);
+    hipLaunchKernelGGL(v_from_u_kernel, dim3((g.ntot + 255) / 256), dim3(256), 0, 0, g.ntot,
+                       gd->params.eta_damping, yn);
+  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
   hipFree(X); hipFree(scale); hipFree(idx); hipFree(y);

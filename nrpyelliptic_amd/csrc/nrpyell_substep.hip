@@ -315,15 +315,9 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
       SolverDeviceState *st = a.state;
       st->steps = st->nchecks;  // completed steps when stage 1 read y_n
       st->nchecks++;
-      st->last_log10 = l;
-      if (st->nhist < NRPYELL_HIST) {
-        st->hist_step[st->nhist] = st->steps;
-        st->hist_log10[st->nhist] = l;
-        st->nhist++;
-      }
-      // stages 3 and 4 of this step then skip, leaving y_n as it was after st->steps steps
-      if (!(l == l)) st->done = 2;
-      else if (l <= a.l2_target) st->done = 1;
+      // once done is set, stages 3 and 4 of this step skip, leaving y_n as it was after
+      // st->steps steps
+      record_check(st, l, a.l2_target);
     }
   }
 }
@@ -356,6 +350,9 @@ __global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<
   const unsigned q = xcd_logical(blockIdx.x, gridDim.x);
   const int c = block_cfg[q];
   const SubstepArgs<T> a = cfgs[c];
+  // a converged configuration's blocks leave before issuing any load (the batch keeps them
+  // in the launch until the host compacts it)
+  if (*a.done) return;
   const unsigned local = q - block_first[c], ng = (unsigned)(a.g.N2 / PL);
   substep_body<T, STAGE, TX, PL>(a, local / ng, local % ng, local, a.ntile0 * a.ntile1 * ng);
 }

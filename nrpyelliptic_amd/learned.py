@@ -61,11 +61,11 @@ def generate_dataset(configs: List[config.PunctureConfig], rows_per_config=8192,
     return X, T, row, stats
 
 
-def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem):
+def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem, smooth_passes: int = 2):
     """Zero-guess residual vs NN-guess residual; keep the NN guess only if its residual is lower."""
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
     r0 = P.log10_residual()
-    net.seed(P)
+    net.seed(P, smooth_passes)
     r1 = P.log10_residual()
     used = r1 < r0
     if not used:
@@ -80,7 +80,8 @@ def _solve(S):
         return None
 
 
-def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_steps=10000, log=print):
+def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_steps=10000, smooth_passes=2,
+             log=print):
     """Per configuration: steps and seconds to `target` from the zero guess (FP64) and from the
     NN guess with the fallback (FP64 or mixed precision), single-configuration solver, and the
     max |du| between the two converged states."""
@@ -91,7 +92,7 @@ def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_step
         P.set("y_n_gfs", np.zeros(2 * P.ntot))
         rz = _solve(S)
         u_zero = P.get("y_n_gfs")[: P.ntot]
-        used, r0, r1 = seed_with_fallback(net, P)
+        used, r0, r1 = seed_with_fallback(net, P, smooth_passes)
         S2 = P.solver(log10_target=target, precision=precision, max_steps=max_steps)
         rn = _solve(S2)
         ok = rz is not None and rn is not None and bool(rz.converged and rn.converged)
@@ -111,7 +112,8 @@ def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_step
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--train-configs", type=int, default=1024)
-    ap.add_argument("--rows-per-config", type=int, default=8192)
+    ap.add_argument("--rows-per-config", type=int, default=4096)
+    ap.add_argument("--smooth", type=int, default=2)
     ap.add_argument("--eval-configs", type=int, default=64)
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8192)
@@ -136,8 +138,8 @@ def main():
         f"{losses[-50:].mean():.3e}")
     net.save(args.out)
     eval_cfgs = config.random_batch(args.eval_configs, seed=900, grids=False, Nxx=(128, 128, 16))
-    rows64 = evaluate(net, eval_cfgs, precision=64, target=args.target, log=log)
-    rows32 = evaluate(net, eval_cfgs, precision=32, target=args.target, log=log)
+    rows64 = evaluate(net, eval_cfgs, precision=64, target=args.target, smooth_passes=args.smooth, log=log)
+    rows32 = evaluate(net, eval_cfgs, precision=32, target=args.target, smooth_passes=args.smooth, log=log)
 
     def summary(rows):
         n_all = len(rows)

@@ -40,7 +40,7 @@ def _bind():
     L.nrpyell_mlp_feature_rows.restype = C.c_long
     L.nrpyell_mlp_features.argtypes = [C.POINTER(abi.griddata_struct), vp, vp, vp]
     L.nrpyell_mlp_targets.argtypes = [C.POINTER(abi.griddata_struct), vp, vp, vp]
-    L.nrpyell_mlp_seed.argtypes = [vp, C.POINTER(abi.griddata_struct)]
+    L.nrpyell_mlp_seed.argtypes = [vp, C.POINTER(abi.griddata_struct), C.c_int]
     L.nrpyell_memcpy_d2d.argtypes = [vp, vp, C.c_size_t]
     L.nrpyell_mlp_sample_rows.argtypes = [vp, vp, C.c_long, C.c_long, C.c_ulonglong, vp, vp]
     L._mlp_bound = True
@@ -126,8 +126,8 @@ class InitialGuessMLP:
                                         losses.ctypes.data_as(C.POINTER(C.c_double))), "mlp_train")
         return losses
 
-    def seed(self, problem) -> None:
-        check(_bind().nrpyell_mlp_seed(self.h, C.byref(problem.gd)), "mlp_seed")
+    def seed(self, problem, smooth_passes: int = 0) -> None:
+        check(_bind().nrpyell_mlp_seed(self.h, C.byref(problem.gd), int(smooth_passes)), "mlp_seed")
 
     def close(self):
         if self.h:

@@ -361,7 +361,8 @@ class BatchSolver:
         res = (solve_result * n)()
         sec = C.c_double()
         rc = lib().nrpyell_batch_solve(self.h, res, C.byref(sec))
-        if rc not in (NRPYELL_OK, -4):
+        # not converged / NaN in some configurations: reported per configuration (converged = 0)
+        if rc not in (NRPYELL_OK, -4, -5):
             check(rc, "batch_solve")
         return list(res), sec.value
 
