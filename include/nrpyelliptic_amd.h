@@ -252,7 +252,8 @@ typedef struct {
   double integration_radius;  /* L2 mask radius, 100 in the reference driver (main.c:150) */
   int tile_i0;            /* i0 width of the fused substep kernel's 256-thread tile: 16, 32
                              or 64 (0 = default); the i1 height is 256 / tile_i0            */
-  int planes;             /* i2 planes per thread: 1, or 2 with tile_i0 32/64 (0 = 1)     */
+  int planes;             /* i2 planes per thread: 1, or 2 with tile_i0 32/64 (0 = 2 where
+                             the tile allows it, else 1)                                    */
 } nrpyell_solve_options;
 
 typedef struct {

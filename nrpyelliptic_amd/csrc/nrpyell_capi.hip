@@ -560,6 +560,7 @@ struct nrpyell_solver {
     a.done = &dstate->done;
     a.tx = tx;
     a.pl = pl;
+    a.nbc_rows = (g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS;
     a.ntile0 = ntile0;
     a.ntile1 = ntile1;
     a.w2d = nullptr;
@@ -654,7 +655,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     set_last_error("tile_i0 must be 16, 32 or 64");
     return NRPYELL_EINVAL;
   }
-  const int pl = s->opt.planes > 0 ? s->opt.planes : 1;
+  const int pl = s->opt.planes > 0 ? s->opt.planes : (tx >= 32 ? 2 : 1);
   if (pl != 1 && !(pl == 2 && tx >= 32)) {
     set_last_error("planes must be 1, or 2 with tile_i0 32 or 64");
     return NRPYELL_EINVAL;
@@ -1047,7 +1048,8 @@ struct nrpyell_batch {
         in_graph++;
         nrpyell_solver *s = solvers[k];
         first[j] = (int)bcfg.size();
-        const int nb = s->ntile0 * s->ntile1 * (s->g.N2 / s->pl);
+        const int nb = s->ntile0 * s->ntile1 * (s->g.N2 / s->pl) +
+                       (s->g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS * s->g.N2;
         for (int q = 0; q < nb; q++) bcfg.push_back(j);
       }
       if (!g.d_block_cfg) {  // first build: every member is active, the buffers get their full size

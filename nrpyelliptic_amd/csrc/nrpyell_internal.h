@@ -28,6 +28,9 @@ struct SolverDeviceState {
 // decade.
 #define NRPYELL_DIVERGE_DECADES 4.0
 
+// i1 rows handled by one BC block of the fused substep kernel
+#define NRPYELL_BC_ROWS 16
+
 // Bookkeeping of one residual check (log10 L2 = l) taken after st->steps steps: history,
 // done = 1 at the target, 2 on NaN, 3 on divergence.
 __device__ inline void record_check(SolverDeviceState *st, double l, double target) {
@@ -80,6 +83,7 @@ struct SubstepArgs {
   const int *done;  // nonzero: converged, every launch returns at once
   int tx;           // tile width along i0 (threads), tile height along i1 = blockDim/tx
   int pl;           // i2 planes per thread (1 or 2)
+  int nbc_rows;     // BC blocks per i2 plane: ceil(N1 / NRPYELL_BC_ROWS)
   int ntile0, ntile1;
   // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
   // is c^2 times the residual. Non-null l2_partial enables it.

@@ -5,7 +5,7 @@
 //      precombined (i0,i1) metric coefficients, v - eta u, and c^2 (lap u + AA^2/(8 (psi+u)^7));
 //   2. the radiation condition on the 5 outer ghost layers of the x0-max face (the reference's
 //      innermost-first recurrence, apply_bcs_outerradiation_and_inner.c:369-408, in closed
-//      form), done by the blocks that own the last interior i0 of each (i1,i2) row, from LDS;
+//      form), done by separate light BC blocks in the same launch, from LDS;
 //   3. the stage update (RK4 k_odd/k_even scheme) of every interior and outer point;
 //   4. the inner ghost zones: each updated point writes its own parity images (the closed form
 //      of the bcstruct map, verified against the table on the host), which is the inner-BC
@@ -18,8 +18,9 @@
 //      c^2 times the residual, so a block reduction and a last-block finalize give the
 //      reference's every-step convergence check (main.c:146-151) for free.
 //
-// Tiles are TX (i0, across lanes) x TY (i1) points of one i2 plane, 256 threads; the u plane
-// with its i0/i1 halos is staged in LDS, the i2 arm is read from global memory (coalesced).
+// Interior tiles are TX (i0, across lanes) x TY (i1) points of one or two i2 planes, 256
+// threads; the u plane with its i0/i1 halos is staged in LDS, the i2 arm is read from global
+// memory (coalesced).
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -62,23 +63,41 @@ __device__ __forceinline__ void fd_sums(const T m[6], const T p[6], T uu, T &s2,
   s1 = fma((T)FD1_W5, p[5] - m[5], s1);
 }
 
-// One block's work: tile `tile` (t0 + ntile0*t1) of the PL consecutive i2 planes
-// NG + PL*pgroup ...; `bid` is the block's index among the nblk blocks of this configuration
-// (for the L2 partial sums). With PL = 2 every thread updates the same (i0,i1) on two planes:
-// the metric coefficients and weights are read once, and the i2 arms share 12 planes instead
-// of 2 x 11.
+// RHS at one point (rhs_eval.c:86-110): ku = v - eta u and kv = c^2 (lap u + AA^2/(8 (psi+u)^7))
+// from the stencil arms, with c^2 folded into K and AAc. The interior blocks and the BC blocks
+// both evaluate it at the last interior point, from the same inputs: bit-identical values.
+template <typename T>
+__device__ __forceinline__ void point_rhs(T uu, T vv, T psi, T AAc, const T K[5], const T xm[6], const T xp[6],
+                                          const T ym[6], const T yp[6], const T zm[6], const T zp[6], T eta, T &ku,
+                                          T &kv) {
+  T s00, s0, s11, s1d, s22, s2d;
+  fd_sums<T>(xm, xp, uu, s00, s0);
+  fd_sums<T>(ym, yp, uu, s11, s1d);
+  fd_sums<T>(zm, zp, uu, s22, s2d);
+  const T X = psi + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
+  kv = AAc / X7;
+  kv = fma(K[0], s00, kv);
+  kv = fma(K[1], s11, kv);
+  kv = fma(K[2], s22, kv);
+  kv = fma(K[3], s1d, kv);
+  kv = fma(K[4], s0, kv);
+  ku = fma(-eta, uu, vv);
+}
+
+// ---------------------------------------------------------------------------------------
+// Interior blocks: tile `tile` (t0 + ntile0*t1) of the PL consecutive i2 planes
+// NG + PL*pgroup ...; `bid` is the block's index among the nblk interior blocks of this
+// configuration (for the L2 partial sums). With PL = 2 every thread updates the same (i0,i1)
+// on two planes: the metric coefficients and weights are read once, and the i2 arms share 12
+// planes instead of 2 x 11. The outer layers belong to the BC blocks (bc_body).
+// ---------------------------------------------------------------------------------------
 template <typename T, int STAGE, int TX, int PL>
-__device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int tile, const int pgroup,
-                                             const unsigned bid, const unsigned nblk) {
+__device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int tile, const int pgroup,
+                                              const unsigned bid, const unsigned nblk) {
   constexpr int TY = 256 / TX;
   constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG, BOX = BX * BY, NR = (BOX + 255) / 256;
-  constexpr int NW = PL + 2 * NG;        // i2 window: planes i2a-NG .. i2a+PL-1+NG
-  constexpr int NV = (12 * BY + 255) / 256;  // v columns ilast-6..ilast+5, all box rows
-  static_assert(12 * TY <= 256 && PL * 10 * TY <= 256, "one BC work item per thread");
-  __shared__ T su[PL][NR * 256];  // u on this block's planes: tile + halos (padded: no store guards)
-  __shared__ T sv[PL][BOX];       // v, the 12 columns the BC reads (BC tiles only)
-  __shared__ T s_rhs0[PL][TY][2], s_A[PL][TY][2][6];
-  __shared__ T s_geo[6][TY][4];
+  constexpr int NW = PL + 2 * NG;  // i2 window: planes i2a-NG .. i2a+PL-1+NG
+  __shared__ T su[PL][NR * 256];   // u on this block's planes: tile + halos (padded: no store guards)
   __shared__ double s_red[4];
 
   const GridDims &g = a.g;
@@ -89,64 +108,28 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   const int i0 = i0s + tx, i1 = i1s + ty;
   const bool valid = i0 < g.N0 + NG && i1 < g.N1 + NG;
   const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
-  const int ilast = g.N0 + NG - 1;  // last interior i0; the radiation BC starts next to it
-  const bool has_bc = (i0s <= ilast) && (ilast < i0s + TX);
-  const int bx0 = ilast - (i0s - NG);  // box column of the last interior point (BC tiles)
   const bool fuse_l2 = (STAGE == 1) && (a.l2_partial != nullptr);
-  const int done = a.done ? *a.done : 0;
+  const int done = *a.done;
 
-  // ---- global loads: LDS staging of the planes, the i2 window and the pointwise inputs.
-  // Every load is unconditional, from an index clamped into the grid: values the stencils do
-  // not reach (box corners of non-BC tiles, columns or rows past a ragged edge) are loaded
-  // but never read, and the loads compile to straight-line code the compiler can keep in
-  // flight across the per-plane barriers.
-  // Issue order is plane-major (vmcnt counts in order): the BC geometry and edge values, then
-  // everything plane 0 needs, then plane 1's box and pointwise values, so plane 0 computes and
-  // stores while plane 1's loads are in flight.
-  constexpr int NGEO = (24 * TY + 255) / 256;
-  T geo_r[NGEO];
-  // BC phase A work item of this thread: (row, gf, layer 0..5); the pt = 0 stencil reaches
-  // column ilast-6, left of the box when the tile holds only the last interior column
-  const int a_row = min(tid / 12, TY - 1), a_gf = (tid % 12) / 6;
-  T edge[PL];
-  if (has_bc) {
-#pragma unroll
-    for (int r = 0; r < NGEO; r++) {
-      const int k = min(tid + 256 * r, 24 * TY - 1);
-      const int pt = k / (4 * TY), row = (k / 4) % TY, comp = k % 4;
-      geo_r[r] = a.geo[((long)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp];
-    }
-#pragma unroll
-    for (int q = 0; q < PL; q++)
-      edge[q] = a.in[a_gf * nt + (long)(i2a + q) * s2 + (long)min(i1s + a_row, g.G1 - 1) * s1 + max(ilast - NG - 1, 0)];
-  }
+  // ---- global loads. Every load is unconditional, from an index clamped into the grid:
+  // values the stencils do not reach (box corners, columns or rows past a ragged edge) are
+  // loaded but never read, and the loads compile to straight-line code the compiler keeps in
+  // flight across the per-plane barriers. Issue order is plane-major (vmcnt counts in order),
+  // so plane 0 computes and stores while plane 1's loads are in flight.
   const int i0c = min(i0, g.N0 + NG - 1), i1c = min(i1, g.N1 + NG - 1);
   const long k2 = (long)i1c * s1 + i0c;
-  T stage_u[PL][NR], stage_v[PL][NV];
-  int v_slot[NV];
+  T stage_u[PL][NR];
   T zw[NW];  // u on the i2 window at (i0,i1), through the periodic map
   T vv[PL], psi[PL], AAc[PL], yn_u[PL], yn_v[PL], rt_u[PL], rt_v[PL], K[5];
   double w2 = 0.0;
 #pragma unroll
   for (int q = 0; q < PL; q++) {
-    // plane q's box: the tile with its i0/i1 halos
 #pragma unroll
     for (int r = 0; r < NR; r++) {
       const int k = tid + 256 * r;
       const int bx = k % BX, by = k / BX;
       const int j0 = min(i0s - NG + bx, g.G0 - 1), j1 = min(i1s - NG + by, g.G1 - 1);
       stage_u[q][r] = a.in[(long)(i2a + q) * s2 + (long)j1 * s1 + j0];
-    }
-    // v for the BC: columns ilast-6..ilast+5 (box columns bx0-6..bx0+5) of every box row
-    if (has_bc) {
-#pragma unroll
-      for (int r = 0; r < NV; r++) {
-        const int k = tid + 256 * r, c = k % 12, by = min(k / 12, BY - 1);
-        const int bx = bx0 - NG - 1 + c;
-        const int j0 = max(ilast - NG - 1 + c, 0), j1 = min(i1s - NG + by, g.G1 - 1);
-        v_slot[r] = (k < 12 * BY && bx >= 0) ? by * BX + bx : -1;
-        stage_v[q][r] = a.in[nt + (long)(i2a + q) * s2 + (long)j1 * s1 + j0];
-      }
     }
     if (q == 0) {
 #pragma unroll
@@ -167,39 +150,14 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
       rt_v[q] = a.rt[nt + p];
     }
   }
-  // outer-layer state for the BC stage update (work item = (plane, row, gf, layer 1..5))
-  const int od_q = min(tid / (10 * TY), PL - 1), od_rem = tid % (10 * TY);
-  const int od_row = od_rem / 10, od_gf = (od_rem % 10) / 5, od_pt = od_rem % 5 + 1;
-  const int od_j1 = min(i1s + od_row, g.G1 - 1);
-  const long od_p = (long)(i2a + od_q) * s2 + (long)od_j1 * s1 + (ilast + od_pt) + od_gf * nt;
-  const bool od_on = has_bc && tid < PL * 10 * TY && i1s + od_row < g.N1 + NG;
-  T od_yn = 0, od_rt = 0;
-  if (has_bc) {
-    od_yn = a.yn[od_p];
-    if (STAGE != 1) od_rt = a.rt[od_p];
-  }
   if (done) return;  // converged: uniform across the grid
 
-  // ---- per plane: stage the box in LDS, then 1+3+4 (interior RHS, stage update, parity
-  // images) and the BC radiation terms of that plane. The planes are pipelined: plane q's
-  // barrier waits only for plane q's box loads, so plane q+1's loads are still in flight while
-  // plane q computes and stores.
-  if (has_bc) {
-#pragma unroll
-    for (int r = 0; r < NGEO; r++)
-      if (tid + 256 * r < 24 * TY) (&s_geo[0][0][0])[tid + 256 * r] = geo_r[r];
-  }
   double l2acc = 0.0;
   const int cb = (ty + NG) * BX + tx + NG;
 #pragma unroll
   for (int q = 0; q < PL; q++) {
 #pragma unroll
     for (int r = 0; r < NR; r++) su[q][tid + 256 * r] = stage_u[q][r];
-    if (has_bc) {
-#pragma unroll
-      for (int r = 0; r < NV; r++)
-        if (v_slot[r] >= 0) sv[q][v_slot[r]] = stage_v[q][r];
-    }
     __syncthreads();
     if (valid) {
       const int i2 = i2a + q;
@@ -212,28 +170,13 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
         ym[k] = su[q][cb - k * BX]; yp[k] = su[q][cb + k * BX];
         zm[k] = zw[NG + q - k]; zp[k] = zw[NG + q + k];
       }
-      T s00, s0, s11, s1d, s22, s2d;
-      fd_sums<T>(xm, xp, uu, s00, s0);
-      fd_sums<T>(ym, yp, uu, s11, s1d);
-      fd_sums<T>(zm, zp, uu, s22, s2d);
       if (STAGE == 1) {
         yn_u[q] = uu;
         yn_v[q] = vv[q];
       }
-      // kv = c^2 (lap u + AA^2/(8 (psi+u)^7)) (rhs_eval.c:110) with c^2 in the tables
-      const T X = psi[q] + uu, X2 = X * X, X4 = X2 * X2, X7 = X4 * X2 * X;
-      T kv = AAc[q] / X7;
-      kv = fma(K[0], s00, kv);
-      kv = fma(K[1], s11, kv);
-      kv = fma(K[2], s22, kv);
-      kv = fma(K[3], s1d, kv);
-      kv = fma(K[4], s0, kv);
-      const T ku = fma(-a.eta, uu, vv[q]);  // rhs_eval.c:109
+      T ku, kv;
+      point_rhs<T>(uu, vv[q], psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, a.eta, ku, kv);
       if (fuse_l2) l2acc += w2 * (double)kv * (double)kv;  // w / c^4 x (c^2 R)^2
-      if (has_bc && i0 == ilast) {
-        s_rhs0[q][ty][0] = ku;
-        s_rhs0[q][ty][1] = kv;
-      }
       T o_u, o_v;
       stage_update<T, STAGE>(ku, a.dt, yn_u[q], rt_u[q], o_u);
       stage_update<T, STAGE>(kv, a.dt, yn_v[q], rt_v[q], o_v);
@@ -248,54 +191,11 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
         a.out[nt + d] = o_v;
       });
     }
-    // ---- 2a: BC radiation terms of plane q. With A_pt = -c (df/dr + f/r) at layer pt
-    // (pt = 0: the last interior point), the reference's innermost-first recurrence
-    // (apply_bcs_outerradiation_and_inner.c:369-408)
-    //   rhs_pt = A_pt + (r_{pt-1}/r_pt)^3 (rhs_{pt-1} - A_{pt-1})
-    // telescopes to rhs_pt = A_pt + (r_0/r_pt)^3 (rhs_0 - A_0), so the layers are independent
-    // (1/r and (r_0/r_pt)^3 come precomputed in the geometry table).
-    if (has_bc && tid < 12 * TY) {
-      const int row = a_row, gf = a_gf, pt = tid % 6;
-      if (i1s + row < g.N1 + NG) {
-        const T *box = gf == 0 ? su[q] : sv[q];
-        const int bx = bx0 + pt, by = row + NG, sh = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
-        T d0 = 0, d1 = 0;
-#pragma unroll
-        for (int j = 0; j < 11; j++) {
-          const int col = bx + (j - 5 - sh);
-          const T f0v = col >= 0 ? box[by * BX + col] : edge[q];  // col < 0: column ilast-6
-          d0 = fma((T)c_fd1_shift[sh][j], f0v, d0);
-          d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * BX + bx], d1);
-        }
-        d0 *= (T)g.invdx0;
-        d1 *= (T)g.invdx1;
-        const T dr = s_geo[pt][row][1] * d0 + s_geo[pt][row][2] * d1;
-        const T cc = gf == 0 ? a.cOB_u : a.cOB_v;
-        s_A[q][row][gf][pt] = -cc * (dr + box[by * BX + bx] * s_geo[pt][row][0]);
-      }
-    }
   }
 
-  // ---- 2b: stage update of the outer layers (all planes) once their radiation terms and
-  // the last interior point's RHS are in LDS
-  if (has_bc) {
-    __syncthreads();
-    if (od_on) {
-      const T rhs = fma(s_geo[od_pt][od_row][3], s_rhs0[od_q][od_row][od_gf] - s_A[od_q][od_row][od_gf][0],
-                        s_A[od_q][od_row][od_gf][od_pt]);
-      T o;
-      T rt_ = od_rt;
-      stage_update<T, STAGE>(rhs, a.dt, od_yn, rt_, o);
-      if (STAGE != 4) a.rt[od_p] = rt_;
-      a.out[od_p] = o;
-      T *outg = a.out + od_gf * nt;
-      for_each_image_in_planes(g, ilast + od_pt, i1s + od_row, i2a + od_q, [&](long d) { outg[d] = o; });
-    }
-  }
-
-  // ---- 5: residual L2 of the stage-1 input: one partial per block (plain store); the
-  // stage-2 launch that follows reduces them in block order, after the kernel boundary has
-  // made them visible (no atomics, no L2 flushes inside the kernel).
+  // ---- residual L2 of the stage-1 input: one partial per block (plain store); the stage-2
+  // launch that follows reduces them in block order, after the kernel boundary has made them
+  // visible (no atomics, no L2 flushes inside the kernel).
   if (fuse_l2) {
     for (int o = 32; o > 0; o >>= 1) l2acc += __shfl_down(l2acc, o, 64);
     if ((tid & 63) == 0) s_red[tid >> 6] = l2acc;
@@ -322,6 +222,126 @@ __device__ __forceinline__ void substep_body(const SubstepArgs<T> &a, const int 
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// BC blocks: the radiation condition on the 5 outer layers (i0 = ilast+1..ilast+5) of
+// NRPYELL_BC_ROWS rows (i1) of one i2 plane. Each row needs the RHS at the last interior point
+// (re-evaluated here with point_rhs), and u and v on the 12 columns ilast-6..ilast+5 with the
+// rows' i1 halos for the one-sided i0 derivatives and the centred i1 derivatives
+// (apply_bcs_outerradiation_and_inner.c:11-141). With the radiation term A_pt = -c (df/dr + f/r)
+// at layer pt (pt = 0: the last interior point), the reference's innermost-first recurrence
+// (:369-408)
+//   rhs_pt = A_pt + (r_{pt-1}/r_pt)^3 (rhs_{pt-1} - A_{pt-1})
+// telescopes to rhs_pt = A_pt + (r_0/r_pt)^3 (rhs_0 - A_0), so the layers are independent
+// (1/r and (r_0/r_pt)^3 come precomputed in the geometry table). The blocks are light and are
+// dispatched first; the interior blocks carry no BC work.
+// ---------------------------------------------------------------------------------------
+template <typename T, int STAGE>
+__device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, const int plane) {
+  constexpr int TYB = NRPYELL_BC_ROWS, BYB = TYB + 2 * NG, NBX = 12, NBOX = NBX * BYB;
+  constexpr int NRB = (NBOX + 255) / 256, NGEO = (24 * TYB + 255) / 256;
+  static_assert(12 * TYB <= 256 && 10 * TYB <= 256, "one BC work item per thread");
+  __shared__ T bu[NRB * 256], bv[NRB * 256];  // [by][c]: u, v at column ilast-6+c, row i1s-NG+by
+  __shared__ T s_rhs0[TYB][2], s_A[TYB][2][6], s_geo[6][TYB][4];
+
+  const GridDims &g = a.g;
+  const int tid = threadIdx.x;
+  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
+  const int ilast = g.N0 + NG - 1;
+  const int i1s = NG + rb * TYB, i2 = NG + plane;
+  const long pl = (long)i2 * s2;
+  const int done = *a.done;
+
+  T st_u[NRB], st_v[NRB];
+#pragma unroll
+  for (int r = 0; r < NRB; r++) {
+    const int k = tid + 256 * r, c = k % NBX, by = min(k / NBX, BYB - 1);
+    const long q = pl + (long)min(i1s - NG + by, g.G1 - 1) * s1 + max(ilast - NG - 1 + c, 0);
+    st_u[r] = a.in[q];
+    st_v[r] = a.in[nt + q];
+  }
+  T geo_r[NGEO];
+#pragma unroll
+  for (int r = 0; r < NGEO; r++) {
+    const int k = min(tid + 256 * r, 24 * TYB - 1);
+    const int pt = k / (4 * TYB), row = (k / 4) % TYB, comp = k % 4;
+    geo_r[r] = a.geo[((long)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp];
+  }
+  // RHS inputs at the last interior point of row tid (threads tid < TYB)
+  const int rrow = min(tid, TYB - 1);
+  const long k2 = (long)min(i1s + rrow, g.N1 + NG - 1) * s1 + ilast;
+  T zw[2 * NG + 1], K[5], vv = 0, psi = 1, AAc = 0;
+  if (tid < TYB) {
+#pragma unroll
+    for (int j = 0; j <= 2 * NG; j++) zw[j] = j == NG ? (T)0 : a.in[(long)wrap2(g, i2 - NG + j) * s2 + k2];
+#pragma unroll
+    for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
+    vv = a.in[nt + pl + k2];
+    psi = a.src[pl + k2];
+    AAc = a.src[nt + pl + k2];
+  }
+  // outer-layer point of this thread (work item = (row, gf, layer 1..5))
+  const int od_row = min(tid / 10, TYB - 1), od_gf = (tid % 10) / 5, od_pt = tid % 5 + 1;
+  const long od_p = pl + (long)min(i1s + od_row, g.G1 - 1) * s1 + (ilast + od_pt) + od_gf * nt;
+  const bool od_on = tid < 10 * TYB && i1s + od_row < g.N1 + NG;
+  const T od_yn = a.yn[od_p];
+  const T od_rt = STAGE != 1 ? a.rt[od_p] : (T)0;
+  if (done) return;
+#pragma unroll
+  for (int r = 0; r < NRB; r++) {
+    bu[tid + 256 * r] = st_u[r];
+    bv[tid + 256 * r] = st_v[r];
+  }
+#pragma unroll
+  for (int r = 0; r < NGEO; r++)
+    if (tid + 256 * r < 24 * TYB) (&s_geo[0][0][0])[tid + 256 * r] = geo_r[r];
+  __syncthreads();
+
+  if (tid < TYB && i1s + tid < g.N1 + NG) {
+    const int cb = (tid + NG) * NBX + NG + 1;  // column ilast
+    T xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+#pragma unroll
+    for (int k = 1; k <= 5; k++) {
+      xm[k] = bu[cb - k]; xp[k] = bu[cb + k];
+      ym[k] = bu[cb - k * NBX]; yp[k] = bu[cb + k * NBX];
+      zm[k] = zw[NG - k]; zp[k] = zw[NG + k];
+    }
+    T ku, kv;
+    point_rhs<T>(bu[cb], vv, psi, AAc, K, xm, xp, ym, yp, zm, zp, a.eta, ku, kv);
+    s_rhs0[tid][0] = ku;
+    s_rhs0[tid][1] = kv;
+  }
+  if (tid < 12 * TYB) {
+    const int row = tid / 12, rem = tid % 12, gf = rem / 6, pt = rem % 6;
+    if (i1s + row < g.N1 + NG) {
+      const T *box = gf == 0 ? bu : bv;
+      const int c0 = NG + 1 + pt, by = row + NG, sh = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
+      T d0 = 0, d1 = 0;
+#pragma unroll
+      for (int j = 0; j < 11; j++) {
+        d0 = fma((T)c_fd1_shift[sh][j], box[by * NBX + c0 + (j - 5 - sh)], d0);
+        d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * NBX + c0], d1);
+      }
+      d0 *= (T)g.invdx0;
+      d1 *= (T)g.invdx1;
+      const T dr = s_geo[pt][row][1] * d0 + s_geo[pt][row][2] * d1;
+      const T cc = gf == 0 ? a.cOB_u : a.cOB_v;
+      s_A[row][gf][pt] = -cc * (dr + box[by * NBX + c0] * s_geo[pt][row][0]);
+    }
+  }
+  __syncthreads();
+  if (od_on) {
+    const T rhs = fma(s_geo[od_pt][od_row][3], s_rhs0[od_row][od_gf] - s_A[od_row][od_gf][0],
+                      s_A[od_row][od_gf][od_pt]);
+    T o;
+    T rt_ = od_rt;
+    stage_update<T, STAGE>(rhs, a.dt, od_yn, rt_, o);
+    if (STAGE != 4) a.rt[od_p] = rt_;
+    a.out[od_p] = o;
+    T *outg = a.out + od_gf * nt;
+    for_each_image_in_planes(g, ilast + od_pt, i1s + od_row, i2, [&](long d) { outg[d] = o; });
+  }
+}
+
 // XCD-aware block order. Workgroups are dispatched to the 8 XCDs round robin (workgroup L to
 // XCD L % 8), and each XCD has its own L2. Logical block q = (L % 8) * (n / 8) + L / 8 gives
 // every XCD one contiguous run of logical blocks; with the i2 plane group fastest in q, the
@@ -332,17 +352,23 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned L, unsigned n) {
   return L < 8 * per ? (L % 8) * per + L / 8 : L;
 }
 
-// One configuration: a 1-D grid of ntile0*ntile1*(N2/PL) blocks.
+// One configuration: the nbc_rows*N2 BC blocks first (dispatched first, spread over the XCDs),
+// then the ntile0*ntile1*(N2/PL) interior blocks in XCD-aware order.
 template <typename T, int STAGE, int TX, int PL>
 __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
-  const unsigned n = gridDim.x, q = xcd_logical(blockIdx.x, n);
-  const unsigned ng = (unsigned)(a.g.N2 / PL);
-  substep_body<T, STAGE, TX, PL>(a, q / ng, q % ng, q, n);
+  const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
+  if (L < nb) {
+    bc_body<T, STAGE>(a, L % a.nbc_rows, L / a.nbc_rows);
+    return;
+  }
+  const unsigned ni = gridDim.x - nb, q = xcd_logical(L - nb, ni), ng = (unsigned)(a.g.N2 / PL);
+  interior_body<T, STAGE, TX, PL>(a, q / ng, q % ng, q, ni);
 }
 
 // Many configurations (uneven grids) in one launch: logical block q works for configuration
-// block_cfg[q], as that configuration's block q - block_first[cfg]. Configurations that have
-// converged return at once (their own done flag).
+// block_cfg[q], as that configuration's block q - block_first[cfg] (its BC blocks first, then
+// its interior blocks). Configurations that have converged return at once (their own done
+// flag) and are dropped from the launch when the host compacts the batch.
 template <typename T, int STAGE, int TX, int PL>
 __global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<T> *__restrict__ cfgs,
                                                               const int *__restrict__ block_cfg,
@@ -350,16 +376,19 @@ __global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<
   const unsigned q = xcd_logical(blockIdx.x, gridDim.x);
   const int c = block_cfg[q];
   const SubstepArgs<T> a = cfgs[c];
-  // a converged configuration's blocks leave before issuing any load (the batch keeps them
-  // in the launch until the host compacts it)
-  if (*a.done) return;
-  const unsigned local = q - block_first[c], ng = (unsigned)(a.g.N2 / PL);
-  substep_body<T, STAGE, TX, PL>(a, local / ng, local % ng, local, a.ntile0 * a.ntile1 * ng);
+  if (*a.done) return;  // leave before issuing any load
+  const unsigned local = q - block_first[c], nb = (unsigned)(a.nbc_rows * a.g.N2);
+  if (local < nb) {
+    bc_body<T, STAGE>(a, local % a.nbc_rows, local / a.nbc_rows);
+    return;
+  }
+  const unsigned li = local - nb, ng = (unsigned)(a.g.N2 / PL);
+  interior_body<T, STAGE, TX, PL>(a, li / ng, li % ng, li, a.ntile0 * a.ntile1 * ng);
 }
 
 template <typename T, int STAGE>
 static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
-  dim3 grd(a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(256);
+  dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(256);
   if (a.pl == 2) {
     switch (a.tx) {
       case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64, 2>), grd, blk, 0, st, a); break;
