@@ -645,6 +645,11 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     set_last_error("grid too small for the fused solver (need Nxx0,Nxx1 >= 10 and even Nxx2)");
     return NRPYELL_EINVAL;
   }
+  if ((unsigned long)3 * g.ntot * sizeof(double) >= (1ul << 32)) {
+    // the fused kernels address each array with 32-bit byte offsets
+    set_last_error("grid too large for the fused solver: each array must stay under 4 GiB");
+    return NRPYELL_EINVAL;
+  }
   std::string err;
   if (ghostmap_verify(g, &gd->bcstruct, &err) != NRPYELL_OK) {
     set_last_error(err);

@@ -103,6 +103,18 @@ __host__ __device__ inline void for_each_image_in_planes(const GridDims &g, int 
   }
 }
 
+// base[i] with a 32-bit element index: the address is the 64-bit (uniform) base plus a 32-bit
+// byte offset, which the compiler emits as one saddr global load/store without per-lane 64-bit
+// address arithmetic. Callers keep every array under 2^32 bytes.
+template <typename T>
+__device__ __forceinline__ T ld32(const T *__restrict__ base, unsigned i) {
+  return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T *base, unsigned i, T v) {
+  *(T *)((char *)base + i * (unsigned)sizeof(T)) = v;
+}
+
 // 10th-order centred FD weights (rhs_eval.c:23-27; Fornberg).
 #define FD1_W1 (5.0 / 6.0)
 #define FD1_W2 (5.0 / 21.0)

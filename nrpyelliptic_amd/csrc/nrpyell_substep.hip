@@ -107,7 +107,8 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
   const int i0s = NG + t0 * TX, i1s = NG + t1 * TY;
   const int i0 = i0s + tx, i1 = i1s + ty;
   const bool valid = i0 < g.N0 + NG && i1 < g.N1 + NG;
-  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
+  // 32-bit element indices (the host caps a configuration at 2^32 bytes per array)
+  const unsigned nt = (unsigned)g.ntot, s1 = g.G0, s2 = (unsigned)(g.G0 * g.G1);
   const bool fuse_l2 = (STAGE == 1) && (a.l2_partial != nullptr);
   const int done = *a.done;
 
@@ -117,7 +118,7 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
   // flight across the per-plane barriers. Issue order is plane-major (vmcnt counts in order),
   // so plane 0 computes and stores while plane 1's loads are in flight.
   const int i0c = min(i0, g.N0 + NG - 1), i1c = min(i1, g.N1 + NG - 1);
-  const long k2 = (long)i1c * s1 + i0c;
+  const unsigned k2 = (unsigned)i1c * s1 + i0c;
   T stage_u[PL][NR];
   T zw[NW];  // u on the i2 window at (i0,i1), through the periodic map
   T vv[PL], psi[PL], AAc[PL], yn_u[PL], yn_v[PL], rt_u[PL], rt_v[PL], K[5];
@@ -129,25 +130,25 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
       const int k = tid + 256 * r;
       const int bx = k % BX, by = k / BX;
       const int j0 = min(i0s - NG + bx, g.G0 - 1), j1 = min(i1s - NG + by, g.G1 - 1);
-      stage_u[q][r] = a.in[(long)(i2a + q) * s2 + (long)j1 * s1 + j0];
+      stage_u[q][r] = ld32(a.in, (unsigned)(i2a + q) * s2 + (unsigned)j1 * s1 + j0);
     }
     if (q == 0) {
 #pragma unroll
-      for (int jj = 0; jj < NW; jj++) zw[jj] = a.in[(long)wrap2(g, i2a - NG + jj) * s2 + k2];
+      for (int jj = 0; jj < NW; jj++) zw[jj] = ld32(a.in, (unsigned)wrap2(g, i2a - NG + jj) * s2 + k2);
 #pragma unroll
-      for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
-      if (fuse_l2) w2 = a.w2d[k2];
+      for (int k = 0; k < 5; k++) K[k] = ld32(a.coef, k * s2 + k2);
+      if (fuse_l2) w2 = ld32(a.w2d, k2);
     }
-    const long p = (long)(i2a + q) * s2 + k2;
-    vv[q] = a.in[nt + p];
-    psi[q] = a.src[p];
-    AAc[q] = a.src[nt + p];
+    const unsigned p = (unsigned)(i2a + q) * s2 + k2;
+    vv[q] = ld32(a.in, nt + p);
+    psi[q] = ld32(a.src, p);
+    AAc[q] = ld32(a.src, nt + p);
     yn_u[q] = 0; yn_v[q] = 0; rt_u[q] = 0; rt_v[q] = 0;
     if (STAGE != 1) {  // stage 1 updates its own input (y_n)
-      yn_u[q] = a.yn[p];
-      yn_v[q] = a.yn[nt + p];
-      rt_u[q] = a.rt[p];
-      rt_v[q] = a.rt[nt + p];
+      yn_u[q] = ld32(a.yn, p);
+      yn_v[q] = ld32(a.yn, nt + p);
+      rt_u[q] = ld32((const T *)a.rt, p);
+      rt_v[q] = ld32((const T *)a.rt, nt + p);
     }
   }
   if (done) return;  // converged: uniform across the grid
@@ -161,7 +162,7 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
     __syncthreads();
     if (valid) {
       const int i2 = i2a + q;
-      const long p = (long)i2 * s2 + k2;
+      const unsigned p = (unsigned)i2 * s2 + k2;
       const T uu = zw[NG + q];
       T xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
 #pragma unroll
@@ -181,14 +182,14 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
       stage_update<T, STAGE>(ku, a.dt, yn_u[q], rt_u[q], o_u);
       stage_update<T, STAGE>(kv, a.dt, yn_v[q], rt_v[q], o_v);
       if (STAGE != 4) {
-        a.rt[p] = rt_u[q];
-        a.rt[nt + p] = rt_v[q];
+        st32(a.rt, p, rt_u[q]);
+        st32(a.rt, nt + p, rt_v[q]);
       }
-      a.out[p] = o_u;
-      a.out[nt + p] = o_v;
+      st32(a.out, p, o_u);
+      st32(a.out, nt + p, o_v);
       for_each_image_in_planes(g, i0, i1, i2, [&](long d) {
-        a.out[d] = o_u;
-        a.out[nt + d] = o_v;
+        st32(a.out, (unsigned)d, o_u);
+        st32(a.out, nt + (unsigned)d, o_v);
       });
     }
   }
@@ -245,46 +246,46 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x;
-  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
+  const unsigned nt = (unsigned)g.ntot, s1 = g.G0, s2 = (unsigned)(g.G0 * g.G1);
   const int ilast = g.N0 + NG - 1;
   const int i1s = NG + rb * TYB, i2 = NG + plane;
-  const long pl = (long)i2 * s2;
+  const unsigned pl = (unsigned)i2 * s2;
   const int done = *a.done;
 
   T st_u[NRB], st_v[NRB];
 #pragma unroll
   for (int r = 0; r < NRB; r++) {
     const int k = tid + 256 * r, c = k % NBX, by = min(k / NBX, BYB - 1);
-    const long q = pl + (long)min(i1s - NG + by, g.G1 - 1) * s1 + max(ilast - NG - 1 + c, 0);
-    st_u[r] = a.in[q];
-    st_v[r] = a.in[nt + q];
+    const unsigned q = pl + (unsigned)min(i1s - NG + by, g.G1 - 1) * s1 + max(ilast - NG - 1 + c, 0);
+    st_u[r] = ld32(a.in, q);
+    st_v[r] = ld32(a.in, nt + q);
   }
   T geo_r[NGEO];
 #pragma unroll
   for (int r = 0; r < NGEO; r++) {
     const int k = min(tid + 256 * r, 24 * TYB - 1);
     const int pt = k / (4 * TYB), row = (k / 4) % TYB, comp = k % 4;
-    geo_r[r] = a.geo[((long)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp];
+    geo_r[r] = ld32(a.geo, ((unsigned)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp);
   }
   // RHS inputs at the last interior point of row tid (threads tid < TYB)
   const int rrow = min(tid, TYB - 1);
-  const long k2 = (long)min(i1s + rrow, g.N1 + NG - 1) * s1 + ilast;
+  const unsigned k2 = (unsigned)min(i1s + rrow, g.N1 + NG - 1) * s1 + ilast;
   T zw[2 * NG + 1], K[5], vv = 0, psi = 1, AAc = 0;
   if (tid < TYB) {
 #pragma unroll
-    for (int j = 0; j <= 2 * NG; j++) zw[j] = j == NG ? (T)0 : a.in[(long)wrap2(g, i2 - NG + j) * s2 + k2];
+    for (int j = 0; j <= 2 * NG; j++) zw[j] = j == NG ? (T)0 : ld32(a.in, (unsigned)wrap2(g, i2 - NG + j) * s2 + k2);
 #pragma unroll
-    for (int k = 0; k < 5; k++) K[k] = a.coef[k * s2 + k2];
-    vv = a.in[nt + pl + k2];
-    psi = a.src[pl + k2];
-    AAc = a.src[nt + pl + k2];
+    for (int k = 0; k < 5; k++) K[k] = ld32(a.coef, k * s2 + k2);
+    vv = ld32(a.in, nt + pl + k2);
+    psi = ld32(a.src, pl + k2);
+    AAc = ld32(a.src, nt + pl + k2);
   }
   // outer-layer point of this thread (work item = (row, gf, layer 1..5))
   const int od_row = min(tid / 10, TYB - 1), od_gf = (tid % 10) / 5, od_pt = tid % 5 + 1;
-  const long od_p = pl + (long)min(i1s + od_row, g.G1 - 1) * s1 + (ilast + od_pt) + od_gf * nt;
+  const unsigned od_p = pl + (unsigned)min(i1s + od_row, g.G1 - 1) * s1 + (ilast + od_pt) + od_gf * nt;
   const bool od_on = tid < 10 * TYB && i1s + od_row < g.N1 + NG;
-  const T od_yn = a.yn[od_p];
-  const T od_rt = STAGE != 1 ? a.rt[od_p] : (T)0;
+  const T od_yn = ld32(a.yn, od_p);
+  const T od_rt = STAGE != 1 ? ld32((const T *)a.rt, od_p) : (T)0;
   if (done) return;
 #pragma unroll
   for (int r = 0; r < NRB; r++) {
@@ -335,10 +336,10 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
     T o;
     T rt_ = od_rt;
     stage_update<T, STAGE>(rhs, a.dt, od_yn, rt_, o);
-    if (STAGE != 4) a.rt[od_p] = rt_;
-    a.out[od_p] = o;
-    T *outg = a.out + od_gf * nt;
-    for_each_image_in_planes(g, ilast + od_pt, i1s + od_row, i2, [&](long d) { outg[d] = o; });
+    if (STAGE != 4) st32(a.rt, od_p, rt_);
+    st32(a.out, od_p, o);
+    for_each_image_in_planes(g, ilast + od_pt, i1s + od_row, i2,
+                             [&](long d) { st32(a.out, od_gf * nt + (unsigned)d, o); });
   }
 }
 
