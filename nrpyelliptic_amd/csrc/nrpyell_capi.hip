@@ -506,10 +506,9 @@ struct nrpyell_solver {
   hipStream_t stream = nullptr;
   int tx = 32, pl = 1, ntile0 = 0, ntile1 = 0;
   double *coef64 = nullptr;   // metric coefficients (residual kernel)
-  double *kc64 = nullptr;     // c^2 x coefficients (substep kernel)
+  double *kc64 = nullptr;     // per (i0,i1): c^2 x 5 coefficients, L2 weight / c^4 (substep kernel)
   float *kc32 = nullptr;
-  double *src64 = nullptr;    // psi | 0.125 c^2 AA^2 (substep kernel)
-  double *w2c = nullptr;      // L2 weights / c^4 (fused stage-1 check)
+  double *src64 = nullptr;    // per point: psi, 0.125 c^2 AA^2 (substep kernel)
   double *geo64 = nullptr;
   float *geo32 = nullptr;
   double *w2d = nullptr;
@@ -528,7 +527,7 @@ struct nrpyell_solver {
   ~nrpyell_solver() {
     if (graph64) hipGraphExecDestroy(graph64);
     if (graph32) hipGraphExecDestroy(graph32);
-    hipFree(coef64); hipFree(kc64); hipFree(kc32); hipFree(src64); hipFree(w2c);
+    hipFree(coef64); hipFree(kc64); hipFree(kc32); hipFree(src64);
     hipFree(geo64); hipFree(geo32); hipFree(w2d); hipFree(partial);
     hipFree(l2_out); hipFree(l2_count); hipFree(dstate);
     if (hstate) hipHostFree(hstate);
@@ -580,7 +579,7 @@ struct nrpyell_solver {
       SubstepArgs<double> a = args<double>(s, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs,
                                            src64, kc64, geo64);
       if (s <= 2 && fuse_check) {  // stage 1 writes the partials, stage 2 reduces them
-        a.w2d = w2c;
+        a.w2d = w2d;
         a.l2_partial = partial;
         a.l2_count = l2_count;
         a.state = dstate;
@@ -683,13 +682,12 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   host_l2_weights(&gd->params, gd->xx, s->opt.integration_radius, w.data(), &s->volume);
   s->w2d = upload(w.data(), w.size());
   // tables with the wave speed folded in (auxevol and rfm are read here, once per solver)
-  s->kc64 = dalloc<double>(5 * n2);
-  s->kc32 = dalloc<float>(5 * n2);
-  s->w2c = dalloc<double>(n2);
+  s->kc64 = dalloc<double>(6 * n2);
+  s->kc32 = dalloc<float>(6 * n2);
   s->src64 = dalloc<double>(2 * g.ntot);
   int *bad = dalloc<int>(1);
   HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
-  launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->w2c, s->src64, bad, s->stream);
+  launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->src64, bad, s->stream);
   int hbad = 0;
   HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -918,7 +916,7 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
       s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, s->src64,
                       s->kc64, s->geo64);
   if (fuse) {
-    a.w2d = s->w2c;
+    a.w2d = s->w2d;
     a.l2_partial = s->partial;
     a.l2_count = s->l2_count;
     a.state = s->dstate;
@@ -1115,7 +1113,7 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
         SubstepArgs<double> a = s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs,
                                                 f.y_nplus1_running_total_gfs, s->src64, s->kc64, s->geo64);
         if (stage <= 2) {
-          a.w2d = s->w2c;
+          a.w2d = s->w2d;
           a.l2_partial = s->partial;
           a.l2_count = s->l2_count;
           a.state = s->dstate;

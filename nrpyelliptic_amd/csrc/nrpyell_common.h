@@ -115,6 +115,24 @@ __device__ __forceinline__ void st32(T *base, unsigned i, T v) {
   *(T *)((char *)base + i * (unsigned)sizeof(T)) = v;
 }
 
+// Two consecutive elements base[i], base[i+1] (i even, base 16-byte aligned) in one load.
+template <typename T>
+struct vec2_of;
+template <>
+struct vec2_of<double> {
+  using type = double2;
+};
+template <>
+struct vec2_of<float> {
+  using type = float2;
+};
+template <typename T>
+__device__ __forceinline__ void ld2x32(const T *__restrict__ base, unsigned i, T &x, T &y) {
+  const typename vec2_of<T>::type v = *(const typename vec2_of<T>::type *)((const char *)base + i * (unsigned)sizeof(T));
+  x = v.x;
+  y = v.y;
+}
+
 // 10th-order centred FD weights (rhs_eval.c:23-27; Fornberg).
 #define FD1_W1 (5.0 / 6.0)
 #define FD1_W2 (5.0 / 21.0)

@@ -75,8 +75,8 @@ struct SubstepArgs {
   T *out;         // stage output (stage 1-3) or y_n (stage 4)
   const T *yn;    // y_n
   T *rt;          // running total
-  const T *src;   // psi | 0.125 c^2 AA^2 (c the wave speed)
-  const T *coef;  // c^2 x the 5 metric coefficients, 5 x (G0*G1)
+  const T *src;   // per point (psi, 0.125 c^2 AA^2), c the wave speed
+  const T *coef;  // per (i0,i1): c^2 x the 5 metric coefficients, then the L2 weight / c^4
   const T *geo;   // outer-face geometry: [pt 0..5][i1][1/r, dxx0/dr, dxx1/dr, (r_0/r_pt)^3], pt 0 = last interior i0
   T cOB_u, cOB_v;
   T eta, dt;
@@ -87,7 +87,7 @@ struct SubstepArgs {
   int ntile0, ntile1;
   // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
   // is c^2 times the residual. Non-null l2_partial enables it.
-  const double *w2d;          // L2 weights per (i0,i1): dV / c^4 inside the radius, 0 outside
+  const double *w2d;          // unused by the substep kernel (the weight is in coef)
   double *l2_partial;         // one partial sum per block
   unsigned int *l2_count;     // blocks finished; the last block reduces and resets it
   SolverDeviceState *state;   // convergence state updated by the last block
@@ -104,7 +104,7 @@ void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float
 template <typename T>
 void launch_fill_ghost_planes(const GridDims &g, T *y, int ngf, hipStream_t st);
 void launch_fold(const GridDims &g, const double *aux, const double *coef, const double *w2d, double *kc64,
-                 float *kc32, double *w2c, double *src, int *bad, hipStream_t st);
+                 float *kc32, double *src, int *bad, hipStream_t st);
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 template <typename T>

@@ -485,32 +485,36 @@ void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st) {
 //   0.125 c^2 AA     (per point, next to psi),
 // and evaluates kv = c^2 (lap u + AA/(8 (psi+u)^7)) without reading c. A point whose wave speed
 // differs from that of its (i0,i1) column on plane NG raises *bad.
+// kc[6 k2 + j]: j < 5 the folded coefficients, j = 5 the folded weight, per (i0,i1) in one
+// 48-byte record (three 16-byte loads in the FP64 kernel).
 __global__ void fold_coef_kernel(GridDims g, const double *aux, const double *coef, const double *w2d, double *kc64,
-                                 float *kc32, double *w2c) {
+                                 float *kc32) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x, i1 = blockIdx.y;
   if (i0 >= g.G0) return;
   const long k2 = i0 + (long)g.G0 * i1, n2 = (long)g.G0 * g.G1;
   const double c = aux[WAVESPEEDGF * g.ntot + idx3(g, i0, i1, NG)], c2 = c * c;
   for (int k = 0; k < 5; k++) {
     const double v = c2 * coef[k * n2 + k2];
-    kc64[k * n2 + k2] = v;
-    kc32[k * n2 + k2] = (float)v;
+    kc64[6 * k2 + k] = v;
+    kc32[6 * k2 + k] = (float)v;
   }
-  w2c[k2] = c2 > 0.0 ? w2d[k2] / (c2 * c2) : 0.0;
+  const double w = c2 > 0.0 ? w2d[k2] / (c2 * c2) : 0.0;
+  kc64[6 * k2 + 5] = w;
+  kc32[6 * k2 + 5] = (float)w;
 }
 __global__ void fold_source_kernel(GridDims g, const double *aux, double *src, int *bad) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x, i1 = blockIdx.y, i2 = blockIdx.z;
   if (i0 >= g.G0) return;
   const long p = idx3(g, i0, i1, i2), nt = g.ntot;
   const double c = aux[WAVESPEEDGF * nt + p];
-  src[p] = aux[PSI_BACKGROUNDGF * nt + p];
-  src[nt + p] = 0.125 * (c * c) * aux[ADD_TIMES_AUUGF * nt + p];
+  src[2 * p] = aux[PSI_BACKGROUNDGF * nt + p];                    // interleaved per point:
+  src[2 * p + 1] = 0.125 * (c * c) * aux[ADD_TIMES_AUUGF * nt + p];  // one 16-byte load
   const bool inside = i0 >= NG && i0 < g.N0 + NG && i1 >= NG && i1 < g.N1 + NG && i2 >= NG && i2 < g.N2 + NG;
   if (inside && c != aux[WAVESPEEDGF * nt + idx3(g, i0, i1, NG)]) *bad = 1;
 }
 void launch_fold(const GridDims &g, const double *aux, const double *coef, const double *w2d, double *kc64,
-                 float *kc32, double *w2c, double *src, int *bad, hipStream_t st) {
-  hipLaunchKernelGGL(fold_coef_kernel, dim3((g.G0 + 63) / 64, g.G1), dim3(64), 0, st, g, aux, coef, w2d, kc64, kc32, w2c);
+                 float *kc32, double *src, int *bad, hipStream_t st) {
+  hipLaunchKernelGGL(fold_coef_kernel, dim3((g.G0 + 63) / 64, g.G1), dim3(64), 0, st, g, aux, coef, w2d, kc64, kc32);
   hipLaunchKernelGGL(fold_source_kernel, dim3((g.G0 + 63) / 64, g.G1, g.G2), dim3(64), 0, st, g, aux, src, bad);
 }
 

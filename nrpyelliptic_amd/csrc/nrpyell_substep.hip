@@ -135,14 +135,15 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
     if (q == 0) {
 #pragma unroll
       for (int jj = 0; jj < NW; jj++) zw[jj] = ld32(a.in, (unsigned)wrap2(g, i2a - NG + jj) * s2 + k2);
-#pragma unroll
-      for (int k = 0; k < 5; k++) K[k] = ld32(a.coef, k * s2 + k2);
-      if (fuse_l2) w2 = ld32(a.w2d, k2);
+      T kw;
+      ld2x32(a.coef, 6 * k2, K[0], K[1]);
+      ld2x32(a.coef, 6 * k2 + 2, K[2], K[3]);
+      ld2x32(a.coef, 6 * k2 + 4, K[4], kw);
+      if (fuse_l2) w2 = (double)kw;
     }
     const unsigned p = (unsigned)(i2a + q) * s2 + k2;
     vv[q] = ld32(a.in, nt + p);
-    psi[q] = ld32(a.src, p);
-    AAc[q] = ld32(a.src, nt + p);
+    ld2x32(a.src, 2 * p, psi[q], AAc[q]);
     yn_u[q] = 0; yn_v[q] = 0; rt_u[q] = 0; rt_v[q] = 0;
     if (STAGE != 1) {  // stage 1 updates its own input (y_n)
       yn_u[q] = ld32(a.yn, p);
@@ -274,11 +275,12 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   if (tid < TYB) {
 #pragma unroll
     for (int j = 0; j <= 2 * NG; j++) zw[j] = j == NG ? (T)0 : ld32(a.in, (unsigned)wrap2(g, i2 - NG + j) * s2 + k2);
-#pragma unroll
-    for (int k = 0; k < 5; k++) K[k] = ld32(a.coef, k * s2 + k2);
+    T kw;
+    ld2x32(a.coef, 6 * k2, K[0], K[1]);
+    ld2x32(a.coef, 6 * k2 + 2, K[2], K[3]);
+    ld2x32(a.coef, 6 * k2 + 4, K[4], kw);
     vv = ld32(a.in, nt + pl + k2);
-    psi = ld32(a.src, pl + k2);
-    AAc = ld32(a.src, nt + pl + k2);
+    ld2x32(a.src, 2 * (pl + k2), psi, AAc);
   }
   // outer-layer point of this thread (work item = (row, gf, layer 1..5))
   const int od_row = min(tid / 10, TYB - 1), od_gf = (tid % 10) / 5, od_pt = tid % 5 + 1;
