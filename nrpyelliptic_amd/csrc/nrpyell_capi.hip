@@ -250,6 +250,7 @@ struct DeviceBc {
   const double *xx_key0 = nullptr, *xx_key1 = nullptr;
   RadPoint *rad[NGHOSTS * 3] = {};
   int nrad[NGHOSTS * 3] = {};
+  bool rows = false;  // x0 face only, same rows in every layer: one launch for all layers
   ~DeviceBc() {
     hipFree(dst); hipFree(src); hipFree(par0);
     for (auto *p : rad) hipFree(p);
@@ -330,6 +331,16 @@ static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const
     }
     d.rad[s] = upload(pts.data(), n);
   }
+  // one-launch form: only x0-face points, and layer gz lists the rows of layer 0 in the same
+  // order, each point the outer neighbour of the previous layer's point
+  d.rows = d.nrad[0] > 0;
+  for (int gz = 0; gz < NGHOSTS && d.rows; gz++) {
+    if (d.nrad[3 * gz] != d.nrad[0] || d.nrad[3 * gz + 1] != 0 || d.nrad[3 * gz + 2] != 0) d.rows = false;
+    for (int k = 0; k < d.nrad[0] && d.rows && gz > 0; k++) {
+      const outerpt_bc_struct &a = bc->pure_outer_bc_array[3 * gz][k], &b = bc->pure_outer_bc_array[3 * (gz - 1)][k];
+      if (a.FACEX0 != 1 || b.FACEX0 != 1 || a.i1 != b.i1 || a.i2 != b.i2 || a.i0 != b.i0 + 1) d.rows = false;
+    }
+  }
   d.xx_key0 = xx[0];
   d.xx_key1 = xx[1];
 }
@@ -370,9 +381,16 @@ extern "C" void apply_bcs_outerradiation_and_inner(const paramstruct *params, co
   DeviceBc &d = device_bc(params, bcstruct);
   ensure_radiation_lists(d, params, bcstruct, xx);
   // innermost layer first; within a layer x0, x1, x2 faces (apply_bcs_outerradiation_and_inner.c:431-458)
-  for (int s = 0; s < NGHOSTS * 3; s++)
-    launch_radiation_list(g, d.nrad[s], d.rad[s], gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
+  if (d.rows) {
+    RadLayers L;
+    for (int gz = 0; gz < NGHOSTS; gz++) L.l[gz] = d.rad[3 * gz];
+    launch_radiation_rows(g, d.nrad[0], L, gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
                           custom_f_infinity[0], custom_f_infinity[1], default_stream());
+  } else {
+    for (int s = 0; s < NGHOSTS * 3; s++)
+      launch_radiation_list(g, d.nrad[s], d.rad[s], gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
+                            custom_f_infinity[0], custom_f_infinity[1], default_stream());
+  }
   launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());

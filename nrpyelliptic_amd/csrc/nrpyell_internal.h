@@ -29,7 +29,9 @@ struct SolverDeviceState {
 #define NRPYELL_DIVERGE_DECADES 4.0
 
 // i1 rows handled by one BC block of the fused substep kernel
+#ifndef NRPYELL_BC_ROWS
 #define NRPYELL_BC_ROWS 16
+#endif
 
 // Bookkeeping of one residual check (log10 L2 = l) taken after st->steps steps: history,
 // done = 1 at the target, 2 on NaN, 3 on divergence.
@@ -65,6 +67,9 @@ struct RadPoint {
   long p, q;
   int off[2][3];
   double geo[2][4];
+};
+struct RadLayers {
+  const RadPoint *l[NRPYELL_NG];  // the x0-face list of each ghost layer, same row order
 };
 
 // Arguments of the fused RK4 substep kernel.
@@ -117,6 +122,8 @@ void launch_l2_finalize(const double *partial, long npart, double volume, Solver
                         double target, double *log10_out, hipStream_t st);
 void launch_inner_gather(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs,
                          hipStream_t st);
+void launch_radiation_rows(const GridDims &g, int n, const RadLayers &L, const double *gfs, double *rhs, double c_u,
+                           double c_v, double finf_u, double finf_v, hipStream_t st);
 void launch_radiation_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs, double c_u,
                            double c_v, double finf_u, double finf_v, hipStream_t st);
 void launch_rfm(const GridDims &g, double AMAX, double w, double b, const double *xx0_d, const double *xx1_d,

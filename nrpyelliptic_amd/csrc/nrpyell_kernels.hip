@@ -317,46 +317,78 @@ void launch_inner_gather(long nt, int n, const int *dst, const int *src, const i
   hipLaunchKernelGGL(inner_gather_kernel, dim3((n + 255) / 256), dim3(256), 0, st, nt, n, dst, src, par0, gfs);
 }
 
-// One pure outer point of one (which_gz, dirn) list (radiation_bcs,
-// apply_bcs_outerradiation_and_inner.c:369-408). geo holds r, dxx0/dr, dxx1/dr, dxx2/dr for
-// the point and for its interior neighbour; offsets are the clamped stencil shifts.
+// d f / d r at point w (0: P itself, 1: its interior neighbour) of a radiation record: the
+// 11-point x0 and x1 derivatives with the record's clamped stencil shifts, combined with
+// dxx0/dr and dxx1/dr (apply_bcs_outerradiation_and_inner.c:290-368).
+__device__ inline double radiation_dr(const GridDims &g, const RadPoint &P, int w, const double *f) {
+  const long s1 = g.G0, q = w == 0 ? P.p : P.q;
+  const int o0 = P.off[w][0], o1 = P.off[w][1];
+  double d0 = 0, d1 = 0;
+  for (int j = 0; j < 11; j++) {
+    const double c0 = o0 <= 0 ? c_fd1_shift[-o0][j] : -c_fd1_shift[o0][10 - j];
+    const double c1 = o1 <= 0 ? c_fd1_shift[-o1][j] : -c_fd1_shift[o1][10 - j];
+    d0 = fma(c0, f[q + (j - 5 + o0)], d0);
+    d1 = fma(c1, f[q + (long)(j - 5 + o1) * s1], d1);
+  }
+  return P.geo[w][1] * d0 * g.invdx0 + P.geo[w][2] * d1 * g.invdx1;
+}
+
+// One pure outer point P and one gridfunction f (radiation_bcs,
+// apply_bcs_outerradiation_and_inner.c:369-408). geo holds r, dxx0/dr, dxx1/dr, dxx2/dr for the
+// point and for its interior neighbour; offsets are the clamped stencil shifts.
+__device__ inline void radiation_point(const GridDims &g, const RadPoint &P, const double *f, double *rh, double c,
+                                      double finf) {
+  const double dr0 = radiation_dr(g, P, 0, f), dr1 = radiation_dr(g, P, 1, f);
+  const double r = P.geo[0][0], r_int = P.geo[1][0];
+  const double out_int = -c * (dr1 + (f[P.q] - finf) / r_int);
+  const double kk = r_int * r_int * r_int * (rh[P.q] - out_int);
+  const double rinv = 1.0 / r;
+  rh[P.p] = -c * (dr0 + (f[P.p] - finf) * rinv) + kk * rinv * rinv * rinv;
+}
+
+// One (which_gz, dirn) list, one thread per (point, gridfunction).
 __global__ void radiation_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs,
                                       double c_u, double c_v, double finf_u, double finf_v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const RadPoint P = pts[k];
-  const long nt = g.ntot, s1 = g.G0, s2 = (long)g.G0 * g.G1;
-  for (int gf = 0; gf < 2; gf++) {
-    const double *f = gfs + gf * nt;
-    const double c = gf == 0 ? c_u : c_v, finf = gf == 0 ? finf_u : finf_v;
-    double dr[2];
-    for (int w = 0; w < 2; w++) {
-      const long q = w == 0 ? P.p : P.q;
-      const int o0 = P.off[w][0], o1 = P.off[w][1], o2 = P.off[w][2];
-      double d0 = 0, d1 = 0, d2 = 0;
-      for (int j = 0; j < 11; j++) {
-        const double c0 = o0 <= 0 ? c_fd1_shift[-o0][j] : -c_fd1_shift[o0][10 - j];
-        const double c1 = o1 <= 0 ? c_fd1_shift[-o1][j] : -c_fd1_shift[o1][10 - j];
-        d0 = fma(c0, f[q + (j - 5 + o0)], d0);
-        d1 = fma(c1, f[q + (long)(j - 5 + o1) * s1], d1);
-        (void)o2;
-        (void)s2;
-      }
-      dr[w] = P.geo[w][1] * d0 * g.invdx0 + P.geo[w][2] * d1 * g.invdx1;
-    }
-    const double r = P.geo[0][0], r_int = P.geo[1][0];
-    const double out_int = -c * (dr[1] + (f[P.q] - finf) / r_int);
-    const double kk = r_int * r_int * r_int * (rhs[gf * nt + P.q] - out_int);
-    const double rinv = 1.0 / r;
-    rhs[gf * nt + P.p] = -c * (dr[0] + (f[P.p] - finf) * rinv) + kk * rinv * rinv * rinv;
-  }
+  if (k >= 2 * n) return;
+  const int gf = k / n;
+  radiation_point(g, pts[k % n], gfs + gf * g.ntot, rhs + gf * g.ntot, gf == 0 ? c_u : c_v,
+                  gf == 0 ? finf_u : finf_v);
 }
 
 void launch_radiation_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs, double c_u,
                            double c_v, double finf_u, double finf_v, hipStream_t st) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(radiation_list_kernel, dim3((n + 127) / 128), dim3(128), 0, st, g, n, pts, gfs, rhs, c_u, c_v,
+  hipLaunchKernelGGL(radiation_list_kernel, dim3((2 * n + 63) / 64), dim3(64), 0, st, g, n, pts, gfs, rhs, c_u, c_v,
                      finf_u, finf_v);
+}
+
+// All NG layers in one launch when only the x0 face has pure outer points and every layer
+// lists the same (i1,i2) rows in the same order (SinhSymTP). The reference's innermost-first
+// recurrence rhs_p = A_p + (r_{p-1}/r_p)^3 (rhs_{p-1} - A_{p-1}), A = -c (df/dr + (f - finf)/r),
+// telescopes to rhs_p = A_p + (r_0/r_p)^3 (rhs_0 - A_0) with index 0 the last interior point, so
+// every (row, gridfunction, layer) is independent: one thread each.
+__global__ void radiation_rows_kernel(GridDims g, int n, RadLayers L, const double *gfs, double *rhs, double c_u,
+                                      double c_v, double finf_u, double finf_v) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 2 * NG * n) return;
+  const int row = k % n, gf = (k / n) % 2, gz = k / (2 * n);
+  const double *f = gfs + gf * g.ntot;
+  double *rh = rhs + gf * g.ntot;
+  const double c = gf == 0 ? c_u : c_v, finf = gf == 0 ? finf_u : finf_v;
+  const RadPoint &P = L.l[gz][row], &P0 = L.l[0][row];  // P0.q: the last interior point
+  const double r = P.geo[0][0], r0 = P0.geo[1][0];
+  const double A = -c * (radiation_dr(g, P, 0, f) + (f[P.p] - finf) / r);
+  const double A0 = -c * (radiation_dr(g, P0, 1, f) + (f[P0.q] - finf) / r0);
+  const double q = r0 / r;
+  rh[P.p] = A + q * q * q * (rh[P0.q] - A0);
+}
+
+void launch_radiation_rows(const GridDims &g, int n, const RadLayers &L, const double *gfs, double *rhs, double c_u,
+                           double c_v, double finf_u, double finf_v, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(radiation_rows_kernel, dim3((2 * NG * n + 63) / 64), dim3(64), 0, st, g, n, L, gfs, rhs, c_u,
+                     c_v, finf_u, finf_v);
 }
 
 // ---------------------------------------------------------------------------------------
