@@ -178,6 +178,31 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
                          "nn_used_frac": sum(r["nn_used"] for r in ok) / max(len(ok), 1)}}
 
 
+def batched_solve(n=64, seed=42):
+    """The batched configuration: n random punctures (seed 42: q log-uniform in [1,8], D in
+    [6,20], Newtonian tangential momenta, random spins; grids mixing Nxx0 in {96,128,160},
+    Nxx1 in {96,128}, Nxx2 in {8,16}) relaxed together to log10 L2 <= -10 with tile shapes from
+    the autotuner; throughput counts the substeps of the configurations that converged."""
+    from nrpyelliptic_amd import config, runtime
+    from nrpyelliptic_amd.autotune import TileTuner
+    probs = [runtime.Problem(c) for c in config.random_batch(n, seed=seed)]
+    tuner = TileTuner(cache_path=None, launches=100)
+    t0 = time.perf_counter()
+    tiles = tuner.choose_all(probs)
+    tune_s = time.perf_counter() - t0
+    B = runtime.BatchSolver(probs, tiles=tiles, log10_target=-10.0, max_steps=20000)
+    res, sec = B.solve()
+    conv = [(P, r) for P, r in zip(probs, res) if r.converged]
+    pts = sum(4.0 * P.cfg.Nxx[0] * P.cfg.Nxx[1] * P.cfg.Nxx[2] * r.steps for P, r in conv)
+    out = {"configurations": n, "converged": len(conv), "diverged_or_stopped": n - len(conv), "seconds": sec,
+           "gp_substeps_per_s": pts / sec, "mean_steps": sum(r.steps for _, r in conv) / max(len(conv), 1),
+           "autotune_seconds": tune_s, "tiles": sorted({f"{t[0]}x{t[1]}" for t in tiles})}
+    B.close()
+    for P in probs:
+        P.close()
+    return out
+
+
 def load_traffic():
     """HBM bytes per substep launch from the committed rocprofv3 PMC summary, scaled by the
     copy-kernel calibration recorded with it (null when absent)."""
@@ -245,6 +270,7 @@ def main():
     # per-stage kernel durations (back-to-back launches, HIP events on the solver stream)
     stage_us = {f"stage{k}": round(S.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)}
     dropin_us = dropin_call_us(P)
+    batched = batched_solve()
 
     cpu = None if args.no_cpu_baseline else run_cpu_baseline_subprocess()
     line = {
@@ -269,6 +295,7 @@ def main():
                      "us_per_launch": us_per_launch, "bytes_per_launch": BYTES_PER_GP_SUBSTEP * npts,
                      "stage_us": stage_us},
         "dropin_call_us": dropin_us,
+        "batched_64_seed42": batched,
         "cpu_baseline": cpu,
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
                              "log10_l2": r.log10_l2, "converged": bool(r.converged),
