@@ -684,7 +684,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   }
   s->tx = tx;
   s->pl = pl;
-  const int ty = 256 / tx;
+  const int ty = NRPYELL_BLOCK / tx;
   s->ntile0 = (g.N0 + tx - 1) / tx;
   s->ntile1 = (g.N1 + ty - 1) / ty;
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));

@@ -28,9 +28,13 @@ struct SolverDeviceState {
 // decade.
 #define NRPYELL_DIVERGE_DECADES 4.0
 
-// i1 rows handled by one BC block of the fused substep kernel
+// threads per block of the fused substep kernel (256 or 512), and the i1 rows handled by one
+// of its BC blocks
+#ifndef NRPYELL_BLOCK
+#define NRPYELL_BLOCK 256
+#endif
 #ifndef NRPYELL_BC_ROWS
-#define NRPYELL_BC_ROWS 16
+#define NRPYELL_BC_ROWS (NRPYELL_BLOCK / 16)
 #endif
 
 // Bookkeeping of one residual check (log10 L2 = l) taken after st->steps steps: history,

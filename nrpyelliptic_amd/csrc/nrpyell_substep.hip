@@ -18,8 +18,8 @@
 //      c^2 times the residual, so a block reduction and a last-block finalize give the
 //      reference's every-step convergence check (main.c:146-151) for free.
 //
-// Interior tiles are TX (i0, across lanes) x TY (i1) points of one or two i2 planes, 256
-// threads; the u plane with its i0/i1 halos is staged in LDS, the i2 arm is read from global
+// Interior tiles are TX (i0, across lanes) x TY (i1) points of one or two i2 planes,
+// NRPYELL_BLOCK threads; the u plane with its i0/i1 halos is staged in LDS, the i2 arm is read from global
 // memory (coalesced).
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -63,6 +63,14 @@ __device__ __forceinline__ void fd_sums(const T m[6], const T p[6], T uu, T &s2,
   s1 = fma((T)FD1_W5, p[5] - m[5], s1);
 }
 
+// Fixed-order sum of the per-wave partials of a block.
+template <int NW>
+__device__ __forceinline__ double wave_sum(const double *s) {
+  double acc = (s[0] + s[1]) + (s[2] + s[3]);
+  if (NW == 8) acc += (s[4] + s[5]) + (s[6] + s[7]);
+  return acc;
+}
+
 // RHS at one point (rhs_eval.c:86-110): ku = v - eta u and kv = c^2 (lap u + AA^2/(8 (psi+u)^7))
 // from the stencil arms, with c^2 folded into K and AAc. The interior blocks and the BC blocks
 // both evaluate it at the last interior point, from the same inputs: bit-identical values.
@@ -94,11 +102,11 @@ __device__ __forceinline__ void point_rhs(T uu, T vv, T psi, T AAc, const T K[5]
 template <typename T, int STAGE, int TX, int PL>
 __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int tile, const int pgroup,
                                               const unsigned bid, const unsigned nblk) {
-  constexpr int TY = 256 / TX;
-  constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG, BOX = BX * BY, NR = (BOX + 255) / 256;
+  constexpr int NT = NRPYELL_BLOCK, TY = NT / TX;
+  constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG, BOX = BX * BY, NR = (BOX + NT - 1) / NT;
   constexpr int NW = PL + 2 * NG;  // i2 window: planes i2a-NG .. i2a+PL-1+NG
-  __shared__ T su[PL][NR * 256];   // u on this block's planes: tile + halos (padded: no store guards)
-  __shared__ double s_red[4];
+  __shared__ T su[PL][NR * NT];    // u on this block's planes: tile + halos (padded: no store guards)
+  __shared__ double s_red[NT / 64];
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -127,7 +135,7 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
   for (int q = 0; q < PL; q++) {
 #pragma unroll
     for (int r = 0; r < NR; r++) {
-      const int k = tid + 256 * r;
+      const int k = tid + NT * r;
       const int bx = k % BX, by = k / BX;
       const int j0 = min(i0s - NG + bx, g.G0 - 1), j1 = min(i1s - NG + by, g.G1 - 1);
       stage_u[q][r] = ld32(a.in, (unsigned)(i2a + q) * s2 + (unsigned)j1 * s1 + j0);
@@ -159,7 +167,7 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
 #pragma unroll
   for (int q = 0; q < PL; q++) {
 #pragma unroll
-    for (int r = 0; r < NR; r++) su[q][tid + 256 * r] = stage_u[q][r];
+    for (int r = 0; r < NR; r++) su[q][tid + NT * r] = stage_u[q][r];
     __syncthreads();
     if (valid) {
       const int i2 = i2a + q;
@@ -202,17 +210,17 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
     for (int o = 32; o > 0; o >>= 1) l2acc += __shfl_down(l2acc, o, 64);
     if ((tid & 63) == 0) s_red[tid >> 6] = l2acc;
     __syncthreads();
-    if (tid == 0) a.l2_partial[bid] = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+    if (tid == 0) a.l2_partial[bid] = wave_sum<NT / 64>(s_red);
   }
   if (STAGE == 2 && a.l2_partial != nullptr && bid == 0) {
     double acc = 0.0;
-    for (unsigned i = tid; i < nblk; i += 256) acc += a.l2_partial[i];
+    for (unsigned i = tid; i < nblk; i += NT) acc += a.l2_partial[i];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     __syncthreads();
     if ((tid & 63) == 0) s_red[tid >> 6] = acc;
     __syncthreads();
     if (tid == 0) {
-      const double S = (s_red[0] + s_red[1]) + (s_red[2] + s_red[3]);
+      const double S = wave_sum<NT / 64>(s_red);
       const double l = log10(1e-16 + sqrt(S / a.l2_volume));
       SolverDeviceState *st = a.state;
       st->steps = st->nchecks;  // completed steps when stage 1 read y_n
@@ -240,9 +248,9 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
 template <typename T, int STAGE>
 __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, const int plane) {
   constexpr int TYB = NRPYELL_BC_ROWS, BYB = TYB + 2 * NG, NBX = 12, NBOX = NBX * BYB;
-  constexpr int NRB = (NBOX + 255) / 256, NGEO = (24 * TYB + 255) / 256;
-  static_assert(12 * TYB <= 256 && 10 * TYB <= 256, "one BC work item per thread");
-  __shared__ T bu[NRB * 256], bv[NRB * 256];  // [by][c]: u, v at column ilast-6+c, row i1s-NG+by
+  constexpr int NT = NRPYELL_BLOCK, NRB = (NBOX + NT - 1) / NT, NGEO = (24 * TYB + NT - 1) / NT;
+  static_assert(12 * TYB <= NT && 10 * TYB <= NT, "one BC work item per thread");
+  __shared__ T bu[NRB * NT], bv[NRB * NT];  // [by][c]: u, v at column ilast-6+c, row i1s-NG+by
   __shared__ T s_rhs0[TYB][2], s_A[TYB][2][6], s_geo[6][TYB][4];
 
   const GridDims &g = a.g;
@@ -256,7 +264,7 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   T st_u[NRB], st_v[NRB];
 #pragma unroll
   for (int r = 0; r < NRB; r++) {
-    const int k = tid + 256 * r, c = k % NBX, by = min(k / NBX, BYB - 1);
+    const int k = tid + NT * r, c = k % NBX, by = min(k / NBX, BYB - 1);
     const unsigned q = pl + (unsigned)min(i1s - NG + by, g.G1 - 1) * s1 + max(ilast - NG - 1 + c, 0);
     st_u[r] = ld32(a.in, q);
     st_v[r] = ld32(a.in, nt + q);
@@ -264,7 +272,7 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   T geo_r[NGEO];
 #pragma unroll
   for (int r = 0; r < NGEO; r++) {
-    const int k = min(tid + 256 * r, 24 * TYB - 1);
+    const int k = min(tid + NT * r, 24 * TYB - 1);
     const int pt = k / (4 * TYB), row = (k / 4) % TYB, comp = k % 4;
     geo_r[r] = ld32(a.geo, ((unsigned)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp);
   }
@@ -291,12 +299,12 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   if (done) return;
 #pragma unroll
   for (int r = 0; r < NRB; r++) {
-    bu[tid + 256 * r] = st_u[r];
-    bv[tid + 256 * r] = st_v[r];
+    bu[tid + NT * r] = st_u[r];
+    bv[tid + NT * r] = st_v[r];
   }
 #pragma unroll
   for (int r = 0; r < NGEO; r++)
-    if (tid + 256 * r < 24 * TYB) (&s_geo[0][0][0])[tid + 256 * r] = geo_r[r];
+    if (tid + NT * r < 24 * TYB) (&s_geo[0][0][0])[tid + NT * r] = geo_r[r];
   __syncthreads();
 
   if (tid < TYB && i1s + tid < g.N1 + NG) {
@@ -358,7 +366,7 @@ __device__ __forceinline__ unsigned xcd_logical(unsigned L, unsigned n) {
 // One configuration: the nbc_rows*N2 BC blocks first (dispatched first, spread over the XCDs),
 // then the ntile0*ntile1*(N2/PL) interior blocks in XCD-aware order.
 template <typename T, int STAGE, int TX, int PL>
-__global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
+__global__ void __launch_bounds__(NRPYELL_BLOCK) substep_kernel(SubstepArgs<T> a) {
   const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
   if (L < nb) {
     bc_body<T, STAGE>(a, L % a.nbc_rows, L / a.nbc_rows);
@@ -373,7 +381,7 @@ __global__ void __launch_bounds__(256) substep_kernel(SubstepArgs<T> a) {
 // its interior blocks). Configurations that have converged return at once (their own done
 // flag) and are dropped from the launch when the host compacts the batch.
 template <typename T, int STAGE, int TX, int PL>
-__global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<T> *__restrict__ cfgs,
+__global__ void __launch_bounds__(NRPYELL_BLOCK) substep_batched_kernel(const SubstepArgs<T> *__restrict__ cfgs,
                                                               const int *__restrict__ block_cfg,
                                                               const int *__restrict__ block_first) {
   const unsigned q = xcd_logical(blockIdx.x, gridDim.x);
@@ -391,7 +399,7 @@ __global__ void __launch_bounds__(256) substep_batched_kernel(const SubstepArgs<
 
 template <typename T, int STAGE>
 static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
-  dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(256);
+  dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(NRPYELL_BLOCK);
   if (a.pl == 2) {
     switch (a.tx) {
       case 64: hipLaunchKernelGGL((substep_kernel<T, STAGE, 64, 2>), grd, blk, 0, st, a); break;
@@ -409,7 +417,7 @@ static void launch_stage(const SubstepArgs<T> &a, hipStream_t st) {
 template <typename T, int STAGE>
 static void launch_stage_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first,
                                  int nblocks, int tx, int pl, hipStream_t st) {
-  dim3 grd(nblocks), blk(256);
+  dim3 grd(nblocks), blk(NRPYELL_BLOCK);
   if (pl == 2) {
     switch (tx) {
       case 64: hipLaunchKernelGGL((substep_batched_kernel<T, STAGE, 64, 2>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
