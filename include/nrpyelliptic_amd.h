@@ -330,6 +330,43 @@ int nrpyell_mlp_sample_rows(const void *x_src, const float *t_src, long n_src, l
                             void *x_dst, float *t_dst);
 int nrpyell_memcpy_d2d(void *dst, const void *src, size_t bytes);
 
+/* ---- Einstein Toolkit consumer path (NRPyEllipticET/src/conformally_flat_BBH/) ---------- */
+
+/* Options of the thorn's Initialize_ADMBase (NRPyEllipticET/param.ccl:39-88,142-145). */
+#define NRPYELL_ORBITAL_PLANE_XY 0 /* "xy"/"yx" (default): (x,y,z)_NRPy = (y,z,x)_ETK */
+#define NRPYELL_ORBITAL_PLANE_XZ 1 /* "xz"/"zx": same axes                             */
+#define NRPYELL_LAPSE_PSI_N 0      /* "NRPyEllipticET-psi^n": alpha = psi^n             */
+#define NRPYELL_LAPSE_AVERAGED 1   /* "NRPyEllipticET-averaged"                         */
+#define NRPYELL_ADM_NOUT 17
+typedef struct {
+  int orbital_plane;
+  REAL position_shift[3];
+  int initial_lapse;
+  REAL lapse_exponent_n;   /* param.ccl default -2            */
+  REAL NRPy_epsilon;       /* param.ccl default 1e-8          */
+  int interpolation_order; /* Lagrange order 1..6, default 4  */
+} nrpyell_admbase_options;
+
+/* Replaces NRPyEllipticET_conformally_flat_BBH_Initialize_ADMBase
+ * (NRPyEllipticET_conformally_flat_BBH_Initialize_ADMBase.c:5-26), with its interpolation call
+ * (..._interpolate_solution_to_ADMBase.c:4-61) and ADM formulas
+ * (conformally_flat_BBH_compute_ADM_Cartesian_quantities_from_uu.c:7-103). griddata holds the
+ * solved u (y_n_gfs, ghosts filled, as every nrpyell solve leaves it), the SinhSymTP xx (host)
+ * and the NRPy-frame parameters the solve ran with. cctk_lsh[0]*[1]*[2] points; x, y, z and
+ * the 17 outputs are device arrays. Synchronous. NRPYELL_ENAN when a point's coordinates stay
+ * NaN after the NRPy_epsilon retry (the thorn aborts there), NRPYELL_EINVAL for bad options or
+ * a point outside the grid. */
+int nrpyell_Initialize_ADMBase(const griddata_struct *griddata, const nrpyell_admbase_options *opt,
+                               const int *cctk_lsh, const REAL *x, const REAL *y, const REAL *z, REAL *uuGF,
+                               REAL *alp, REAL *betax, REAL *betay, REAL *betaz, REAL *gxx, REAL *gxy, REAL *gxz,
+                               REAL *gyy, REAL *gyz, REAL *gzz, REAL *kxx, REAL *kxy, REAL *kxz, REAL *kyy,
+                               REAL *kyz, REAL *kzz);
+/* conformally_flat_BBH_compute_TRC (conformally_flat_BBH_set_optimum_eta_damping.c:3-16) */
+REAL compute_TRC(const REAL A, const REAL b, const REAL C_0, const REAL dt, const REAL dxx1, const REAL dxx2);
+/* conformally_flat_BBH_set_optimum_eta_damping (conformally_flat_BBH_set_optimum_eta_damping.c:18-27):
+ * params->eta_damping = 6.9 TRC^-0.79 - 0.7 from AMAX, bScale, dxx1, dxx2 */
+void set_optimum_eta_damping(const REAL dt, const REAL CFL_FACTOR, paramstruct *params);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,31 @@ struct Punctures {
   double S[2][3];
 };
 
+// The Bowen-York conformal extrinsic curvature at the Cartesian point X, and h_n = m_n/(2 r_n)
+// (psi_bg = 1 + h_0 + h_1). Per puncture, with x = X - x_n and r = |x|:
+//   A_ij = (3/2)(P_i x_j + P_j x_i)/r^3 - (3/2)(x.P) delta_ij/r^3 + (3/2)(x.P) x_i x_j/r^5
+//          - 3[(x cross S)_i x_j + (x cross S)_j x_i]/r^5
+// (NRPyElliptic_SourceTerm.py:48-143 differentiated by hand).
+__host__ __device__ inline void bowen_york(const Punctures &pu, const double X[3], double A[3][3], double h[2]) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) A[i][j] = 0.0;
+  for (int n = 0; n < 2; n++) {
+    const double x[3] = {X[0] - pu.pos[n][0], X[1] - pu.pos[n][1], X[2] - pu.pos[n][2]};
+    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], r = sqrt(r2);
+    const double ir3 = 1.0 / (r2 * r), ir5 = ir3 / r2;
+    const double *P = pu.P[n], *S = pu.S[n];
+    const double xP = x[0] * P[0] + x[1] * P[1] + x[2] * P[2];
+    const double xS[3] = {x[1] * S[2] - x[2] * S[1], x[2] * S[0] - x[0] * S[2], x[0] * S[1] - x[1] * S[0]};
+    h[n] = pu.m[n] / (2.0 * r);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        A[i][j] += 1.5 * (P[i] * x[j] + P[j] * x[i]) * ir3 - (i == j ? 1.5 * xP * ir3 : 0.0) +
+                   1.5 * xP * x[i] * x[j] * ir5 - 3.0 * (xS[i] * x[j] + xS[j] * x[i]) * ir5;
+  }
+}
+
 // One outer point for the drop-in radiation BC: point p, interior neighbour q,
 // clamped stencil offsets along x0,x1,x2 for each, geometry r, dxx0/dr, dxx1/dr, dxx2/dr.
 struct RadPoint {

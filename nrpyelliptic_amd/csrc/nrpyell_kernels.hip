@@ -439,10 +439,7 @@ void launch_rfm(const GridDims &g, double AMAX, double w, double b, const double
 }
 
 // auxevol_gfs_all_points.c:9-95 at every point: psi_bg = 1 + sum m_n/(2 r_n) and the
-// Bowen-York A_ij A_ij with, per puncture (x relative to it, r=|x|),
-//   A_ij = (3/2)(P_i x_j + P_j x_i)/r^3 - (3/2)(x.P) delta_ij/r^3 + (3/2)(x.P) x_i x_j/r^5
-//          - 3[(x cross S)_i x_j + (x cross S)_j x_i]/r^5
-// (NRPyElliptic_SourceTerm.py:48-143 differentiated by hand).
+// Bowen-York A_ij A_ij (bowen_york).
 __global__ void auxevol_kernel(GridDims g, SinhParams s, Punctures pu, const double *xx0, const double *xx1,
                                const double *xx2, double *aux) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
@@ -451,22 +448,8 @@ __global__ void auxevol_kernel(GridDims g, SinhParams s, Punctures pu, const dou
   const double x0 = xx0[i0], x1 = xx1[i1], x2 = xx2[i2];
   const double f0 = s.AMAX * (exp(x0 / s.w) - exp(-x0 / s.w)) / s.D;
   const double X[3] = {f0 * sin(x1) * cos(x2), f0 * sin(x1) * sin(x2), sqrt(f0 * f0 + s.b * s.b) * cos(x1)};
-  double A[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, psi = 1.0;
-  for (int n = 0; n < 2; n++) {
-    const double x[3] = {X[0] - pu.pos[n][0], X[1] - pu.pos[n][1], X[2] - pu.pos[n][2]};
-    const double r2 = x[0] * x[0] + x[1] * x[1] + x[2] * x[2], r = sqrt(r2);
-    const double ir3 = 1.0 / (r2 * r), ir5 = ir3 / r2;
-    const double *P = pu.P[n], *S = pu.S[n];
-    const double xP = x[0] * P[0] + x[1] * P[1] + x[2] * P[2];
-    const double xS[3] = {x[1] * S[2] - x[2] * S[1], x[2] * S[0] - x[0] * S[2], x[0] * S[1] - x[1] * S[0]};
-    psi += pu.m[n] / (2.0 * r);
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-        A[i][j] += 1.5 * (P[i] * x[j] + P[j] * x[i]) * ir3 - (i == j ? 1.5 * xP * ir3 : 0.0) +
-                   1.5 * xP * x[i] * x[j] * ir5 - 3.0 * (xS[i] * x[j] + xS[j] * x[i]) * ir5;
-  }
+  double A[3][3], h[2];
+  bowen_york(pu, X, A, h);
   double AA = 0.0;
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -474,7 +457,7 @@ __global__ void auxevol_kernel(GridDims g, SinhParams s, Punctures pu, const dou
     for (int j = 0; j < 3; j++) AA += A[i][j] * A[i][j];
   const long p = idx3(g, i0, i1, i2);
   aux[ADD_TIMES_AUUGF * g.ntot + p] = AA;
-  aux[PSI_BACKGROUNDGF * g.ntot + p] = psi;
+  aux[PSI_BACKGROUNDGF * g.ntot + p] = (1.0 + h[0]) + h[1];
 }
 
 void launch_auxevol(const GridDims &g, double AMAX, double w, double b, const Punctures &pu, const double *xx0_d,

@@ -32,6 +32,16 @@ class solve_options(C.Structure):
                 ("tile_i0", C.c_int), ("planes", C.c_int)]
 
 
+class admbase_options(C.Structure):
+    """nrpyell_admbase_options: the thorn's Initialize_ADMBase parameters."""
+    _fields_ = [("orbital_plane", C.c_int), ("position_shift", C.c_double * 3), ("initial_lapse", C.c_int),
+                ("lapse_exponent_n", C.c_double), ("NRPy_epsilon", C.c_double), ("interpolation_order", C.c_int)]
+
+
+ADMBASE_NAMES = ["uuGF", "alp", "betax", "betay", "betaz", "gxx", "gxy", "gxz", "gyy", "gyz", "gzz",
+                 "kxx", "kxy", "kxz", "kyy", "kyz", "kzz"]
+
+
 class solve_result(C.Structure):
     _fields_ = [("steps", C.c_int), ("fp32_steps", C.c_int), ("log10_l2", C.c_double),
                 ("log10_l2_initial", C.c_double), ("seconds", C.c_double), ("converged", C.c_int)]
@@ -80,6 +90,12 @@ def lib() -> C.CDLL:
                                        C.POINTER(vp)]
     L.nrpyell_batch_solve.argtypes = [vp, C.POINTER(solve_result), C.POINTER(C.c_double)]
     L.nrpyell_batch_destroy.argtypes = [vp]
+    L.nrpyell_Initialize_ADMBase.argtypes = ([C.POINTER(abi.griddata_struct), C.POINTER(admbase_options),
+                                              C.POINTER(C.c_int)] + [abi.REALP] * 20)
+    L.compute_TRC.argtypes = [C.c_double] * 6
+    L.compute_TRC.restype = C.c_double
+    L.set_optimum_eta_damping.argtypes = [C.c_double, C.c_double, C.POINTER(abi.paramstruct)]
+    L.set_optimum_eta_damping.restype = None
     _LIB = L
     return L
 
@@ -262,6 +278,48 @@ class Problem:
                 outer.append([(arr[k].i0, arr[k].i1, arr[k].i2, arr[k].FACEX0, arr[k].FACEX1, arr[k].FACEX2)
                               for k in range(m)])
         return rec, outer
+
+    # --- Einstein Toolkit consumer path ----------------------------------------------------
+    def set_optimum_eta_damping(self) -> float:
+        """The thorn's eta_damping = -1 policy (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:146-156):
+        the fitted optimum, available only for domain_size 1e6, sinh_width 0.07, foci_position 5."""
+        p = self.gd.params
+        if not (p.AMAX == 1e6 and p.SINHWAA == 0.07 and p.bScale == 5):
+            raise NrpyellError("Automatic computation of the optimum eta_damping parameter is only available if "
+                               f"domain_size = 1e6 (got {p.AMAX:e}), sinh_width = 0.07 (got {p.SINHWAA:e}) and "
+                               f"foci_position = 5 (got {p.bScale:e})")
+        lib().set_optimum_eta_damping(self.dt, self.cfg.CFL, C.byref(p))
+        self._solver = None
+        return p.eta_damping
+
+    def initialize_admbase(self, x, y, z, orbital_plane: str = "xy", position_shift=(0.0, 0.0, 0.0),
+                           initial_lapse: str = "NRPyEllipticET-psi^n", lapse_exponent_n: float = -2.0,
+                           NRPy_epsilon: float = 1e-8, interpolation_order: int = 4) -> dict:
+        """ADMBase fields at the Cartesian points (x, y, z) from the current y_n (the thorn's
+        Initialize_ADMBase with its parameter defaults); returns the 17 arrays by name."""
+        if orbital_plane not in ("xy", "yx", "xz", "zx"):
+            raise NrpyellError('Unsupported orbital plane. Available options are "xy" ("yx") or "xz" ("zx").')
+        if initial_lapse not in ("NRPyEllipticET-psi^n", "NRPyEllipticET-averaged"):
+            raise NrpyellError(f"NRPyEllipticET does not support the lapse initial data: {initial_lapse}")
+        o = admbase_options()
+        o.orbital_plane = 0 if orbital_plane in ("xy", "yx") else 1
+        for d in range(3):
+            o.position_shift[d] = float(position_shift[d])
+        o.initial_lapse = 0 if initial_lapse == "NRPyEllipticET-psi^n" else 1
+        o.lapse_exponent_n, o.NRPy_epsilon, o.interpolation_order = lapse_exponent_n, NRPy_epsilon, interpolation_order
+        pts = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (x, y, z)]
+        n = pts[0].size
+        if any(a.size != n for a in pts):
+            raise NrpyellError("x, y, z must have the same size")
+        lsh = (C.c_int * 3)(n, 1, 1)
+        dev = [DeviceArray(max(n, 1)).upload(a) if n else DeviceArray(1) for a in pts]
+        outs = [DeviceArray(max(n, 1)) for _ in ADMBASE_NAMES]
+        check(lib().nrpyell_Initialize_ADMBase(C.byref(self.gd), C.byref(o), lsh, *[d.as_real_p() for d in dev],
+                                               *[d.as_real_p() for d in outs]), "Initialize_ADMBase")
+        res = {k: d.download()[:n] for k, d in zip(ADMBASE_NAMES, outs)}
+        for d in dev + outs:
+            d.free()
+        return res
 
     # --- resident solver ----------------------------------------------------------------------
     def solver(self, **opts) -> "Solver":

@@ -40,6 +40,26 @@ class OrcBc(C.Structure):
                 ("num_outer", C.c_int * 15), ("outer", C.POINTER(abi.outerpt_bc_struct) * 15)]
 
 
+class OrcAdmOpts(C.Structure):
+    _fields_ = [("orbital_plane", C.c_int), ("position_shift", C.c_double * 3), ("initial_lapse", C.c_int),
+                ("lapse_exponent", C.c_double), ("nrpy_epsilon", C.c_double), ("interp_order", C.c_int)]
+
+
+ADM_NAMES = ["uuGF", "alp", "betax", "betay", "betaz", "gxx", "gxy", "gxz", "gyy", "gyz", "gzz",
+             "kxx", "kxy", "kxz", "kyy", "kyz", "kzz"]
+
+
+def adm_opts(orbital_plane="xy", position_shift=(0.0, 0.0, 0.0), initial_lapse="psi^n", lapse_exponent=-2.0,
+             nrpy_epsilon=1e-8, interp_order=4) -> "OrcAdmOpts":
+    o = OrcAdmOpts()
+    o.orbital_plane = 0 if orbital_plane in ("xy", "yx") else 1
+    for d in range(3):
+        o.position_shift[d] = position_shift[d]
+    o.initial_lapse = 0 if initial_lapse in ("psi^n", "NRPyEllipticET-psi^n") else 1
+    o.lapse_exponent, o.nrpy_epsilon, o.interp_order = lapse_exponent, nrpy_epsilon, interp_order
+    return o
+
+
 def _p(a):
     return a.ctypes.data_as(DP)
 
@@ -83,6 +103,17 @@ def lib():
         L.orc_l2_norm.restype = C.c_double
         L.orc_fd1_weight.argtypes = [C.c_int, C.c_int]
         L.orc_fd1_weight.restype = C.c_double
+        L.orc_get_xx_from_xyz.argtypes = [C.c_double, C.c_double, C.c_double, DP, DP]
+        L.orc_get_xx_from_xyz.restype = None
+        L.orc_lagrange_interp3.argtypes = [C.POINTER(C.c_int), DP, DP, DP, C.c_int, DP, DP]
+        L.orc_adm_point.argtypes = [G, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, DP]
+        L.orc_adm_point.restype = None
+        L.orc_initialize_admbase.argtypes = [G, DP, DP, DP, DP, C.POINTER(OrcAdmOpts), C.c_long, DP, DP, DP,
+                                             C.POINTER(DP)]
+        L.orc_compute_TRC.argtypes = [C.c_double] * 6
+        L.orc_compute_TRC.restype = C.c_double
+        L.orc_optimum_eta.argtypes = [C.c_double] * 6
+        L.orc_optimum_eta.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -132,6 +163,20 @@ class OracleProblem:
         L.orc_wavespeed(C.byref(g), cfg.CFL, self.dt, *[_p(a) for a in self.xx], _p(self.aux))
         c = L.orc_wavespeed_at_OB(C.byref(g), _p(self.aux))
         g.cOB[0] = g.cOB[1] = c
+
+    def initialize_admbase(self, u, x, y, z, **opts):
+        """The Einstein Toolkit consumer path on the oracle's grid: u (gridfunction 0 with
+        ghosts) -> dict of the 17 ADMBase arrays at the points (x, y, z)."""
+        o = adm_opts(**opts)
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        u = np.ascontiguousarray(u[: self.ntot], dtype=np.float64)
+        outs = [np.zeros(len(x)) for _ in ADM_NAMES]
+        optr = (DP * len(outs))(*[_p(a) for a in outs])
+        rc = lib().orc_initialize_admbase(C.byref(self.g), *[_p(a) for a in self.xx], _p(u), C.byref(o), len(x),
+                                          _p(x), _p(y), _p(z), optr)
+        if rc != 0:
+            raise RuntimeError(f"oracle initialize_admbase: rc {rc}")
+        return dict(zip(ADM_NAMES, outs))
 
     def _alloc(self, T, n):
         arr = (T * n)()

@@ -25,6 +25,21 @@
  *                          SinhSymTP map, not from the reference's CSE)
  *   orc_mol_step           <=1e-13 rel vs oracle/_ref after 1 and 20 steps
  *   orc_l2_norm            <=1e-13 abs on log10 vs oracle/_ref (1 OpenMP thread)
+ *
+ * Einstein Toolkit consumer path (NRPyEllipticET; its sources need the Cactus headers, so
+ * they cannot be compiled here, tests/test_oracle_adm.py):
+ *   orc_get_xx_from_xyz    vs the compiled reference's SymTP inverse map
+ *                          Cart_to_xx_and_nearest_i0i1i2 (sqrt arguments bit-identical):
+ *                          xx0 = w asinh(xx0_SymTP sinh(1/w)/AMAX), cos xx1 = 0.99999999 cos xx1_SymTP
+ *                          to 1e-15; round trip through the SinhSymTP map
+ *   orc_lagrange_interp3   exact for tensor polynomials of degree <= order; molecule placement
+ *                          of the Cactus AEILocalInterp "Lagrange polynomial interpolation"
+ *                          operator (not in the reference tree: placement parity unpinned)
+ *   orc_adm_point          psi^12 K_ij K_ij = the compiled reference's auxevol AA^2 at grid
+ *                          points (<=1e-12 rel); K_ij traceless; surface integrals
+ *                          (1/8pi) \oint K_ij n^j dA = P_i and (1/8pi) \oint eps_ijk x^j K^kl n_l dA = S_i
+ *                          around a single puncture; lapse closed forms
+ *   orc_compute_TRC        atanh written as a log; eta(TRC = 1) = 6.2
  */
 #include "oracle.h"
 
@@ -609,4 +624,172 @@ double orc_l2_norm(const orc_grid *g, double radius, const double *xx0, const do
         }
       }
   return log10(1e-16 + sqrt(S / V));
+}
+
+/* =====================================================================================
+ * Einstein Toolkit consumer path: the converged u on the SinhSymTP grid -> ADM fields on
+ * the evolution code's Cartesian points (NRPyEllipticET_conformally_flat_BBH_Initialize_ADMBase.c:5-200).
+ * ===================================================================================== */
+
+/* conformally_flat_BBH_get_xx_from_xyz.c:6-20: the closed-form inverse of the SinhSymTP map.
+ * With R2 = x^2+y^2+z^2 and Q = sqrt((R2 - b^2)^2 + 4 b^2 (x^2+y^2)) (expanded as the
+ * reference expands it):
+ *   xx0 = w asinh( sqrt((R2 - b^2 + Q)/2) sinh(1/w) / AMAX )   (the SymTP radial coordinate,
+ *         pulled back through the sinh stretch),
+ *   xx1 = acos( 0.99999999 sign(z) sqrt((1 + (R2 - Q)/b^2)/2) ) (the factor keeps xx1 off the
+ *         axis), xx2 = atan2(y, x). */
+void orc_get_xx_from_xyz(double AMAX, double SINHWAA, double bScale, const double X[3], double xx[3]) {
+  const double b2 = bScale * bScale;
+  const double R2 = X[0] * X[0] + X[1] * X[1] + X[2] * X[2];
+  const double Q = sqrt(-4 * (X[2] * X[2]) * b2 + bScale * bScale * bScale * bScale + 2 * b2 * R2 + R2 * R2);
+  const double ib2 = 1.0 / b2;
+  const double sgn = (double)((X[2] > 0) - (X[2] < 0));
+  xx[0] = SINHWAA * asinh(M_SQRT1_2 * sqrt(-b2 + R2 + Q) * sinh(1.0 / SINHWAA) / AMAX);
+  xx[1] = acos(0.99999999 * M_SQRT1_2 * sqrt(R2 * ib2 - Q * ib2 + 1) * sgn);
+  xx[2] = atan2(X[1], X[0]);
+}
+
+/* Lagrange interpolation with the molecule placement of the Cactus thorn AEILocalInterp
+ * (Einstein Toolkit; "Lagrange polynomial interpolation", which
+ * NRPyEllipticET_conformally_flat_BBH_interpolate_solution_to_ADMBase.c:14-55 requests with
+ * "order=<interpolation_order>", param.ccl:67-75). Per axis: fp = (x - origin)/delta in grid
+ * units; a molecule of order+1 points spans m = m_min..m_min+order, m_min = -floor(order/2),
+ * around the nearest point (odd point count) or the point below x (even count); it is moved
+ * inward where it would leave the array; points more than 1e-10 grid spacings outside the
+ * array are an error. Weights are the Lagrange basis polynomials at fp - centre. */
+static int lagrange_axis(int n, double origin, double delta, int order, double x, int *first, double w[8]) {
+  const double fp = (x - origin) / delta;
+  if (!(fp >= -1e-10 && fp <= (double)(n - 1) + 1e-10)) return -1;
+  const int mmin = -(order / 2), mmax = mmin + order;
+  int c = (order % 2 == 0) ? (int)floor(fp + 0.5) : (int)floor(fp);
+  if (c + mmin < 0) c = -mmin;
+  if (c + mmax > n - 1) c = n - 1 - mmax;
+  const double t = fp - (double)c;
+  for (int m = mmin; m <= mmax; m++) {
+    double v = 1.0;
+    for (int k = mmin; k <= mmax; k++)
+      if (k != m) v *= (t - (double)k) / (double)(m - k);
+    w[m - mmin] = v;
+  }
+  *first = c + mmin;
+  return 0;
+}
+
+int orc_lagrange_interp3(const int dims[3], const double origin[3], const double delta[3], const double *data,
+                         int order, const double pt[3], double *out) {
+  if (order < 1 || order > 6) return -1;
+  int f[3];
+  double w[3][8];
+  for (int d = 0; d < 3; d++)
+    if (lagrange_axis(dims[d], origin[d], delta[d], order, pt[d], &f[d], w[d])) return -1;
+  double acc = 0.0;
+  for (int k = 0; k <= order; k++) {
+    double a1 = 0.0;
+    for (int j = 0; j <= order; j++) {
+      double a0 = 0.0;
+      const double *row = data + (long)f[0] + (long)dims[0] * ((long)(f[1] + j) + (long)dims[1] * (long)(f[2] + k));
+      for (int i = 0; i <= order; i++) a0 += w[0][i] * row[i];
+      a1 += w[1][j] * a0;
+    }
+    acc += w[2][k] * a1;
+  }
+  *out = acc;
+  return 0;
+}
+
+/* conformally_flat_BBH_compute_ADM_Cartesian_quantities_from_uu.c:7-103 and the two lapse
+ * choices (conformally_flat_BBH_lapse_power_of_psi_initial_data.c:6-35,
+ * conformally_flat_BBH_lapse_averaged_initial_data.c:6-41), at one point of the ETK grid:
+ *   psi = 1 + uu + sum_n m_n/(2 r_n),  gamma_ij = psi^4 delta_ij,  beta^i = 0,
+ *   K_ij = psi^-2 A_ij with the Bowen-York A_ij of orc_auxevol,
+ *   alpha = psi^n (psi^n lapse) or 1/2 + (1 - F)/(2 (1 + F)), F = sum_n m_n/(2 r_n) (averaged). */
+void orc_adm_point(const orc_grid *etk, int lapse, double lapse_n, double x, double y, double z, double uu,
+                   double out[ORC_ADM_NOUT]) {
+  const double X[3] = {x, y, z};
+  double A[3][3] = {{0}}, h[2];
+  for (int n = 0; n < 2; n++) {
+    const double r[3] = {X[0] - etk->pos[n][0], X[1] - etk->pos[n][1], X[2] - etk->pos[n][2]};
+    const double r2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2], rr = sqrt(r2);
+    const double ir3 = 1.0 / (r2 * rr), ir5 = ir3 / r2;
+    const double *P = etk->P[n], *S = etk->S[n];
+    const double rP = r[0] * P[0] + r[1] * P[1] + r[2] * P[2];
+    const double rS[3] = {r[1] * S[2] - r[2] * S[1], r[2] * S[0] - r[0] * S[2], r[0] * S[1] - r[1] * S[0]};
+    h[n] = etk->mass[n] / (2.0 * rr);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++)
+        A[i][j] += 1.5 * (P[i] * r[j] + P[j] * r[i]) * ir3 - (i == j ? 1.5 * rP * ir3 : 0.0) +
+                   1.5 * rP * r[i] * r[j] * ir5 - 3.0 * (rS[i] * r[j] + rS[j] * r[i]) * ir5;
+  }
+  /* sums in the reference's order: psi = ((uu + 1) + h0) + h1, psi^4 = ((psi psi) psi) psi */
+  const double F = h[0] + h[1];
+  const double psi = ((uu + 1.0) + h[0]) + h[1], psi4 = psi * psi * psi * psi, ipsi2 = 1.0 / (psi * psi);
+  out[0] = uu;
+  out[1] = lapse == ORC_LAPSE_AVERAGED ? 0.5 * (1.0 - F) / (F + 1.0) + 0.5 : pow((F + uu) + 1.0, lapse_n);
+  out[2] = out[3] = out[4] = 0.0;
+  static const int ij[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  for (int c = 0; c < 6; c++) {
+    out[5 + c] = ij[c][0] == ij[c][1] ? psi4 : 0.0;
+    out[11 + c] = ipsi2 * A[ij[c][0]][ij[c][1]];
+  }
+}
+
+/* NRPyEllipticET_conformally_flat_BBH_Initialize_ADMBase.c:35-200. The solve ran in the NRPy
+ * frame; with orbital plane "xy" the relaxation driver mapped the parameter file's punctures
+ * (x,y,z)_NRPy = (y,z,x)_ETK (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:84-105),
+ * so the ETK-frame punctures the ADM formulas use are the inverse permutation of g's, plus
+ * position_shift. Each ETK point, less the shift, is permuted the same way into the NRPy frame,
+ * mapped to (xx0,xx1,xx2) (retried once, moved by nrpy_epsilon on every axis, if that gives a
+ * NaN) and u is interpolated there. */
+int orc_initialize_admbase(const orc_grid *g, const double *xx0, const double *xx1, const double *xx2,
+                           const double *u, const orc_adm_opts *o, long n, const double *x, const double *y,
+                           const double *z, double *const out[ORC_ADM_NOUT]) {
+  orc_grid etk = *g;
+  const int xy = o->orbital_plane == ORC_PLANE_XY;
+  for (int p = 0; p < 2; p++) {
+    double *v[3] = {etk.pos[p], etk.P[p], etk.S[p]};
+    const double *w[3] = {g->pos[p], g->P[p], g->S[p]};
+    for (int q = 0; q < 3; q++) {
+      if (xy) {  /* ETK (x,y,z) = NRPy (z,x,y) */
+        v[q][0] = w[q][2];
+        v[q][1] = w[q][0];
+        v[q][2] = w[q][1];
+      }
+    }
+    for (int d = 0; d < 3; d++) etk.pos[p][d] += o->position_shift[d];
+  }
+  const int dims[3] = {g->Ng[0], g->Ng[1], g->Ng[2]};
+  const double origin[3] = {xx0[0], xx1[0], xx2[0]};
+  const double delta[3] = {xx0[1] - xx0[0], xx1[1] - xx1[0], xx2[1] - xx2[0]};
+  int rc = 0;
+  for (long i = 0; i < n; i++) {
+    const double s[3] = {x[i] - o->position_shift[0], y[i] - o->position_shift[1], z[i] - o->position_shift[2]};
+    double X[3], xs[3];
+    if (xy) { X[0] = s[1]; X[1] = s[2]; X[2] = s[0]; }
+    else { X[0] = s[0]; X[1] = s[1]; X[2] = s[2]; }
+    orc_get_xx_from_xyz(g->AMAX, g->SINHWAA, g->bScale, X, xs);
+    if (isnan(xs[0] * xs[1] * xs[2])) {
+      for (int d = 0; d < 3; d++) X[d] += o->nrpy_epsilon;
+      orc_get_xx_from_xyz(g->AMAX, g->SINHWAA, g->bScale, X, xs);
+      if (isnan(xs[0] * xs[1] * xs[2])) return -1;
+    }
+    double uu;
+    if (orc_lagrange_interp3(dims, origin, delta, u, o->interp_order, xs, &uu)) return -2;
+    double v[ORC_ADM_NOUT];
+    orc_adm_point(&etk, o->initial_lapse, o->lapse_exponent, x[i], y[i], z[i], uu, v);
+    for (int c = 0; c < ORC_ADM_NOUT; c++) out[c][i] = v[c];
+  }
+  return rc;
+}
+
+/* conformally_flat_BBH_set_optimum_eta_damping.c:3-27: the relaxation time-scale ratio of
+ * arXiv:2111.02424 eq. (B6),
+ *   TRC = dt/(C_0 dxx1) atanh(A/sqrt(A^2 + b^2 cos^2(dxx1/2))) cos(dxx1/2) cos(dxx2/2),
+ * and the fitted optimum damping of its eq. (36), eta = 6.9 TRC^-0.79 - 0.7. */
+double orc_compute_TRC(double A, double b, double C_0, double dt, double dxx1, double dxx2) {
+  const double c1 = cos(dxx1 / 2.0), c2 = cos(dxx2 / 2.0);
+  const double t = A / sqrt(A * A + b * b * c1 * c1);
+  return dt / (C_0 * dxx1) * atanh(t) * c1 * c2;
+}
+double orc_optimum_eta(double AMAX, double bScale, double dxx1, double dxx2, double dt, double CFL) {
+  return 6.9 * pow(orc_compute_TRC(AMAX, bScale, CFL, dt, dxx1, dxx2), -0.79) - 0.7;
 }

@@ -83,4 +83,37 @@ double orc_l2_norm(const orc_grid *g, double radius, const double *xx0, const do
  * at i0+offset-5+j; exposed for the closed-form test. */
 double orc_fd1_weight(int offset, int j);
 
+/* ---- Einstein Toolkit consumer path (NRPyEllipticET/src/conformally_flat_BBH/) ---------- */
+
+/* Options of NRPyEllipticET_conformally_flat_BBH_Initialize_ADMBase (param.ccl:39-88,142). */
+#define ORC_PLANE_XY 0      /* "xy"/"yx": (x,y,z)_NRPy = (y,z,x)_ETK */
+#define ORC_PLANE_XZ 1      /* "xz"/"zx": same axes */
+#define ORC_LAPSE_PSI_N 0   /* "NRPyEllipticET-psi^n"   */
+#define ORC_LAPSE_AVERAGED 1 /* "NRPyEllipticET-averaged" */
+#define ORC_ADM_NOUT 17     /* uu, alp, betax..z, gxx gxy gxz gyy gyz gzz, kxx kxy kxz kyy kyz kzz */
+typedef struct {
+  int orbital_plane;
+  double position_shift[3];
+  int initial_lapse;
+  double lapse_exponent;
+  double nrpy_epsilon;
+  int interp_order;          /* Lagrange order, 1..6 */
+} orc_adm_opts;
+
+void   orc_get_xx_from_xyz(double AMAX, double SINHWAA, double bScale, const double X[3], double xx[3]);
+/* 3-D tensor-product Lagrange interpolation on a uniform grid; 0 on success, -1 outside the grid. */
+int    orc_lagrange_interp3(const int dims[3], const double origin[3], const double delta[3],
+                            const double *data, int order, const double pt[3], double *out);
+/* ADM quantities at one ETK point from uu (punctures of `etk` in the ETK frame, shifted);
+ * out[1..16] = alp, beta (3), gamma_ij (6), K_ij (6); out[0] = uu. */
+void   orc_adm_point(const orc_grid *etk, int lapse, double lapse_n, double x, double y, double z, double uu,
+                     double out[ORC_ADM_NOUT]);
+/* The whole consumer path on n points; g holds the NRPy-frame grid and punctures the solve ran
+ * with. Returns 0, -1 (unrecoverable NaN coordinate) or -2 (point outside the grid). */
+int    orc_initialize_admbase(const orc_grid *g, const double *xx0, const double *xx1, const double *xx2,
+                              const double *u, const orc_adm_opts *o, long n, const double *x,
+                              const double *y, const double *z, double *const out[ORC_ADM_NOUT]);
+double orc_compute_TRC(double A, double b, double C_0, double dt, double dxx1, double dxx2);
+double orc_optimum_eta(double AMAX, double bScale, double dxx1, double dxx2, double dt, double CFL);
+
 #endif
