@@ -61,7 +61,22 @@ def cpu_baseline(steps: int, warmup: int) -> dict:
         step()
     dt = time.perf_counter() - t0
     return {"value": 4 * 262144 * steps / dt, "unit": "gp-substeps/s", "kind": kind, "seconds": dt,
-            "ms_per_step": 1e3 * dt / steps}
+            "ms_per_step": 1e3 * dt / steps, "admbase": cpu_admbase(cfg)}
+
+
+def cpu_admbase(cfg, n=1 << 20):
+    """The consumer path (Cartesian points -> ADMBase) of the C restatement, one thread, on n
+    uniform points of the evolution grid's box (the thorn's own C needs Cactus and does not
+    build here)."""
+    import numpy as np
+    import oracle
+    O = oracle.OracleProblem(cfg)
+    X = np.random.default_rng(0).uniform(-20.0, 20.0, size=(n, 3))
+    u = np.zeros(O.ntot)
+    t0 = time.perf_counter()
+    O.initialize_admbase(u, X[:, 0], X[:, 1], X[:, 2])
+    dt = time.perf_counter() - t0
+    return {"points_per_s": n / dt, "points": n, "seconds": dt, "cores": 1, "kind": "port"}
 
 
 def run_cpu_baseline_subprocess(steps=5000, warmup=5):
@@ -114,6 +129,41 @@ def dropin_call_us(P, n=50):
             fn()
         out[name] = round(1e6 * (time.perf_counter() - t0) / n, 2)
     return out
+
+
+ADMBASE_BYTES_PER_POINT = 24 + 17 * 8  # read x, y, z; write the 17 ADMBase arrays
+
+
+def admbase_throughput(P, n_side=128, half=20.0, repeats=5):
+    """The Einstein Toolkit consumer path on the solved state: an n_side^3 Cartesian grid over
+    [-half, half]^3 (x fastest, as a Cactus grid function), points and outputs resident on the
+    device; wall time per synchronous nrpyell_Initialize_ADMBase call."""
+    import ctypes as C
+    import numpy as np
+    from nrpyelliptic_amd import runtime
+    g = np.linspace(-half, half, n_side)
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    n = X.size
+    dev = [runtime.DeviceArray(n).upload(a.ravel()) for a in (X, Y, Z)]
+    outs = [runtime.DeviceArray(n) for _ in runtime.ADMBASE_NAMES]
+    o = runtime.admbase_options()
+    o.orbital_plane, o.initial_lapse, o.lapse_exponent_n, o.NRPy_epsilon, o.interpolation_order = 0, 0, -2.0, 1e-8, 4
+    lsh = (C.c_int * 3)(n_side, n_side, n_side)
+    L = runtime.lib()
+
+    def call():
+        runtime.check(L.nrpyell_Initialize_ADMBase(C.byref(P.gd), C.byref(o), lsh, *[d.as_real_p() for d in dev],
+                                                   *[d.as_real_p() for d in outs]), "Initialize_ADMBase")
+    call()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        call()
+    sec = (time.perf_counter() - t0) / repeats
+    for d in dev + outs:
+        d.free()
+    return {"points": n, "ms_per_call": 1e3 * sec, "points_per_s": n / sec,
+            "bytes_per_point": ADMBASE_BYTES_PER_POINT,
+            "achieved_GBs": ADMBASE_BYTES_PER_POINT * n / sec / 1e9, "interpolation_order": 4}
 
 
 def pmc_probe():
@@ -225,6 +275,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--timed-only", action="store_true",
                     help="only the warmup and the timed steps (the region the profiles cover)")
+    ap.add_argument("--admbase-only", action="store_true",
+                    help="only the solve and the Einstein Toolkit consumer path (ADMBase on 128^3 points)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe()
@@ -246,6 +298,13 @@ def main():
     npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
     S = P.solver(steps_per_graph=100)
 
+    if args.admbase_only:
+        # 3000 graph-replayed steps relax u to ~1e-10 (the solve stops at 3144); the kernel's
+        # cost does not depend on the values
+        P.set("y_n_gfs", np.zeros(2 * P.ntot))
+        S.timed_steps(3000)
+        print(json.dumps(admbase_throughput(P)), flush=True)
+        return
     # warmup, then exactly --steps production steps from u = v = 0 (HIP events, solver stream)
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
     S.timed_steps(args.warmup)
@@ -270,9 +329,12 @@ def main():
     # per-stage kernel durations (back-to-back launches, HIP events on the solver stream)
     stage_us = {f"stage{k}": round(S.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)}
     dropin_us = dropin_call_us(P)
+    admbase = admbase_throughput(P)
     batched = batched_solve()
 
     cpu = None if args.no_cpu_baseline else run_cpu_baseline_subprocess()
+    if cpu is not None and isinstance(cpu.get("admbase"), dict):
+        admbase["cpu_oracle"] = cpu.pop("admbase")
     line = {
         "metric": METRIC,
         "value": value,
@@ -295,6 +357,7 @@ def main():
                      "us_per_launch": us_per_launch, "bytes_per_launch": BYTES_PER_GP_SUBSTEP * npts,
                      "stage_us": stage_us},
         "dropin_call_us": dropin_us,
+        "et_consumer_admbase": admbase,
         "batched_64_seed42": batched,
         "cpu_baseline": cpu,
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,

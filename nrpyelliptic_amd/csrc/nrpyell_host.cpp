@@ -8,7 +8,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <list>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "nrpyell_internal.h"
@@ -147,44 +151,81 @@ extern "C" void set_Nxx_dxx_invdx_params__and__xx(const int EigenCoord, const in
   }
 }
 
-int host_bcstruct(const paramstruct *p, REAL *xx[3], bc_struct *bc, std::string *err) {
-  const int G0 = p->Nxx_plus_2NGHOSTS0, G1 = p->Nxx_plus_2NGHOSTS1, G2 = p->Nxx_plus_2NGHOSTS2;
-  memset(bc, 0, sizeof(*bc));
-  // Step 1 (bcstruct_set_up.c:355-406): inner points in storage order.
+namespace {
+
+// The tables of one grid, as host_bcstruct hands them out.
+struct BcTables {
   std::vector<innerpt_bc_struct> inner;
-  inner.reserve((size_t)G0 * G1 * G2 / 2);
-  for (int i2 = 0; i2 < G2; i2++)
+  std::vector<outerpt_bc_struct> outer[NG * 3];
+  bc_info_struct info;
+};
+
+// bcstruct_set_up.c:355-520 on the host. The per-point work (the SymTP round trip of every
+// ghost point and, for inner points, the 10 parity signs) runs on up to 16 threads over i2
+// planes into per-point records; the lists are then assembled serially in the reference's
+// storage and face order, so the tables are the same bytes a serial pass produces.
+int build_bc_tables(const paramstruct *p, REAL *xx[3], BcTables &T, std::string *err) {
+  const int G0 = p->Nxx_plus_2NGHOSTS0, G1 = p->Nxx_plus_2NGHOSTS1, G2 = p->Nxx_plus_2NGHOSTS2;
+  const long plane = (long)G0 * G1;
+  // per point: source index (-1: interior), flags bit0 round-trip failure, bit1 parity failure
+  std::vector<int> src((size_t)plane * G2, -1);
+  std::vector<unsigned char> flag((size_t)plane * G2, 0);
+  std::vector<int8_t> par((size_t)plane * G2 * 10, 0);
+  auto work = [&](int i2) {
     for (int i1 = 0; i1 < G1; i1++)
       for (int i0 = 0; i0 < G0; i0++) {
         if (interior(p, i0, i1, i2)) continue;
+        const long k = i0 + (long)G0 * i1 + plane * i2;
         int t[3];
         if (!round_trip(p, xx, i0, i1, i2, t)) {
-          if (err) *err = "Error in SinhSymTP coordinate system: Cartesian disagreement at (" + std::to_string(i0) +
-                          "," + std::to_string(i1) + "," + std::to_string(i2) + ")";
-          return NRPYELL_EGHOSTMAP;
+          flag[k] |= 1;
+          continue;
         }
-        if (t[0] == i0 && t[1] == i1 && t[2] == i2) continue;
-        innerpt_bc_struct e;
-        memset(&e, 0, sizeof(e));
-        e.dstpt = i0 + G0 * (i1 + G1 * i2);
-        e.srcpt = t[0] + G0 * (t[1] + G1 * t[2]);
+        src[k] = t[0] + G0 * (t[1] + G1 * t[2]);
+        if (src[k] == (int)k) continue;
         const double a[3] = {xx[0][i0], xx[1][i1], xx[2][i2]}, c[3] = {xx[0][t[0]], xx[1][t[1]], xx[2][t[2]]};
-        if (!parity_signs(p, a, c, e.parity)) {
-          if (err) *err = "Parity evaluated to a value not within 8 significant digits of +1 or -1";
-          return NRPYELL_EGHOSTMAP;
-        }
-        inner.push_back(e);
+        if (!parity_signs(p, a, c, &par[10 * k])) flag[k] |= 2;
       }
-  bc->bc_info.num_inner_boundary_points = (int)inner.size();
-  bc->inner_bc_array = (innerpt_bc_struct *)malloc(sizeof(innerpt_bc_struct) * (inner.empty() ? 1 : inner.size()));
-  if (!inner.empty()) memcpy(bc->inner_bc_array, inner.data(), sizeof(innerpt_bc_struct) * inner.size());
+  };
+  const int nth = std::max(1, std::min({16, G2, (int)std::thread::hardware_concurrency()}));
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nth; t++)
+    pool.emplace_back([&, t] {
+      for (int i2 = t; i2 < G2; i2 += nth) work(i2);
+    });
+  for (auto &th : pool) th.join();
+
+  // Step 1 (bcstruct_set_up.c:355-406): inner points in storage order; the first failure in
+  // that order is the one reported.
+  T.inner.clear();
+  memset(&T.info, 0, sizeof(T.info));
+  for (long k = 0; k < plane * G2; k++) {
+    if (flag[k] & 1) {
+      const int i0 = (int)(k % G0), i1 = (int)((k / G0) % G1), i2 = (int)(k / plane);
+      if (err) *err = "Error in SinhSymTP coordinate system: Cartesian disagreement at (" + std::to_string(i0) + "," +
+                      std::to_string(i1) + "," + std::to_string(i2) + ")";
+      return NRPYELL_EGHOSTMAP;
+    }
+    if (src[k] < 0 || src[k] == (int)k) continue;
+    if (flag[k] & 2) {
+      if (err) *err = "Parity evaluated to a value not within 8 significant digits of +1 or -1";
+      return NRPYELL_EGHOSTMAP;
+    }
+    innerpt_bc_struct e;
+    memset(&e, 0, sizeof(e));
+    e.dstpt = (int)k;
+    e.srcpt = src[k];
+    memcpy(e.parity, &par[10 * k], 10);
+    T.inner.push_back(e);
+  }
+  T.info.num_inner_boundary_points = (int)T.inner.size();
 
   // Step 2 (bcstruct_set_up.c:411-520): face loop bounds growing one layer at a time; outer
-  // lists stored (and allocated) per access slot dirn + 3*which_gz.
+  // lists stored per access slot dirn + 3*which_gz.
   int lo[3] = {NG, NG, NG}, hi[3] = {G0 - NG, G1 - NG, G2 - NG};
   for (int gz = 0; gz < NG; gz++)
     for (int d = 0; d < 3; d++) {
-      int *rl = bc->bc_info.bc_loop_bounds[gz][2 * d], *rh = bc->bc_info.bc_loop_bounds[gz][2 * d + 1];
+      int *rl = T.info.bc_loop_bounds[gz][2 * d], *rh = T.info.bc_loop_bounds[gz][2 * d + 1];
       for (int e = 0; e < 3; e++) { rl[2 * e] = lo[e]; rl[2 * e + 1] = hi[e]; }
       rl[2 * d] = lo[d] - 1; rl[2 * d + 1] = lo[d];
       lo[d]--;
@@ -194,19 +235,20 @@ int host_bcstruct(const paramstruct *p, REAL *xx[3], bc_struct *bc, std::string 
     }
   for (int gz = 0; gz < NG; gz++)
     for (int dirn = 0; dirn < 3; dirn++) {
-      std::vector<outerpt_bc_struct> list;
+      std::vector<outerpt_bc_struct> &list = T.outer[dirn + 3 * gz];
+      list.clear();
       for (int side = 0; side < 2; side++) {
         const int face = 2 * dirn + side;
-        const int *r = bc->bc_info.bc_loop_bounds[gz][face];
+        const int *r = T.info.bc_loop_bounds[gz][face];
         for (int i2 = r[4]; i2 < r[5]; i2++)
           for (int i1 = r[2]; i1 < r[3]; i1++)
             for (int i0 = r[0]; i0 < r[1]; i0++) {
-              int t[3];
-              if (!round_trip(p, xx, i0, i1, i2, t)) {
+              const long k = i0 + (long)G0 * i1 + plane * i2;
+              if (flag[k] & 1) {
                 if (err) *err = "Error in SinhSymTP coordinate system: Cartesian disagreement";
                 return NRPYELL_EGHOSTMAP;
               }
-              if (!(t[0] == i0 && t[1] == i1 && t[2] == i2)) continue;
+              if (src[k] != (int)k) continue;
               outerpt_bc_struct o;
               o.i0 = (short)i0; o.i1 = (short)i1; o.i2 = (short)i2;
               o.FACEX0 = (int8_t)((face == 0) - (face == 1));
@@ -215,11 +257,74 @@ int host_bcstruct(const paramstruct *p, REAL *xx[3], bc_struct *bc, std::string 
               list.push_back(o);
             }
       }
-      const int slot = dirn + 3 * gz;
-      bc->bc_info.num_pure_outer_boundary_points[gz][dirn] = (int)list.size();
-      bc->pure_outer_bc_array[slot] = (outerpt_bc_struct *)malloc(sizeof(outerpt_bc_struct) * (list.size() + 1));
-      if (!list.empty()) memcpy(bc->pure_outer_bc_array[slot], list.data(), sizeof(outerpt_bc_struct) * list.size());
+      T.info.num_pure_outer_boundary_points[gz][dirn] = (int)list.size();
     }
+  return NRPYELL_OK;
+}
+
+// Tables of the last few grids. bcstruct_set_up reads only the grid sizes, AMAX, SINHWAA,
+// bScale, dxx and the xx arrays, so a configuration on a grid seen before (the batched solver,
+// learned-guess data generation) gets a copy of the same tables without the per-point work.
+struct BcCacheEntry {
+  int G[3];
+  double AMAX, SINHWAA, bScale, dxx[3];
+  std::vector<double> xx[3];
+  std::shared_ptr<const BcTables> tables;
+};
+std::mutex g_bc_mu;
+std::list<BcCacheEntry> g_bc_cache_host;
+constexpr size_t BC_CACHE_MAX = 16;
+
+bool same_grid(const BcCacheEntry &e, const paramstruct *p, REAL *const xx[3]) {
+  const int G[3] = {p->Nxx_plus_2NGHOSTS0, p->Nxx_plus_2NGHOSTS1, p->Nxx_plus_2NGHOSTS2};
+  const double dxx[3] = {p->dxx0, p->dxx1, p->dxx2};
+  for (int d = 0; d < 3; d++)
+    if (e.G[d] != G[d] || e.dxx[d] != dxx[d]) return false;
+  if (e.AMAX != p->AMAX || e.SINHWAA != p->SINHWAA || e.bScale != p->bScale) return false;
+  for (int d = 0; d < 3; d++)
+    if (memcmp(e.xx[d].data(), xx[d], sizeof(double) * G[d]) != 0) return false;
+  return true;
+}
+
+}  // namespace
+
+int host_bcstruct(const paramstruct *p, REAL *xx[3], bc_struct *bc, std::string *err) {
+  memset(bc, 0, sizeof(*bc));
+  std::shared_ptr<const BcTables> T;
+  {
+    std::lock_guard<std::mutex> lk(g_bc_mu);
+    for (auto it = g_bc_cache_host.begin(); it != g_bc_cache_host.end(); ++it)
+      if (same_grid(*it, p, xx)) {
+        T = it->tables;
+        g_bc_cache_host.splice(g_bc_cache_host.begin(), g_bc_cache_host, it);  // most recent first
+        break;
+      }
+  }
+  if (!T) {
+    auto built = std::make_shared<BcTables>();
+    const int rc = build_bc_tables(p, xx, *built, err);
+    if (rc != NRPYELL_OK) return rc;
+    T = built;
+    BcCacheEntry e;
+    e.G[0] = p->Nxx_plus_2NGHOSTS0; e.G[1] = p->Nxx_plus_2NGHOSTS1; e.G[2] = p->Nxx_plus_2NGHOSTS2;
+    e.AMAX = p->AMAX; e.SINHWAA = p->SINHWAA; e.bScale = p->bScale;
+    e.dxx[0] = p->dxx0; e.dxx[1] = p->dxx1; e.dxx[2] = p->dxx2;
+    for (int d = 0; d < 3; d++) e.xx[d].assign(xx[d], xx[d] + e.G[d]);
+    e.tables = T;
+    std::lock_guard<std::mutex> lk(g_bc_mu);
+    g_bc_cache_host.push_front(std::move(e));
+    if (g_bc_cache_host.size() > BC_CACHE_MAX) g_bc_cache_host.pop_back();
+  }
+  // the caller owns (and frees) malloc'ed copies, as with the reference's tables
+  const size_t ni = T->inner.size();
+  bc->bc_info = T->info;
+  bc->inner_bc_array = (innerpt_bc_struct *)malloc(sizeof(innerpt_bc_struct) * (ni ? ni : 1));
+  if (ni) memcpy(bc->inner_bc_array, T->inner.data(), sizeof(innerpt_bc_struct) * ni);
+  for (int slot = 0; slot < NG * 3; slot++) {
+    const size_t no = T->outer[slot].size();
+    bc->pure_outer_bc_array[slot] = (outerpt_bc_struct *)malloc(sizeof(outerpt_bc_struct) * (no + 1));
+    if (no) memcpy(bc->pure_outer_bc_array[slot], T->outer[slot].data(), sizeof(outerpt_bc_struct) * no);
+  }
   return NRPYELL_OK;
 }
 

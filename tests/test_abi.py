@@ -119,6 +119,35 @@ def test_bcstruct_loop_bounds_match_reference(amax):
     assert bytes(gd.bcstruct.bc_info) == bytes(R.gd.bcstruct.bc_info)
 
 
+def _inner_bytes(gd):
+    bc = gd.bcstruct
+    n = bc.bc_info.num_inner_boundary_points
+    return C.string_at(C.cast(bc.inner_bc_array, C.c_void_p), n * C.sizeof(abi.innerpt_bc_struct))
+
+
+def test_bcstruct_cache_hands_out_equal_independent_copies():
+    """A grid seen before gets the same table bytes from the per-grid cache, in fresh
+    allocations the caller owns; another AMAX (which changes the parities) is built anew."""
+    cfg = config.shipped_free_parameters((24, 16, 8))
+    a, _ = _host_tables(cfg)
+    b, _ = _host_tables(cfg)
+    assert _inner_bytes(a) == _inner_bytes(b) and bytes(a.bcstruct.bc_info) == bytes(b.bcstruct.bc_info)
+    assert C.addressof(a.bcstruct.inner_bc_array.contents) != C.addressof(b.bcstruct.inner_bc_array.contents)
+    b.bcstruct.inner_bc_array[3].srcpt += 1  # a caller's change stays in its own copy
+    c, _ = _host_tables(cfg)
+    assert _inner_bytes(c) == _inner_bytes(a)
+    cfg2 = config.shipped_free_parameters((24, 16, 8))
+    cfg2.AMAX = 1.0e4
+    d, _ = _host_tables(cfg2)
+    O = oracle.OracleProblem(cfg2)
+    dst, src, par = O.inner_arrays()
+    raw = np.frombuffer(_inner_bytes(d), dtype=np.dtype([("dst", "<i4"), ("src", "<i4"), ("par", "i1", 10),
+                                                           ("pad", "V2")]))
+    assert np.array_equal(raw["dst"], dst) and np.array_equal(raw["src"], src) and np.array_equal(raw["par"], par)
+    for gd in (a, b, c, d):
+        runtime.lib().nrpyell_bcstruct_free(C.byref(gd.bcstruct))
+
+
 def test_ghostmap_check_rejects_a_corrupted_table():
     """A planted mistake in the table (one source index off by one) must be refused, so the
     fused kernels never run on a grid whose ghost map they do not reproduce."""
