@@ -58,11 +58,14 @@ __device__ inline bool axis_weights(int n, double origin, double delta, double x
   const double t = fp - (double)c;
 #pragma unroll
   for (int m = MMIN; m <= MMAX; m++) {
-    double v = 1.0;
+    double v = 1.0, den = 1.0;  // den: an integer, folded at compile time
 #pragma unroll
     for (int k = MMIN; k <= MMAX; k++)
-      if (k != m) v *= (t - (double)k) / (double)(m - k);
-    w[m - MMIN] = v;
+      if (k != m) {
+        v *= t - (double)k;
+        den *= (double)(m - k);
+      }
+    w[m - MMIN] = v * (1.0 / den);
   }
   first = c + MMIN;
   return true;

@@ -656,7 +656,8 @@ void orc_get_xx_from_xyz(double AMAX, double SINHWAA, double bScale, const doubl
  * units; a molecule of order+1 points spans m = m_min..m_min+order, m_min = -floor(order/2),
  * around the nearest point (odd point count) or the point below x (even count); it is moved
  * inward where it would leave the array; points more than 1e-10 grid spacings outside the
- * array are an error. Weights are the Lagrange basis polynomials at fp - centre. */
+ * array are an error. Weights are the Lagrange basis polynomials at fp - centre, each the product
+ * of (t - k) times the reciprocal of its (integer) denominator. */
 static int lagrange_axis(int n, double origin, double delta, int order, double x, int *first, double w[8]) {
   const double fp = (x - origin) / delta;
   if (!(fp >= -1e-10 && fp <= (double)(n - 1) + 1e-10)) return -1;
@@ -666,10 +667,13 @@ static int lagrange_axis(int n, double origin, double delta, int order, double x
   if (c + mmax > n - 1) c = n - 1 - mmax;
   const double t = fp - (double)c;
   for (int m = mmin; m <= mmax; m++) {
-    double v = 1.0;
+    double v = 1.0, den = 1.0;
     for (int k = mmin; k <= mmax; k++)
-      if (k != m) v *= (t - (double)k) / (double)(m - k);
-    w[m - mmin] = v;
+      if (k != m) {
+        v *= t - (double)k;
+        den *= (double)(m - k);
+      }
+    w[m - mmin] = v * (1.0 / den);
   }
   *first = c + mmin;
   return 0;
