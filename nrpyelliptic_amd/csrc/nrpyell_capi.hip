@@ -580,6 +580,9 @@ struct nrpyell_solver {
     a.nbc_rows = (g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS;
     a.ntile0 = ntile0;
     a.ntile1 = ntile1;
+    a.ng_mul = div_mul((unsigned)(g.N2 / pl));
+    a.nt0_mul = div_mul((unsigned)ntile0);
+    a.nbc_mul = div_mul((unsigned)a.nbc_rows);
     a.w2d = nullptr;
     a.l2_partial = nullptr;
     a.l2_count = nullptr;
@@ -687,6 +690,15 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   const int ty = NRPYELL_BLOCK / tx;
   s->ntile0 = (g.N0 + tx - 1) / tx;
   s->ntile1 = (g.N1 + ty - 1) / ty;
+  {
+    // the kernels split block indices with div_by: exact while index x divisor <= 2^32
+    const double lim = 4294967296.0, nbc = (double)((g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS);
+    const double nsub = (double)s->ntile0 * s->ntile1 * (g.N2 / pl);
+    if (nsub * (g.N2 / pl) > lim || (double)s->ntile0 * s->ntile1 * s->ntile0 > lim || nbc * nbc * g.N2 > lim) {
+      set_last_error("grid too large for the fused solver's block indexing");
+      return NRPYELL_EINVAL;
+    }
+  }
   HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   const long n2 = (long)g.G0 * g.G1;
   s->coef64 = dalloc<double>(5 * n2);

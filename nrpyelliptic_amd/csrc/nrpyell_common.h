@@ -26,6 +26,23 @@ __host__ __device__ inline int wrap2(const GridDims &g, int j) {
   return m + NRPYELL_NG;
 }
 
+// wrap2 for indices within one period of the interior planes, j - NG in [-N2, 2 N2): one
+// conditional add and one conditional subtract instead of an integer modulo (whose generic
+// sequence the compiler emits for a runtime N2).
+__host__ __device__ inline int wrap2_near(const GridDims &g, int j) {
+  int m = j - NRPYELL_NG;
+  m += m < 0 ? g.N2 : 0;
+  m -= m >= g.N2 ? g.N2 : 0;
+  return m + NRPYELL_NG;
+}
+
+// floor(q / d) as a multiply-high with mul = ceil(2^32 / d) (div_mul): exact whenever
+// q * d <= 2^32, which the host checks for every block index it launches.
+__host__ __device__ inline unsigned div_by(unsigned q, unsigned long long mul) {
+  return (unsigned)(((unsigned long long)q * mul) >> 32);
+}
+inline unsigned long long div_mul(unsigned d) { return ((1ull << 32) + d - 1) / d; }
+
 // Ghost -> source map of the SinhSymTP grid, the closed form of the eigen-coordinate round
 // trip of bcstruct_set_up.c:27-194:
 //   xx0 < 0 : (xx0,xx1,xx2) ~ (-xx0, xx1, xx2+pi)   i0 -> 2*NG-1-i0, half turn in phi
@@ -94,7 +111,7 @@ __host__ __device__ inline void for_each_image_in_planes(const GridDims &g, int 
   if (i1 >= NRPYELL_NG && i1 < 2 * NRPYELL_NG) b_c[nb++] = 2 * NRPYELL_NG - 1 - i1;
   if (i1 >= g.N1 && i1 < g.N1 + NRPYELL_NG && i1 >= NRPYELL_NG) b_c[nb++] = 2 * (g.N1 + NRPYELL_NG) - 1 - i1;
   if (!ra && nb == 0) return;
-  const int j2h = wrap2(g, i2 + g.N2 / 2);  // one reflection: half turn
+  const int j2h = wrap2_near(g, i2 + g.N2 / 2);  // one reflection: half turn (i2 interior)
   const int a = 2 * NRPYELL_NG - 1 - i0;
   if (ra) f(idx3(g, a, i1, j2h));
   for (int ib = 0; ib < nb; ib++) {

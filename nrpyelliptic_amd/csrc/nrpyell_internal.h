@@ -119,6 +119,8 @@ struct SubstepArgs {
   int pl;           // i2 planes per thread (1 or 2)
   int nbc_rows;     // BC blocks per i2 plane: ceil(N1 / NRPYELL_BC_ROWS)
   int ntile0, ntile1;
+  // div_mul of N2/pl, ntile0 and nbc_rows: the kernels split block indices by them
+  unsigned long long ng_mul, nt0_mul, nbc_mul;
   // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
   // is c^2 times the residual. Non-null l2_partial enables it.
   const double *w2d;          // unused by the substep kernel (the weight is in coef)

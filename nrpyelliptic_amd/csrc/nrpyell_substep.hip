@@ -110,7 +110,7 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
-  const int t0 = tile % a.ntile0, t1 = tile / a.ntile0;
+  const int t1 = (int)div_by((unsigned)tile, a.nt0_mul), t0 = tile - t1 * a.ntile0;
   const int i2a = NG + PL * pgroup;
   const int i0s = NG + t0 * TX, i1s = NG + t1 * TY;
   const int i0 = i0s + tx, i1 = i1s + ty;
@@ -141,8 +141,15 @@ __device__ __forceinline__ void interior_body(const SubstepArgs<T> &a, const int
       stage_u[q][r] = ld32(a.in, (unsigned)(i2a + q) * s2 + (unsigned)j1 * s1 + j0);
     }
     if (q == 0) {
+      // the window spans plane indices i2a - NG .. i2a + PL - 1 + NG: within one period of the
+      // interior planes once N2 >= 6
+      if (g.N2 >= 6) {
 #pragma unroll
-      for (int jj = 0; jj < NW; jj++) zw[jj] = ld32(a.in, (unsigned)wrap2(g, i2a - NG + jj) * s2 + k2);
+        for (int jj = 0; jj < NW; jj++) zw[jj] = ld32(a.in, (unsigned)wrap2_near(g, i2a - NG + jj) * s2 + k2);
+      } else {
+#pragma unroll
+        for (int jj = 0; jj < NW; jj++) zw[jj] = ld32(a.in, (unsigned)wrap2(g, i2a - NG + jj) * s2 + k2);
+      }
       T kw;
       ld2x32(a.coef, 6 * k2, K[0], K[1]);
       ld2x32(a.coef, 6 * k2 + 2, K[2], K[3]);
@@ -281,8 +288,14 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   const unsigned k2 = (unsigned)min(i1s + rrow, g.N1 + NG - 1) * s1 + ilast;
   T zw[2 * NG + 1], K[5], vv = 0, psi = 1, AAc = 0;
   if (tid < TYB) {
+    if (g.N2 >= 6) {  // the window is within one period of the interior planes
 #pragma unroll
-    for (int j = 0; j <= 2 * NG; j++) zw[j] = j == NG ? (T)0 : ld32(a.in, (unsigned)wrap2(g, i2 - NG + j) * s2 + k2);
+      for (int j = 0; j <= 2 * NG; j++)
+        zw[j] = j == NG ? (T)0 : ld32(a.in, (unsigned)wrap2_near(g, i2 - NG + j) * s2 + k2);
+    } else {
+#pragma unroll
+      for (int j = 0; j <= 2 * NG; j++) zw[j] = j == NG ? (T)0 : ld32(a.in, (unsigned)wrap2(g, i2 - NG + j) * s2 + k2);
+    }
     T kw;
     ld2x32(a.coef, 6 * k2, K[0], K[1]);
     ld2x32(a.coef, 6 * k2 + 2, K[2], K[3]);
@@ -369,11 +382,13 @@ template <typename T, int STAGE, int TX, int PL>
 __global__ void __launch_bounds__(NRPYELL_BLOCK) substep_kernel(SubstepArgs<T> a) {
   const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
   if (L < nb) {
-    bc_body<T, STAGE>(a, L % a.nbc_rows, L / a.nbc_rows);
+    const unsigned pln = div_by(L, a.nbc_mul);
+    bc_body<T, STAGE>(a, (int)(L - pln * (unsigned)a.nbc_rows), (int)pln);
     return;
   }
   const unsigned ni = gridDim.x - nb, q = xcd_logical(L - nb, ni), ng = (unsigned)(a.g.N2 / PL);
-  interior_body<T, STAGE, TX, PL>(a, q / ng, q % ng, q, ni);
+  const unsigned tile = div_by(q, a.ng_mul);
+  interior_body<T, STAGE, TX, PL>(a, (int)tile, (int)(q - tile * ng), q, ni);
 }
 
 // Many configurations (uneven grids) in one launch: logical block q works for configuration
@@ -390,11 +405,13 @@ __global__ void __launch_bounds__(NRPYELL_BLOCK) substep_batched_kernel(const Su
   if (*a.done) return;  // leave before issuing any load
   const unsigned local = q - block_first[c], nb = (unsigned)(a.nbc_rows * a.g.N2);
   if (local < nb) {
-    bc_body<T, STAGE>(a, local % a.nbc_rows, local / a.nbc_rows);
+    const unsigned pln = div_by(local, a.nbc_mul);
+    bc_body<T, STAGE>(a, (int)(local - pln * (unsigned)a.nbc_rows), (int)pln);
     return;
   }
   const unsigned li = local - nb, ng = (unsigned)(a.g.N2 / PL);
-  interior_body<T, STAGE, TX, PL>(a, li / ng, li % ng, li, a.ntile0 * a.ntile1 * ng);
+  const unsigned tile = div_by(li, a.ng_mul);
+  interior_body<T, STAGE, TX, PL>(a, (int)tile, (int)(li - tile * ng), li, a.ntile0 * a.ntile1 * ng);
 }
 
 template <typename T, int STAGE>
