@@ -279,6 +279,33 @@ class Problem:
                               for k in range(m)])
         return rec, outer
 
+    # --- checkpoint / resume ------------------------------------------------------------------
+    _CKPT_PARAMS = ("AMAX", "SINHWAA", "bScale", "eta_damping", "bare_mass_0", "bare_mass_1",
+                    "puncture_0_x", "puncture_0_y", "puncture_0_z", "puncture_1_x", "puncture_1_y", "puncture_1_z",
+                    "P0_x", "P0_y", "P0_z", "P1_x", "P1_y", "P1_z", "S0_x", "S0_y", "S0_z", "S1_x", "S1_y", "S1_z",
+                    "dxx0", "dxx1", "dxx2")
+
+    def save_state(self, path: str, steps: int = 0) -> None:
+        """y_n (u and v, ghosts included), the steps taken and the grid and puncture parameters,
+        to an .npz file of plain arrays. A solve resumed from it continues bit for bit: an RK4
+        step reads nothing but y_n."""
+        p = self.gd.params
+        np.savez(path, y_n=self.get("y_n_gfs"), steps=np.int64(steps),
+                 grid=np.array([p.Nxx0, p.Nxx1, p.Nxx2], dtype=np.int64),
+                 params=np.array([getattr(p, k) for k in self._CKPT_PARAMS]), dt=np.float64(self.dt))
+
+    def load_state(self, path: str) -> int:
+        """Restore y_n from save_state's file (refused unless grid, parameters and dt are those of
+        this problem); returns the steps recorded with it."""
+        d = np.load(path)
+        p = self.gd.params
+        if (not np.array_equal(d["grid"], [p.Nxx0, p.Nxx1, p.Nxx2])
+                or not np.array_equal(d["params"], [getattr(p, k) for k in self._CKPT_PARAMS])
+                or float(d["dt"]) != self.dt):
+            raise NrpyellError(f"{path}: checkpoint of a different grid, puncture set or time step")
+        self.set("y_n_gfs", d["y_n"])
+        return int(d["steps"])
+
     # --- Einstein Toolkit consumer path ----------------------------------------------------
     def set_optimum_eta_damping(self) -> float:
         """The thorn's eta_damping = -1 policy (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:146-156):
@@ -372,6 +399,17 @@ class Solver:
         ms, l2 = C.c_double(), C.c_double()
         check(lib().nrpyell_solver_timed_steps(self.h, n, C.byref(ms), C.byref(l2)), "timed_steps")
         return ms.value, l2.value
+
+    def write_residual_log(self, path: str, max_entries: int = 4096) -> int:
+        """The residual history in main.c's residual_l2_norm.txt format (main.c:153-155):
+        "%6d %10.4e %.17e" = step, time = step * dt, log10 L2 of the residual; returns the
+        number of lines."""
+        steps, vals = self.history(max_entries)
+        dt = self.prob.dt
+        with open(path, "w") as f:
+            for n, l in zip(steps, vals):
+                f.write("%6d %10.4e %.17e\n" % (int(n), float(n) * dt, float(l)))
+        return len(steps)
 
     def history(self, max_entries: int = 4096):
         steps = (C.c_int * max_entries)()
