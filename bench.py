@@ -236,7 +236,7 @@ def batched_solve(n=64, seed=42):
     from nrpyelliptic_amd import config, runtime
     from nrpyelliptic_amd.autotune import TileTuner
     probs = [runtime.Problem(c) for c in config.random_batch(n, seed=seed)]
-    tuner = TileTuner(cache_path=None, launches=100)
+    tuner = TileTuner(cache_path=None)
     t0 = time.perf_counter()
     tiles = tuner.choose_all(probs)
     tune_s = time.perf_counter() - t0

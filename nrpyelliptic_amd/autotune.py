@@ -2,11 +2,13 @@
 
 The fused kernel puts i0 across the lanes of a 256-thread block (tile TX x 256/TX) and gives
 each thread one or two i2 planes; which shape keeps HBM busiest depends on N0 (ragged last
-tile, BC-tile share) and on the number of blocks per launch. This module times the fused
-substep (stage 1 with the residual check, the production launch) for each candidate shape on
-the configuration's own grid and caches the winner per (N0, N1, N2, precision) in a JSON
-file, so a batch of configurations is grouped by shape without re-timing shapes it has
-already seen.
+tile, BC-tile share) and on the number of blocks per launch. This module times whole
+graph-replayed RK4 steps (the four fused launches with the residual check, as the solver runs
+them) for each candidate shape on the configuration's own grid, restoring the configuration's
+state afterwards, and caches the winner per (N0, N1, N2, precision) in a JSON file, so a batch
+of configurations is grouped by shape without re-timing shapes it has already seen. (Timing
+one stage launched back to back misranks variants whose effect shows only when the next stage
+reads what this one wrote.)
 """
 from __future__ import annotations
 
@@ -14,7 +16,7 @@ import json
 import os
 from typing import Dict, Iterable, Optional, Sequence
 
-from .runtime import Problem, Solver
+from .runtime import NrpyellError, Problem, Solver
 
 CANDIDATES = ((16, 1), (32, 1), (64, 1), (32, 2), (64, 2))  # (tile_i0, planes)
 _DEFAULT_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tile_cache.json")
@@ -31,24 +33,35 @@ class TileTuner:
     `cache_path=None` keeps the table in memory only."""
 
     def __init__(self, cache_path: Optional[str] = _DEFAULT_CACHE, candidates: Sequence[int] = CANDIDATES,
-                 launches: int = 200):
+                 steps: int = 50):
         self.cache_path = cache_path
         self.candidates = tuple(candidates)
-        self.launches = launches
+        self.steps = steps
         self.table: Dict[str, dict] = {}
         if cache_path and os.path.exists(cache_path):
             with open(cache_path) as f:
                 self.table = json.load(f)
 
     def measure(self, prob: Problem, precision: int = 64) -> Dict[tuple, float]:
-        """Microseconds per fused substep launch for every candidate shape."""
+        """Microseconds per fused substep launch, over whole RK4 steps, for every candidate
+        shape the grid admits. The steps are timed in FP64 (the FP32 phase runs the same
+        kernel on half the bytes); the configuration's state is restored afterwards."""
         out = {}
-        for tx, pl in self.candidates:
-            s = Solver(prob, precision=precision, tile_i0=tx, planes=pl)
-            try:
-                out[(tx, pl)] = s.time_substep(0, self.launches)
-            finally:
-                s.close()
+        saved = prob.get("y_n_gfs")
+        try:
+            for tx, pl in self.candidates:
+                try:
+                    s = Solver(prob, tile_i0=tx, planes=pl, steps_per_graph=self.steps)
+                except NrpyellError:
+                    continue  # shape not available on this grid
+                try:
+                    s.timed_steps(self.steps)  # warm-up (graph instantiation, caches)
+                    ms, _ = s.timed_steps(self.steps)
+                    out[(tx, pl)] = 1e3 * ms / (4 * self.steps)
+                finally:
+                    s.close()
+        finally:
+            prob.set("y_n_gfs", saved)
         return out
 
     def choose(self, prob: Problem, precision: int = 64) -> tuple:
