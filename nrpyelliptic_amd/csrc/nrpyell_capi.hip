@@ -1042,11 +1042,14 @@ struct nrpyell_batch {
     SubstepArgs<double> *d_args[5] = {};
     int *d_block_cfg = nullptr, *d_block_first = nullptr;  // sized for every member
     int nblocks = 0;                                       // blocks of the members in the graph
+    hipStream_t side = nullptr;                            // the group's branch while capturing
+    hipEvent_t joined = nullptr;
   };
   std::vector<Group> groups;
   SolverDeviceState **d_states = nullptr;
   double *d_heads = nullptr, *h_heads = nullptr;
   hipGraphExec_t graph = nullptr;
+  hipEvent_t fork = nullptr;
   int in_graph = 0;  // configurations the current graph launches blocks for
   ~nrpyell_batch() {
     if (graph) hipGraphExecDestroy(graph);
@@ -1054,7 +1057,10 @@ struct nrpyell_batch {
       for (auto *p : g.d_args) hipFree(p);
       hipFree(g.d_block_cfg);
       hipFree(g.d_block_first);
+      if (g.side) hipStreamDestroy(g.side);
+      if (g.joined) hipEventDestroy(g.joined);
     }
+    if (fork) hipEventDestroy(fork);
     hipFree(d_states);
     hipFree(d_heads);
     if (h_heads) hipHostFree(h_heads);
@@ -1097,8 +1103,30 @@ struct nrpyell_batch {
     HIPCHK(hipStreamSynchronize(stream));
     if (graph) HIPCHK(hipGraphExecDestroy(graph));
     hipGraph_t gr;
+    int nlive = 0;
+    for (auto &g : groups) nlive += g.nblocks > 0;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    for (int k = 0; k < opt.steps_per_graph; k++) step(stream);
+    if (nlive <= 1) {
+      for (int k = 0; k < opt.steps_per_graph; k++) step(stream);
+    } else {
+      // the groups' configurations are independent: one branch per group (its steps in order,
+      // on a stream forked from the capture and joined at the end), so the GPU can overlap the
+      // groups' launches instead of running them one after another in every stage
+      if (!fork) HIPCHK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(fork, stream));
+      for (auto &g : groups) {
+        if (g.nblocks == 0) continue;
+        if (!g.side) HIPCHK(hipStreamCreateWithFlags(&g.side, hipStreamNonBlocking));
+        if (!g.joined) HIPCHK(hipEventCreateWithFlags(&g.joined, hipEventDisableTiming));
+        HIPCHK(hipStreamWaitEvent(g.side, fork, 0));
+        for (int k = 0; k < opt.steps_per_graph; k++)
+          for (int stage = 1; stage <= 4; stage++)
+            launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl,
+                                           stage, g.side);
+        HIPCHK(hipEventRecord(g.joined, g.side));
+        HIPCHK(hipStreamWaitEvent(stream, g.joined, 0));
+      }
+    }
     HIPCHK(hipStreamEndCapture(stream, &gr));
     HIPCHK(hipGraphInstantiate(&graph, gr, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(gr));
