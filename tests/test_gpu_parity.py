@@ -1,7 +1,7 @@
 """GPU parity of the drop-in entry points of libnrpyell_hip.so against the oracle
 (oracle/liboracle.so, itself pinned to the compiled reference in test_oracle.py).
 
-Tolerances (north star): a single RHS evaluation agrees to <= 1e-12 relative max-norm
+Tolerances: a single RHS evaluation agrees to <= 1e-12 relative max-norm
 (rhs_eval and residual_all_points are in fact bit-exact: same FMA tree, IEEE division);
 bcstruct tables bit-exact; converged u to <= 1e-10 max-abs.
 """
