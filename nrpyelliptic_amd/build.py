@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnrpyell_hip.so")
 SOURCES = ["nrpyell_host.cpp", "nrpyell_kernels.hip", "nrpyell_substep.hip", "nrpyell_capi.hip", "nrpyell_mlp.hip",
-           "nrpyell_batch.hip", "nrpyell_adm.hip"]
+           "nrpyell_adm.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

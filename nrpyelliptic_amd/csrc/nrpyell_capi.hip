@@ -252,8 +252,8 @@ struct DeviceBc {
   int nrad[NGHOSTS * 3] = {};
   bool rows = false;  // x0 face only, same rows in every layer: one launch for all layers
   ~DeviceBc() {
-    hipFree(dst); hipFree(src); hipFree(par0);
-    for (auto *p : rad) hipFree(p);
+    (void)hipFree(dst); (void)hipFree(src); (void)hipFree(par0);
+    for (auto *p : rad) (void)hipFree(p);
   }
 };
 
@@ -308,7 +308,7 @@ static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const
   if (d.xx_key0 == xx[0] && d.xx_key1 == xx[1]) return;
   const GridDims g = dims_from_params(params);
   for (int s = 0; s < NGHOSTS * 3; s++) {
-    hipFree(d.rad[s]);
+    (void)hipFree(d.rad[s]);
     d.rad[s] = nullptr;
     const int gz = s / 3, dirn = s % 3;
     const int n = bc->bc_info.num_pure_outer_boundary_points[gz][dirn];
@@ -403,7 +403,7 @@ struct L2Weights {
   double *partial = nullptr;
   double *result = nullptr;
   double volume = 0.0;
-  ~L2Weights() { hipFree(w2d); hipFree(partial); hipFree(result); }
+  ~L2Weights() { (void)hipFree(w2d); (void)hipFree(partial); (void)hipFree(result); }
 };
 static std::vector<std::unique_ptr<L2Weights>> g_l2_cache;
 
@@ -533,7 +533,6 @@ struct nrpyell_solver {
   double volume = 0.0;
   double *partial = nullptr;
   double *l2_out = nullptr;
-  unsigned int *l2_count = nullptr;
   SolverDeviceState *dstate = nullptr;
   SolverDeviceState *hstate = nullptr;  // pinned
   // FP32 shadow state
@@ -543,14 +542,14 @@ struct nrpyell_solver {
   double graph32_target = 0.0;
   double *cached_aux = nullptr;
   ~nrpyell_solver() {
-    if (graph64) hipGraphExecDestroy(graph64);
-    if (graph32) hipGraphExecDestroy(graph32);
-    hipFree(coef64); hipFree(kc64); hipFree(kc32); hipFree(src64);
-    hipFree(geo64); hipFree(geo32); hipFree(w2d); hipFree(partial);
-    hipFree(l2_out); hipFree(l2_count); hipFree(dstate);
-    if (hstate) hipHostFree(hstate);
-    hipFree(y32); hipFree(ko32); hipFree(ke32); hipFree(rt32); hipFree(src32);
-    if (stream) hipStreamDestroy(stream);
+    if (graph64) (void)hipGraphExecDestroy(graph64);
+    if (graph32) (void)hipGraphExecDestroy(graph32);
+    (void)hipFree(coef64); (void)hipFree(kc64); (void)hipFree(kc32); (void)hipFree(src64);
+    (void)hipFree(geo64); (void)hipFree(geo32); (void)hipFree(w2d); (void)hipFree(partial);
+    (void)hipFree(l2_out); (void)hipFree(dstate);
+    if (hstate) (void)hipHostFree(hstate);
+    (void)hipFree(y32); (void)hipFree(ko32); (void)hipFree(ke32); (void)hipFree(rt32); (void)hipFree(src32);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 
   template <typename T>
@@ -583,9 +582,7 @@ struct nrpyell_solver {
     a.ng_mul = div_mul((unsigned)(g.N2 / pl));
     a.nt0_mul = div_mul((unsigned)ntile0);
     a.nbc_mul = div_mul((unsigned)a.nbc_rows);
-    a.w2d = nullptr;
     a.l2_partial = nullptr;
-    a.l2_count = nullptr;
     a.state = nullptr;
     a.l2_volume = 0.0;
     a.l2_target = 0.0;
@@ -600,9 +597,7 @@ struct nrpyell_solver {
       SubstepArgs<double> a = args<double>(s, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs,
                                            src64, kc64, geo64);
       if (s <= 2 && fuse_check) {  // stage 1 writes the partials, stage 2 reduces them
-        a.w2d = w2d;
         a.l2_partial = partial;
-        a.l2_count = l2_count;
         a.state = dstate;
         a.l2_volume = volume;
         a.l2_target = opt.log10_target;
@@ -721,7 +716,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   int hbad = 0;
   HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  hipFree(bad);
+  (void)hipFree(bad);
   if (hbad) {
     set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
                    "wavespeed_gf_all_points sets it");
@@ -731,8 +726,6 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   const long nblk_sub = (long)s->ntile0 * s->ntile1 * (g.N2 / pl);
   s->partial = dalloc<double>(nblk_res > nblk_sub ? nblk_res : nblk_sub);
   s->l2_out = dalloc<double>(1);
-  s->l2_count = dalloc<unsigned int>(1);
-  HIPCHK(hipMemset(s->l2_count, 0, sizeof(unsigned int)));
   s->dstate = dalloc<SolverDeviceState>(1);
   HIPCHK(hipHostMalloc((void **)&s->hstate, sizeof(SolverDeviceState), hipHostMallocDefault));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -744,7 +737,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
 
 extern "C" int nrpyell_solver_destroy(nrpyell_solver *s) {
   if (!s) return NRPYELL_OK;
-  hipStreamSynchronize(s->stream);
+  (void)hipStreamSynchronize(s->stream);
   delete s;
   return NRPYELL_OK;
 }
@@ -815,7 +808,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
     launch_convert_d2f(2 * nt, f.y_n_gfs, s->y32, s->stream);
     launch_convert_d2f(2 * nt, s->src64, s->src32, s->stream);
     if (!s->graph32 || s->graph32_target != s->opt.fp32_switch_log10) {
-      if (s->graph32) hipGraphExecDestroy(s->graph32);
+      if (s->graph32) (void)hipGraphExecDestroy(s->graph32);
       s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
       s->graph32_target = s->opt.fp32_switch_log10;
     }
@@ -858,7 +851,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
   }
   if (s->hstate->done == 0) {
     if (!s->graph64 || s->graph64_steps != s->opt.steps_per_graph) {
-      if (s->graph64) hipGraphExecDestroy(s->graph64);
+      if (s->graph64) (void)hipGraphExecDestroy(s->graph64);
       s->graph64 = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
       s->graph64_steps = s->opt.steps_per_graph;
     }
@@ -918,8 +911,8 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
   read_state(s);
   *ms_out = ms;
   if (log10_out) *log10_out = s->hstate->last_log10;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   HIPCHK(hipGraphExecDestroy(g));
   if (tail) HIPCHK(hipGraphExecDestroy(tail));
   return NRPYELL_OK;
@@ -946,9 +939,7 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
       s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, s->src64,
                       s->kc64, s->geo64);
   if (fuse) {
-    a.w2d = s->w2d;
     a.l2_partial = s->partial;
-    a.l2_count = s->l2_count;
     a.state = s->dstate;
     a.l2_volume = s->volume;
     a.l2_target = -1e300;  // never "converges": every launch does the full work
@@ -972,9 +963,9 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   *us_per_launch = 1000.0 * ms / n;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  if (tmp) hipFree(tmp);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (tmp) (void)hipFree(tmp);
   return NRPYELL_OK;
 }
 
@@ -1052,20 +1043,20 @@ struct nrpyell_batch {
   hipEvent_t fork = nullptr;
   int in_graph = 0;  // configurations the current graph launches blocks for
   ~nrpyell_batch() {
-    if (graph) hipGraphExecDestroy(graph);
+    if (graph) (void)hipGraphExecDestroy(graph);
     for (auto &g : groups) {
-      for (auto *p : g.d_args) hipFree(p);
-      hipFree(g.d_block_cfg);
-      hipFree(g.d_block_first);
-      if (g.side) hipStreamDestroy(g.side);
-      if (g.joined) hipEventDestroy(g.joined);
+      for (auto *p : g.d_args) (void)hipFree(p);
+      (void)hipFree(g.d_block_cfg);
+      (void)hipFree(g.d_block_first);
+      if (g.side) (void)hipStreamDestroy(g.side);
+      if (g.joined) (void)hipEventDestroy(g.joined);
     }
-    if (fork) hipEventDestroy(fork);
-    hipFree(d_states);
-    hipFree(d_heads);
-    if (h_heads) hipHostFree(h_heads);
+    if (fork) (void)hipEventDestroy(fork);
+    (void)hipFree(d_states);
+    (void)hipFree(d_heads);
+    if (h_heads) (void)hipHostFree(h_heads);
     for (auto *s : solvers) nrpyell_solver_destroy(s);
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
   }
   void step(hipStream_t st) {
     for (int stage = 1; stage <= 4; stage++)
@@ -1171,9 +1162,7 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
         SubstepArgs<double> a = s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs,
                                                 f.y_nplus1_running_total_gfs, s->src64, s->kc64, s->geo64);
         if (stage <= 2) {
-          a.w2d = s->w2d;
           a.l2_partial = s->partial;
-          a.l2_count = s->l2_count;
           a.state = s->dstate;
           a.l2_volume = s->volume;
           a.l2_target = b->opt.log10_target;
@@ -1236,7 +1225,7 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
 
 extern "C" int nrpyell_batch_destroy(nrpyell_batch *b) {
   if (!b) return NRPYELL_OK;
-  hipStreamSynchronize(b->stream);
+  (void)hipStreamSynchronize(b->stream);
   delete b;
   return NRPYELL_OK;
 }

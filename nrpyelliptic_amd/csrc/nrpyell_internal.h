@@ -122,11 +122,11 @@ struct SubstepArgs {
   // div_mul of N2/pl, ntile0 and nbc_rows: the kernels split block indices by them
   unsigned long long ng_mul, nt0_mul, nbc_mul;
   // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
-  // is c^2 times the residual. Non-null l2_partial enables it.
-  const double *w2d;          // unused by the substep kernel (the weight is in coef)
+  // is c^2 times the residual (weighted by the L2 weight / c^4 of the coefficient record).
+  // Non-null l2_partial enables it; block 0 of the stage-2 launch that follows sums the
+  // partials and updates the convergence state.
   double *l2_partial;         // one partial sum per block
-  unsigned int *l2_count;     // blocks finished; the last block reduces and resets it
-  SolverDeviceState *state;   // convergence state updated by the last block
+  SolverDeviceState *state;   // convergence state
   double l2_volume, l2_target;
 };
 

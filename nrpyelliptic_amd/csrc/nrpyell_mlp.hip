@@ -342,18 +342,18 @@ struct nrpyell_mlp {
   double *loss = nullptr;
   hipStream_t st = nullptr;
   ~nrpyell_mlp() {
-    hipFree(p); hipFree(g); hipFree(m); hipFree(v); hipFree(pw);
-    for (auto *h : Hs) hipFree(h);
-    for (auto *z : Zs) hipFree(z);
-    hipFree(dZa); hipFree(dZb); hipFree(y); hipFree(tb); hipFree(loss); hipFree(dy);
-    if (st) hipStreamDestroy(st);
+    (void)hipFree(p); (void)hipFree(g); (void)hipFree(m); (void)hipFree(v); (void)hipFree(pw);
+    for (auto *h : Hs) (void)hipFree(h);
+    for (auto *z : Zs) (void)hipFree(z);
+    (void)hipFree(dZa); (void)hipFree(dZb); (void)hipFree(y); (void)hipFree(tb); (void)hipFree(loss); (void)hipFree(dy);
+    if (st) (void)hipStreamDestroy(st);
   }
   int K(int l) const { return l == 1 ? MLP_K0 : MLP_W; }
   void ensure(int M) {
     if (M <= cap) return;
-    for (auto *h : Hs) hipFree(h);
-    for (auto *z : Zs) hipFree(z);
-    hipFree(dZa); hipFree(dZb); hipFree(y); hipFree(tb);
+    for (auto *h : Hs) (void)hipFree(h);
+    for (auto *z : Zs) (void)hipFree(z);
+    (void)hipFree(dZa); (void)hipFree(dZb); (void)hipFree(y); (void)hipFree(tb);
     Hs[0] = mlp_alloc<bf16_t>((size_t)M * MLP_K0);
     for (int l = 1; l <= MLP_HIDDEN; l++) {
       Hs[l] = mlp_alloc<bf16_t>((size_t)M * MLP_W);
@@ -363,7 +363,7 @@ struct nrpyell_mlp {
     dZb = mlp_alloc<bf16_t>((size_t)M * MLP_W);
     y = mlp_alloc<float>(M);
     tb = mlp_alloc<float>(M);
-    hipFree(dy);
+    (void)hipFree(dy);
     dy = mlp_alloc<float>(M);
     cap = M;
   }
@@ -468,7 +468,7 @@ extern "C" int nrpyell_mlp_create(unsigned long long seed, nrpyell_mlp **out) {
 
 extern "C" int nrpyell_mlp_destroy(nrpyell_mlp *s) {
   if (s) {
-    hipStreamSynchronize(s->st);
+    (void)hipStreamSynchronize(s->st);
     delete s;
   }
   return NRPYELL_OK;
@@ -556,7 +556,7 @@ extern "C" int nrpyell_mlp_train(nrpyell_mlp *s, const void *X, const float *t, 
   HIPCHK(hipGetLastError());
   if (losses) HIPCHK(hipMemcpyAsync(losses, dl, nsteps * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
-  hipFree(dl);
+  (void)hipFree(dl);
   return NRPYELL_OK;
 }
 
@@ -619,7 +619,7 @@ extern "C" int nrpyell_mlp_features(const griddata_struct *gd, void *X, double *
                      x2, (bf16_t *)X, scale, idx);
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
-  hipFree(x0); hipFree(x1); hipFree(x2);
+  (void)hipFree(x0); (void)hipFree(x1); (void)hipFree(x2);
   return NRPYELL_OK;
 }
 
@@ -671,6 +671,6 @@ extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd, int smooth_
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
-  hipFree(X); hipFree(scale); hipFree(idx); hipFree(y);
+  (void)hipFree(X); (void)hipFree(scale); (void)hipFree(idx); (void)hipFree(y);
   return NRPYELL_OK;
 }
