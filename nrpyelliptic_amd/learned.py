@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--batch", type=int, default=8192)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--lr-segments", type=int, default=1,
+                    help="split --steps into this many segments, halving the learning rate after each "
+                         "(Adam moments carry over)")
     ap.add_argument("--target", type=float, default=-10.0)
     ap.add_argument("--max-steps", type=int, default=10000)
     ap.add_argument("--out", default="mlp_initial_guess.npz")
@@ -133,8 +136,13 @@ def main():
         f"{stats['points'] / max(stats['seconds'], 1e-9):.3e} gp-substeps/s; tiles {tuner.table}")
     net = InitialGuessMLP(seed=7)
     t0 = time.time()
-    losses = net.train(X, T, N, batch=args.batch, steps=args.steps, lr=args.lr, seed=11)
-    log(f"train: {args.steps} steps in {time.time() - t0:.1f}s, loss {losses[:10].mean():.3e} -> "
+    per = max(1, args.steps // max(1, args.lr_segments))
+    parts = []
+    for k in range(max(1, args.lr_segments)):
+        parts.append(net.train(X, T, N, batch=args.batch, steps=per, lr=args.lr * 0.5 ** k, seed=11 + k))
+        log(f"train: segment {k + 1}, lr {args.lr * 0.5 ** k:.2e}, loss {parts[-1][-50:].mean():.3e}")
+    losses = np.concatenate(parts)
+    log(f"train: {len(losses)} steps in {time.time() - t0:.1f}s, loss {losses[:10].mean():.3e} -> "
         f"{losses[-50:].mean():.3e}")
     net.save(args.out)
     eval_cfgs = config.random_batch(args.eval_configs, seed=900, grids=False, Nxx=(128, 128, 16))
