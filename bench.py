@@ -32,10 +32,11 @@ BYTES_PER_GP_SUBSTEP = 92.0
 PUBLISHED_GP_SUBSTEPS = 4 * 262144 * 9001 / 146.68
 
 
-def cpu_baseline(steps: int, warmup: int) -> dict:
+def cpu_baseline(steps: int, warmup: int, reps: int = 5, admbase: bool = True) -> dict:
     """The reference driver's loop body (residual + L2 + MoL step, main.c:138-187) of the
     compiled reference C at 128x128x16, or the C restatement when the reference build is
-    absent. Runs in this (child) process; OMP threads come from the environment."""
+    absent, in `reps` repetitions of steps/reps steps. Runs in this (child) process; OMP
+    threads come from the environment."""
     import oracle
     from nrpyelliptic_amd import config
 
@@ -56,12 +57,21 @@ def cpu_baseline(steps: int, warmup: int) -> dict:
             O.mol_step(st)
     for _ in range(warmup):
         step()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step()
-    dt = time.perf_counter() - t0
-    return {"value": 4 * 262144 * steps / dt, "unit": "gp-substeps/s", "kind": kind, "seconds": dt,
-            "ms_per_step": 1e3 * dt / steps, "admbase": cpu_admbase(cfg)}
+    per = max(1, steps // max(1, reps))
+    times = []
+    for _ in range(max(1, reps)):
+        t0 = time.perf_counter()
+        for _ in range(per):
+            step()
+        times.append(time.perf_counter() - t0)
+    dt = sum(times)
+    ms = sorted(1e3 * t / per for t in times)
+    out = {"value": 4 * 262144 * per * len(times) / dt, "unit": "gp-substeps/s", "kind": kind, "seconds": dt,
+           "ms_per_step": 1e3 * dt / (per * len(times)), "ms_per_step_min": ms[0],
+           "ms_per_step_median": ms[len(ms) // 2], "repetitions": len(times), "steps_per_repetition": per}
+    if admbase:
+        out["admbase"] = cpu_admbase(cfg)
+    return out
 
 
 def cpu_admbase(cfg, n=1 << 20):
@@ -79,15 +89,25 @@ def cpu_admbase(cfg, n=1 << 20):
     return {"points_per_s": n / dt, "points": n, "seconds": dt, "cores": 1, "kind": "port"}
 
 
-def run_cpu_baseline_subprocess(steps=5000, warmup=5):
-    cores = os.cpu_count() or 1
-    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+def _cpu_child(args, threads, timeout=600):
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), *args], env=env, capture_output=True, text=True,
+                       timeout=timeout)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def run_cpu_baseline_subprocess(steps=5000, warmup=5, reps=5, single_thread_steps=100):
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     try:
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", str(steps), str(warmup)],
-                           env=env, capture_output=True, text=True, timeout=600)
-        out = json.loads(r.stdout.strip().splitlines()[-1])
+        out = _cpu_child(["--cpu-baseline-only", str(steps), str(warmup), str(reps)], cores)
     except Exception as e:  # the baseline is reported, never required
         return {"value": None, "unit": "gp-substeps/s", "cores": cores, "kind": None, "sample": f"failed: {e}"}
+    try:  # one thread, a short sample (the reference's 1-thread rate)
+        one = _cpu_child(["--cpu-baseline-only", str(single_thread_steps), "2", "1", "--no-admbase"], 1, timeout=300)
+        out["single_thread"] = {"value": one["value"], "ms_per_step": one["ms_per_step"], "steps": single_thread_steps,
+                                "cores": 1}
+    except Exception as e:
+        out["single_thread"] = {"value": None, "sample": f"failed: {e}"}
     cpu = "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -97,7 +117,8 @@ def run_cpu_baseline_subprocess(steps=5000, warmup=5):
                     break
     except OSError:
         pass
-    out.update(cores=cores, sample=f"{steps} RK4 steps (+{warmup} warmup) of the 128x128x16 solve from u=v=0, "
+    out.update(cores=cores, sample=f"{steps} RK4 steps in {reps} repetitions (+{warmup} warmup) of the 128x128x16 "
+                                   f"solve from u=v=0, "
                                    f"residual+L2 every step as main.c; OpenMP {cores} threads on '{cpu}'")
     return out
 
@@ -270,7 +291,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cpu-baseline-only", nargs=2, type=int, default=None)
+    ap.add_argument("--cpu-baseline-only", nargs="+", type=int, default=None,
+                    help="STEPS WARMUP [REPS]: the CPU leg alone (child process)")
+    ap.add_argument("--no-admbase", action="store_true", help="with --cpu-baseline-only: skip the consumer path")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--timed-only", action="store_true",
@@ -283,7 +306,8 @@ def main():
         return
 
     if args.cpu_baseline_only:
-        print(json.dumps(cpu_baseline(*args.cpu_baseline_only)))
+        a = args.cpu_baseline_only
+        print(json.dumps(cpu_baseline(a[0], a[1], a[2] if len(a) > 2 else 5, admbase=not args.no_admbase)))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs on one GPU only (the solve is one GPU's full, unsharded "
