@@ -27,6 +27,11 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 #define MLP_W 256
 #define MLP_HIDDEN 5  // hidden layers of width 256 (6 linear layers in total)
 
+// Parameter offsets (elements) of W_l and b_l, l = 1..5, and of the output layer (index 6).
+struct MlpOffsets {
+  long w[MLP_HIDDEN + 2], b[MLP_HIDDEN + 2];
+};
+
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
 __device__ __forceinline__ bf16_t to_bf16(float x) { return (bf16_t)x; }
 
@@ -97,6 +102,90 @@ __global__ void __launch_bounds__(256) mfma_gemm_kernel(const bf16_t *__restrict
       atomicAdd(&G[o], acc[r]);
     }
   }
+}
+
+// Inference, all six layers in one launch: a block takes 64 rows through the five hidden
+// layers with the activations held in LDS (two 64 x 256 BF16 buffers, ping-pong), so nothing
+// but the 64 input features and the output leave the chip; the pre-activations are not kept
+// (training uses the per-layer kernels, whose backward needs them). Each of the 4 waves owns 64
+// output columns: 2 x 2 MFMA accumulators over the 64 rows; the weights (544 KB of BF16, shared
+// by every block) are read from L2 as 16-byte MFMA operands. Same arithmetic as the per-layer
+// path: FP32 accumulation over BF16 operands in ascending K, z = acc + b, h = bf16(silu(z)),
+// output y = sum_k h_k w_k + b in FP32.
+constexpr int FI_R = 64, FI_LD = MLP_W + 8;  // rows per block; LDS row stride (528 B)
+
+__global__ void __launch_bounds__(256) mlp_forward_fused_kernel(const bf16_t *__restrict__ X, const bf16_t *__restrict__ pw,
+                                                                const float *__restrict__ p, MlpOffsets off, int M,
+                                                                float *__restrict__ y) {
+  __shared__ __attribute__((aligned(16))) bf16_t Hbuf[2][FI_R * FI_LD];
+  __shared__ float s_out[4][FI_R];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 31, lj = lane >> 5;
+  const int r0 = blockIdx.x * FI_R;
+  // input features: 64 rows x 64 BF16 = 512 chunks of 16 B, 2 per thread (rows past M clamped)
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int e = tid + 256 * q, row = e >> 3, ch = e & 7;
+    const int m = min(r0 + row, M - 1);
+    *(bf16x8 *)(&Hbuf[0][row * FI_LD + 8 * ch]) = *(const bf16x8 *)(X + (long)m * MLP_K0 + 8 * ch);
+  }
+  __syncthreads();
+  int cur = 0;
+  const int wc = wave * 64;
+  for (int l = 1; l <= MLP_HIDDEN; l++) {
+    const int Kl = l == 1 ? MLP_K0 : MLP_W;
+    const bf16_t *W = pw + off.w[l];  // W_l [256 x K_l]
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) acc[x][yy][r] = 0.0f;
+    const bf16_t *Hin = Hbuf[cur];
+    for (int k = 0; k < Kl; k += 16) {
+      bf16x8 a[2], b[2];
+#pragma unroll
+      for (int x = 0; x < 2; x++) a[x] = *(const bf16x8 *)(Hin + (32 * x + li) * FI_LD + k + 8 * lj);
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++) b[yy] = *(const bf16x8 *)(W + (long)(wc + 32 * yy + li) * Kl + k + 8 * lj);
+#pragma unroll
+      for (int x = 0; x < 2; x++)
+#pragma unroll
+        for (int yy = 0; yy < 2; yy++)
+          acc[x][yy] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[x], b[yy], acc[x][yy], 0, 0, 0);
+    }
+    bf16_t *Hout = Hbuf[cur ^ 1];
+    const float *bias = p + off.b[l];
+    // accumulator r of lane (li, lj) in tile (x, yy): row 8*(r/4) + 4*lj + r%4, column li
+#pragma unroll
+    for (int x = 0; x < 2; x++)
+#pragma unroll
+      for (int yy = 0; yy < 2; yy++) {
+        const int c = wc + 32 * yy + li;
+        const float bc = bias[c];
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const int row = 32 * x + 8 * (r >> 2) + 4 * lj + (r & 3);
+          const float z = acc[x][yy][r] + bc;
+          Hout[row * FI_LD + c] = to_bf16(z * sigmoidf_(z));
+        }
+      }
+    __syncthreads();
+    cur ^= 1;
+  }
+  // output layer: wave w sums columns 64w..64w+63 of every row, then the 4 partials are added
+  const bf16_t *Hl = Hbuf[cur];
+  const float *wout = p + off.w[MLP_HIDDEN + 1];
+  for (int row = lane; row < FI_R; row += 64) {
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int c = 0; c < 64; c++) acc += (float)Hl[row * FI_LD + wc + c] * wout[wc + c];
+    s_out[wave][row] = acc;
+  }
+  __syncthreads();
+  if (tid < FI_R && r0 + tid < M)
+    y[r0 + tid] = ((s_out[0][tid] + s_out[1][tid]) + (s_out[2][tid] + s_out[3][tid])) + p[off.b[MLP_HIDDEN + 1]];
 }
 
 // Output layer (256 -> 1): y[m] = sum_k H[m][k] w[k] + b (FP32 dot over BF16 activations).
@@ -500,10 +589,13 @@ static int check_rows(int M) {
 // Forward pass on M rows of device BF16 features [M x 32]; y (device, FP32) gets M outputs.
 extern "C" int nrpyell_mlp_forward(nrpyell_mlp *s, const void *x, int M, float *y_out) {
   if (check_rows(M)) return NRPYELL_EINVAL;
-  s->ensure(M);
-  HIPCHK(hipMemcpyAsync(s->Hs[0], x, (size_t)M * MLP_K0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, s->st));
-  s->forward(M);
-  HIPCHK(hipMemcpyAsync(y_out, s->y, (size_t)M * sizeof(float), hipMemcpyDeviceToDevice, s->st));
+  MlpOffsets off;
+  for (int l = 1; l <= MLP_HIDDEN + 1; l++) {
+    off.w[l] = s->w_off[l];
+    off.b[l] = s->b_off[l];
+  }
+  hipLaunchKernelGGL(mlp_forward_fused_kernel, dim3((M + FI_R - 1) / FI_R), dim3(256), 0, s->st, (const bf16_t *)x,
+                     s->pw, s->p, off, M, y_out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(s->st));
   return NRPYELL_OK;
