@@ -254,7 +254,32 @@ typedef struct {
                              or 64 (0 = default); the i1 height is 256 / tile_i0            */
   int planes;             /* i2 planes per thread: 1, or 2 with tile_i0 32/64 (0 = 2 where
                              the tile allows it, else 1)                                    */
+  int outer_bc;           /* NRPYELL_OUTER_BC_RADIATION: apply_bcs_outerradiation_and_inner;
+                             NRPYELL_OUTER_BC_SOMMERFELD6: the Einstein Toolkit thorn's BC
+                             (nrpyell_apply_bcs_sommerfeld with f_infinity 0, falloff 3)      */
+  double outer_bc_speed;  /* wave speed of the outer BC; 0 = the BC's default: the grid's
+                             uu/vv_wavespeed_at_OB (radiation), NRPYELL_ET_SOMMERFELD_SPEED
+                             (Sommerfeld)                                                   */
 } nrpyell_solve_options;
+
+#define NRPYELL_OUTER_BC_RADIATION 0
+#define NRPYELL_OUTER_BC_SOMMERFELD6 1
+/* The wave speed the thorn's Sommerfeld BC uses on every grid
+ * (conformally_flat_BBH_apply_bcs_sommerfeld.h:12): the outer-boundary wave speed of a
+ * 64x64x4 grid with AMAX 1e6, SINHWAA 0.07, bScale 5. */
+#define NRPYELL_ET_SOMMERFELD_SPEED 4.4793122676309114e+06
+
+/* The Einstein Toolkit thorn's outer BC on the RHS, then the inner parity copy (device arrays).
+ * Replaces conformally_flat_BBH_apply_bcs_sommerfeld (NRPyEllipticET/src/conformally_flat_BBH/
+ * conformally_flat_BBH_apply_bcs_sommerfeld.h:139-222), which reads the thorn's own
+ * bc_struct and fixes wavespeed = NRPYELL_ET_SOMMERFELD_SPEED, f_infinity = 0 and
+ * radial_falloff_power = 3 (:10-12); here the tables are this ABI's bc_struct (the same ghost
+ * maps) and the three are arguments. 7-point 6th-order derivatives; at each pure outer point
+ *   rhs = -c (d_r f + (f - f_inf)/r) + (rhs_q + c (d_r f_q + (f_q - f_inf)/r_q)) (r_q/r)^n,
+ * q the neighbour one point inward. NRPYELL_OK or NRPYELL_EINVAL. */
+int nrpyell_apply_bcs_sommerfeld(const paramstruct *params, const bc_struct *bcstruct, REAL *xx[3],
+                                 const REAL wavespeed[NUM_EVOL_GFS], const REAL f_infinity[NUM_EVOL_GFS],
+                                 const REAL radial_falloff_power[NUM_EVOL_GFS], REAL *gfs, REAL *rhs_gfs);
 
 typedef struct {
   int steps;              /* RK4 steps taken                                               */

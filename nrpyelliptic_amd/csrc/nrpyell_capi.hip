@@ -251,9 +251,12 @@ struct DeviceBc {
   RadPoint *rad[NGHOSTS * 3] = {};
   int nrad[NGHOSTS * 3] = {};
   bool rows = false;  // x0 face only, same rows in every layer: one launch for all layers
+  // the same points with the Einstein Toolkit thorn's 7-point stencil shifts
+  RadPoint *som[NGHOSTS * 3] = {};
   ~DeviceBc() {
     (void)hipFree(dst); (void)hipFree(src); (void)hipFree(par0);
     for (auto *p : rad) (void)hipFree(p);
+    for (auto *p : som) (void)hipFree(p);
   }
 };
 
@@ -304,32 +307,50 @@ static int stencil_offset(int idx, int face, int G) {
   return off;
 }
 
+// Stencil direction of the thorn's 7-point derivative (conformally_flat_BBH_apply_bcs_sommerfeld.h:
+// 49-105): one-sided along the face normal and wherever idx + NGHOSTS >= G (backward) or
+// idx - NGHOSTS <= 0 (forward); 0 = centred.
+static int sommerfeld_shift(int idx, int face, int G) {
+  if (abs(face) != 1 && idx + NGHOSTS < G && idx - NGHOSTS > 0) return 0;
+  int sh = face;
+  if (idx + NGHOSTS >= G) sh = -1;
+  if (idx - NGHOSTS <= 0) sh = +1;
+  return sh;
+}
+
 static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const bc_struct *bc, REAL *xx[3]) {
   if (d.xx_key0 == xx[0] && d.xx_key1 == xx[1]) return;
   const GridDims g = dims_from_params(params);
   for (int s = 0; s < NGHOSTS * 3; s++) {
     (void)hipFree(d.rad[s]);
-    d.rad[s] = nullptr;
+    (void)hipFree(d.som[s]);
+    d.rad[s] = d.som[s] = nullptr;
     const int gz = s / 3, dirn = s % 3;
     const int n = bc->bc_info.num_pure_outer_boundary_points[gz][dirn];
     d.nrad[s] = n;
     if (n == 0) continue;
-    std::vector<RadPoint> pts(n);
+    std::vector<RadPoint> pts(n), spts(n);
     for (int k = 0; k < n; k++) {
       const outerpt_bc_struct &o = bc->pure_outer_bc_array[s][k];
       const int f[3] = {o.FACEX0, o.FACEX1, o.FACEX2};
       const int P[2][3] = {{o.i0, o.i1, o.i2}, {o.i0 + f[0], o.i1 + f[1], o.i2 + f[2]}};
-      RadPoint &r = pts[k];
+      RadPoint &r = pts[k], &t = spts[k];
       for (int w = 0; w < 2; w++) {
         r.off[w][0] = stencil_offset(P[w][0], f[0], g.G0);
         r.off[w][1] = stencil_offset(P[w][1], f[1], g.G1);
         r.off[w][2] = 0;
         host_radial_geometry(params, xx[0][P[w][0]], xx[1][P[w][1]], r.geo[w]);
+        // the thorn evaluates the neighbour's derivative with the outer point's face
+        t.off[w][0] = sommerfeld_shift(P[w][0], f[0], g.G0);
+        t.off[w][1] = sommerfeld_shift(P[w][1], f[1], g.G1);
+        t.off[w][2] = 0;
+        for (int c = 0; c < 4; c++) t.geo[w][c] = r.geo[w][c];
       }
-      r.p = idx3(g, P[0][0], P[0][1], P[0][2]);
-      r.q = idx3(g, P[1][0], P[1][1], P[1][2]);
+      r.p = t.p = idx3(g, P[0][0], P[0][1], P[0][2]);
+      r.q = t.q = idx3(g, P[1][0], P[1][1], P[1][2]);
     }
     d.rad[s] = upload(pts.data(), n);
+    d.som[s] = upload(spts.data(), n);
   }
   // one-launch form: only x0-face points, and layer gz lists the rows of layer 0 in the same
   // order, each point the outer neighbour of the previous layer's point
@@ -338,7 +359,7 @@ static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const
     if (d.nrad[3 * gz] != d.nrad[0] || d.nrad[3 * gz + 1] != 0 || d.nrad[3 * gz + 2] != 0) d.rows = false;
     for (int k = 0; k < d.nrad[0] && d.rows && gz > 0; k++) {
       const outerpt_bc_struct &a = bc->pure_outer_bc_array[3 * gz][k], &b = bc->pure_outer_bc_array[3 * (gz - 1)][k];
-      if (a.FACEX0 != 1 || b.FACEX0 != 1 || a.i1 != b.i1 || a.i2 != b.i2 || a.i0 != b.i0 + 1) d.rows = false;
+      if (a.FACEX0 != -1 || b.FACEX0 != -1 || a.i1 != b.i1 || a.i2 != b.i2 || a.i0 != b.i0 + 1) d.rows = false;
     }
   }
   d.xx_key0 = xx[0];
@@ -394,6 +415,32 @@ extern "C" void apply_bcs_outerradiation_and_inner(const paramstruct *params, co
   launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
   HIPCHK(hipGetLastError());
   HIPCHK(hipDeviceSynchronize());
+}
+
+extern "C" int nrpyell_apply_bcs_sommerfeld(const paramstruct *params, const bc_struct *bcstruct, REAL *xx[3],
+                                            const REAL wavespeed[NUM_EVOL_GFS], const REAL f_infinity[NUM_EVOL_GFS],
+                                            const REAL radial_falloff_power[NUM_EVOL_GFS], REAL *gfs,
+                                            REAL *rhs_gfs) {
+  if (!params || !bcstruct || !xx || !wavespeed || !f_infinity || !radial_falloff_power || !gfs || !rhs_gfs) {
+    set_last_error("nrpyell_apply_bcs_sommerfeld: null argument");
+    return NRPYELL_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  const GridDims g = dims_from_params(params);
+  DeviceBc &d = device_bc(params, bcstruct);
+  ensure_radiation_lists(d, params, bcstruct, xx);
+  // innermost layer first: each layer reads the RHS its inner neighbour layer just received
+  SommerfeldParams sp;
+  for (int k = 0; k < 2; k++) {
+    sp.c[k] = wavespeed[k];
+    sp.finf[k] = f_infinity[k];
+    sp.falloff[k] = radial_falloff_power[k];
+  }
+  for (int s = 0; s < NGHOSTS * 3; s++) launch_sommerfeld_list(g, d.nrad[s], d.som[s], gfs, rhs_gfs, sp, default_stream());
+  launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipDeviceSynchronize());
+  return NRPYELL_OK;
 }
 
 // L2 weight tables cached per (grid, radius, coordinates).
@@ -569,8 +616,11 @@ struct nrpyell_solver {
     a.src = src;
     a.coef = coef;
     a.geo = geo;
-    a.cOB_u = (T)gd->params.uu_wavespeed_at_OB;
-    a.cOB_v = (T)gd->params.vv_wavespeed_at_OB;
+    a.bc_kind = opt.outer_bc;
+    const double c_default = opt.outer_bc == NRPYELL_OUTER_BC_SOMMERFELD6 ? NRPYELL_ET_SOMMERFELD_SPEED : 0.0;
+    const double c_fixed = opt.outer_bc_speed > 0 ? opt.outer_bc_speed : c_default;
+    a.cOB_u = (T)(c_fixed > 0 ? c_fixed : gd->params.uu_wavespeed_at_OB);
+    a.cOB_v = (T)(c_fixed > 0 ? c_fixed : gd->params.vv_wavespeed_at_OB);
     a.eta = (T)gd->params.eta_damping;
     a.dt = (T)dt;
     a.done = &dstate->done;
@@ -642,6 +692,8 @@ extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
   o->integration_radius = 100.0;  // main.c:150
   o->tile_i0 = 0;
   o->planes = 0;
+  o->outer_bc = NRPYELL_OUTER_BC_RADIATION;
+  o->outer_bc_speed = 0.0;
 }
 
 extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell_solve_options *opt,
@@ -669,6 +721,10 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
   if (ghostmap_verify(g, &gd->bcstruct, &err) != NRPYELL_OK) {
     set_last_error(err);
     return NRPYELL_EGHOSTMAP;
+  }
+  if (s->opt.outer_bc != NRPYELL_OUTER_BC_RADIATION && s->opt.outer_bc != NRPYELL_OUTER_BC_SOMMERFELD6) {
+    set_last_error("outer_bc must be NRPYELL_OUTER_BC_RADIATION or NRPYELL_OUTER_BC_SOMMERFELD6");
+    return NRPYELL_EINVAL;
   }
   int tx = s->opt.tile_i0 > 0 ? s->opt.tile_i0 : 32;
   if (tx != 16 && tx != 32 && tx != 64) {

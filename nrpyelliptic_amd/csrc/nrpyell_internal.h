@@ -97,6 +97,11 @@ struct RadPoint {
   int off[2][3];
   double geo[2][4];
 };
+// Wave speed, value at infinity and radial falloff power per evolved gridfunction of the
+// Einstein Toolkit thorn's Sommerfeld BC.
+struct SommerfeldParams {
+  double c[2], finf[2], falloff[2];
+};
 struct RadLayers {
   const RadPoint *l[NRPYELL_NG];  // the x0-face list of each ghost layer, same row order
 };
@@ -112,7 +117,8 @@ struct SubstepArgs {
   const T *src;   // per point (psi, 0.125 c^2 AA^2), c the wave speed
   const T *coef;  // per (i0,i1): c^2 x the 5 metric coefficients, then the L2 weight / c^4
   const T *geo;   // outer-face geometry: [pt 0..5][i1][1/r, dxx0/dr, dxx1/dr, (r_0/r_pt)^3], pt 0 = last interior i0
-  T cOB_u, cOB_v;
+  T cOB_u, cOB_v;   // wave speed of the outer BC
+  int bc_kind;      // NRPYELL_OUTER_BC_RADIATION or NRPYELL_OUTER_BC_SOMMERFELD6
   T eta, dt;
   const int *done;  // nonzero: converged, every launch returns at once
   int tx;           // tile width along i0 (threads), tile height along i1 = blockDim/tx
@@ -155,6 +161,8 @@ void launch_inner_gather(long nt, int n, const int *dst, const int *src, const i
                          hipStream_t st);
 void launch_radiation_rows(const GridDims &g, int n, const RadLayers &L, const double *gfs, double *rhs, double c_u,
                            double c_v, double finf_u, double finf_v, hipStream_t st);
+void launch_sommerfeld_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs,
+                            const SommerfeldParams &sp, hipStream_t st);
 void launch_radiation_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs, double c_u,
                            double c_v, double finf_u, double finf_v, hipStream_t st);
 void launch_rfm(const GridDims &g, double AMAX, double w, double b, const double *xx0_d, const double *xx1_d,

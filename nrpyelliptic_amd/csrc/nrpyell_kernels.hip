@@ -391,6 +391,64 @@ void launch_radiation_rows(const GridDims &g, int n, const RadLayers &L, const d
                      c_v, finf_u, finf_v);
 }
 
+// The Einstein Toolkit thorn's outer BC (conformally_flat_BBH_apply_bcs_sommerfeld.h:16-222):
+// 7-point 6th-order first derivatives, one-sided (sh = -1 backward, +1 forward, summed from the
+// point outward) or centred (sh = 0, summed from +3 down to -3), then
+//   rhs_p = -c (d_r f_p + (f_p - finf)/r_p) + (rhs_q + c (d_r f_q + (f_q - finf)/r_q)) (r_q/r_p)^n
+// with q the neighbour one point inward; each layer reads the RHS of the layer inside it, so
+// the layers are separate launches.
+__device__ inline double sommerfeld_fd(const double *f, long q, long st, int sh) {
+  if (sh == 0) {
+    double s = (1.0 / 60.0) * f[q + 3 * st];
+    s = fma(-3.0 / 20.0, f[q + 2 * st], s);
+    s = fma(3.0 / 4.0, f[q + st], s);
+    s = fma(-3.0 / 4.0, f[q - st], s);
+    s = fma(3.0 / 20.0, f[q - 2 * st], s);
+    return fma(-1.0 / 60.0, f[q - 3 * st], s);
+  }
+  const long d = sh * st;
+  double s = (-49.0 / 20.0) * f[q];
+  s = fma(6.0, f[q + d], s);
+  s = fma(-15.0 / 2.0, f[q + 2 * d], s);
+  s = fma(20.0 / 3.0, f[q + 3 * d], s);
+  s = fma(-15.0 / 4.0, f[q + 4 * d], s);
+  s = fma(6.0 / 5.0, f[q + 5 * d], s);
+  s = fma(-1.0 / 6.0, f[q + 6 * d], s);
+  return sh * s;
+}
+
+__device__ inline double sommerfeld_dr(const GridDims &g, const RadPoint &P, int w, const double *f) {
+  const long q = w == 0 ? P.p : P.q;
+  const double d0 = sommerfeld_fd(f, q, 1, P.off[w][0]) * g.invdx0;
+  const double d1 = sommerfeld_fd(f, q, g.G0, P.off[w][1]) * g.invdx1;
+  return P.geo[w][1] * d0 + P.geo[w][2] * d1;
+}
+
+__global__ void sommerfeld_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs,
+                                       SommerfeldParams sp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 2 * n) return;
+  const int gf = k / n;
+  const RadPoint &P = pts[k % n];
+  const double *f = gfs + gf * g.ntot;
+  double *rh = rhs + gf * g.ntot;
+  const double c = sp.c[gf], finf = sp.finf[gf], nf = sp.falloff[gf];
+  const double invr = 1.0 / P.geo[0][0];
+  double val = -c * (sommerfeld_dr(g, P, 0, f) + invr * (f[P.p] - finf));
+  if (nf > 0) {
+    const double invr_q = 1.0 / P.geo[1][0];
+    const double extrap = c * (sommerfeld_dr(g, P, 1, f) + invr_q * (f[P.q] - finf));
+    val += (rh[P.q] + extrap) * pow(P.geo[1][0] * invr, nf);
+  }
+  rh[P.p] = val;
+}
+
+void launch_sommerfeld_list(const GridDims &g, int n, const RadPoint *pts, const double *gfs, double *rhs,
+                            const SommerfeldParams &sp, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(sommerfeld_list_kernel, dim3((2 * n + 63) / 64), dim3(64), 0, st, g, n, pts, gfs, rhs, sp);
+}
+
 // ---------------------------------------------------------------------------------------
 // Setup kernels (device-side setup for rfm factors, auxevol and wavespeed).
 // ---------------------------------------------------------------------------------------

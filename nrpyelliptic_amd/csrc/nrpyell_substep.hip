@@ -340,10 +340,33 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
       const T *box = gf == 0 ? bu : bv;
       const int c0 = NG + 1 + pt, by = row + NG, sh = pt < 1 ? 1 : pt;  // stencil shift -max(1,pt)
       T d0 = 0, d1 = 0;
+      if (a.bc_kind == NRPYELL_OUTER_BC_RADIATION) {
 #pragma unroll
-      for (int j = 0; j < 11; j++) {
-        d0 = fma((T)c_fd1_shift[sh][j], box[by * NBX + c0 + (j - 5 - sh)], d0);
-        d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * NBX + c0], d1);
+        for (int j = 0; j < 11; j++) {
+          d0 = fma((T)c_fd1_shift[sh][j], box[by * NBX + c0 + (j - 5 - sh)], d0);
+          d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * NBX + c0], d1);
+        }
+      } else {
+        // the Einstein Toolkit thorn's 7-point stencils (conformally_flat_BBH_apply_bcs_sommerfeld.h:
+        // 49-105): backward one-sided along x0 (columns c0-6..c0, c0-6 = pt >= 0); along x1
+        // forward one-sided on the first interior row (rows by..by+6 < BYB), centred elsewhere
+        const T *col = box + by * NBX + c0;
+        d0 = (T)(-49.0 / 20.0) * col[0];
+#pragma unroll
+        for (int k = 1; k < 7; k++) d0 = fma((T)c_fd1_onesided7[k], col[-k], d0);
+        d0 = -d0;
+        if (i1s + row == NG) {
+          d1 = (T)(-49.0 / 20.0) * col[0];
+#pragma unroll
+          for (int k = 1; k < 7; k++) d1 = fma((T)c_fd1_onesided7[k], col[k * NBX], d1);
+        } else {
+          d1 = (T)(1.0 / 60.0) * col[3 * NBX];
+          d1 = fma((T)(-3.0 / 20.0), col[2 * NBX], d1);
+          d1 = fma((T)(3.0 / 4.0), col[NBX], d1);
+          d1 = fma((T)(-3.0 / 4.0), col[-NBX], d1);
+          d1 = fma((T)(3.0 / 20.0), col[-2 * NBX], d1);
+          d1 = fma((T)(-1.0 / 60.0), col[-3 * NBX], d1);
+        }
       }
       d0 *= (T)g.invdx0;
       d1 *= (T)g.invdx1;

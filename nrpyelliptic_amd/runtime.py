@@ -29,7 +29,13 @@ class solve_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("check_every", C.c_int), ("steps_per_graph", C.c_int),
                 ("log10_target", C.c_double), ("max_steps", C.c_int), ("fp32_switch_log10", C.c_double),
                 ("fp32_plateau_steps", C.c_int), ("integration_radius", C.c_double),
-                ("tile_i0", C.c_int), ("planes", C.c_int)]
+                ("tile_i0", C.c_int), ("planes", C.c_int), ("outer_bc", C.c_int), ("outer_bc_speed", C.c_double)]
+
+
+# outer boundary conditions of the solvers (nrpyell_solve_options.outer_bc)
+OUTER_BC = {"radiation": 0, "sommerfeld6": 1}
+# the Einstein Toolkit thorn's Sommerfeld wave speed (NRPYELL_ET_SOMMERFELD_SPEED)
+ET_SOMMERFELD_SPEED = 4.4793122676309114e+06
 
 
 class admbase_options(C.Structure):
@@ -92,6 +98,9 @@ def lib() -> C.CDLL:
     L.nrpyell_batch_destroy.argtypes = [vp]
     L.nrpyell_Initialize_ADMBase.argtypes = ([C.POINTER(abi.griddata_struct), C.POINTER(admbase_options),
                                               C.POINTER(C.c_int)] + [abi.REALP] * 20)
+    L.nrpyell_apply_bcs_sommerfeld.argtypes = [C.POINTER(abi.paramstruct), C.POINTER(abi.bc_struct), abi.XX3,
+                                               C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                               abi.REALP, abi.REALP]
     L.compute_TRC.argtypes = [C.c_double] * 6
     L.compute_TRC.restype = C.c_double
     L.set_optimum_eta_damping.argtypes = [C.c_double, C.c_double, C.POINTER(abi.paramstruct)]
@@ -107,9 +116,15 @@ def check(rc: int, what: str = "") -> None:
 
 
 def default_options(**kw) -> solve_options:
+    """nrpyell_solve_options with the library defaults, overridden by keyword; outer_bc takes
+    "radiation" or "sommerfeld6" as well as the integer."""
     o = solve_options()
     lib().nrpyell_default_options(C.byref(o))
     for k, v in kw.items():
+        if k == "outer_bc" and isinstance(v, str):
+            if v not in OUTER_BC:
+                raise NrpyellError(f"outer_bc {v!r}: expected one of {sorted(OUTER_BC)}")
+            v = OUTER_BC[v]
         setattr(o, k, v)
     return o
 
@@ -242,6 +257,18 @@ class Problem:
         finf = (C.c_double * 2)(0.0, 0.0)
         lib().apply_bcs_outerradiation_and_inner(C.byref(p), C.byref(self.gd.bcstruct), self.gd.xx, C.byref(ws),
                                                  C.byref(finf), dg.as_real_p(), dr.as_real_p())
+        return dr.download()
+
+    def apply_sommerfeld(self, gfs: np.ndarray, rhs: np.ndarray, speed=(ET_SOMMERFELD_SPEED,) * 2,
+                         f_infinity=(0.0, 0.0), falloff=(3.0, 3.0)) -> np.ndarray:
+        """The Einstein Toolkit thorn's outer BC (nrpyell_apply_bcs_sommerfeld) on a copy of rhs;
+        defaults are the thorn's constants."""
+        dg = DeviceArray(2 * self.ntot).upload(gfs)
+        dr = DeviceArray(2 * self.ntot).upload(rhs)
+        arr = [(C.c_double * 2)(*map(float, a)) for a in (speed, f_infinity, falloff)]
+        check(lib().nrpyell_apply_bcs_sommerfeld(C.byref(self.gd.params), C.byref(self.gd.bcstruct), self.gd.xx,
+                                                 arr[0], arr[1], arr[2], dg.as_real_p(), dr.as_real_p()),
+              "apply_bcs_sommerfeld")
         return dr.download()
 
     def apply_inner_only(self, gfs: np.ndarray) -> np.ndarray:

@@ -24,7 +24,13 @@ class OrcGrid(C.Structure):
     _fields_ = [("N", C.c_int * 3), ("Ng", C.c_int * 3), ("dxx", C.c_double * 3), ("invdx", C.c_double * 3),
                 ("AMAX", C.c_double), ("SINHWAA", C.c_double), ("bScale", C.c_double), ("eta", C.c_double),
                 ("mass", C.c_double * 2), ("pos", (C.c_double * 3) * 2), ("P", (C.c_double * 3) * 2),
-                ("S", (C.c_double * 3) * 2), ("cOB", C.c_double * 2)]
+                ("S", (C.c_double * 3) * 2), ("cOB", C.c_double * 2), ("outer_bc", C.c_int)]
+
+
+ORC_BC_RADIATION, ORC_BC_SOMMERFELD6 = 0, 1
+# the wave speed the Einstein Toolkit thorn's Sommerfeld BC uses on every grid
+# (conformally_flat_BBH_apply_bcs_sommerfeld.h:12)
+ET_SOMMERFELD_SPEED = 4.4793122676309114e+06
 
 
 ORC_RFM = ["f0", "f0_D0", "f0_DD00", "f0_DDD000", "f1", "f1_D1", "f1_DD11", "f2", "f2_D0", "f2_D1",
@@ -97,6 +103,9 @@ def lib():
         L.orc_residual.argtypes = [G, C.POINTER(OrcRfm), DP, DP, DP]
         L.orc_apply_bcs_inner_only.argtypes = [G, C.POINTER(OrcBc), DP]
         L.orc_apply_bcs_radiation_and_inner.argtypes = [G, C.POINTER(OrcBc), DP, DP, DP, DP, DP]
+        L.orc_apply_bcs_sommerfeld.argtypes = [G, C.POINTER(OrcBc), DP, DP, DP, DP, DP, DP, DP, DP]
+        L.orc_sommerfeld_fd1.argtypes = [G, DP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_sommerfeld_fd1.restype = C.c_double
         L.orc_mol_step.argtypes = [G, C.POINTER(OrcRfm), C.POINTER(OrcBc), DP, DP, DP, DP, DP, DP, DP, DP,
                                    C.c_double]
         L.orc_l2_norm.argtypes = [G, C.c_double, DP, DP, DP, DP]
@@ -121,7 +130,9 @@ def lib():
 class OracleProblem:
     """One configuration set up entirely by the C restatement, arrays held in numpy."""
 
-    def __init__(self, cfg: PunctureConfig):
+    def __init__(self, cfg: PunctureConfig, outer_bc: str = "radiation", outer_bc_speed: float = 0.0):
+        """outer_bc "radiation" (NRPyElliptic) or "sommerfeld6" (the Einstein Toolkit thorn's
+        BC, with its hard-coded wave speed unless outer_bc_speed > 0)."""
         L = lib()
         self.cfg = cfg
         g = OrcGrid()
@@ -162,6 +173,11 @@ class OracleProblem:
         L.orc_auxevol(C.byref(g), *[_p(a) for a in self.xx], _p(self.aux))
         L.orc_wavespeed(C.byref(g), cfg.CFL, self.dt, *[_p(a) for a in self.xx], _p(self.aux))
         c = L.orc_wavespeed_at_OB(C.byref(g), _p(self.aux))
+        if outer_bc == "sommerfeld6":
+            g.outer_bc = ORC_BC_SOMMERFELD6
+            c = outer_bc_speed if outer_bc_speed > 0 else ET_SOMMERFELD_SPEED
+        elif outer_bc != "radiation":
+            raise ValueError(f"outer_bc {outer_bc!r}")
         g.cOB[0] = g.cOB[1] = c
 
     def initialize_admbase(self, u, x, y, z, **opts):
@@ -221,6 +237,15 @@ class OracleProblem:
     def apply_radiation_and_inner(self, gfs, rhs):
         lib().orc_apply_bcs_radiation_and_inner(C.byref(self.g), C.byref(self.bc), *[_p(a) for a in self.xx],
                                                 _p(gfs), _p(rhs))
+
+    def apply_sommerfeld(self, gfs, rhs, speed=None, finf=(0.0, 0.0), falloff=(3.0, 3.0)):
+        sp = np.array(speed if speed is not None else self.g.cOB[:], dtype=np.float64)
+        fi, fo = np.array(finf, dtype=np.float64), np.array(falloff, dtype=np.float64)
+        lib().orc_apply_bcs_sommerfeld(C.byref(self.g), C.byref(self.bc), *[_p(a) for a in self.xx], _p(sp),
+                                       _p(fi), _p(fo), _p(gfs), _p(rhs))
+
+    def sommerfeld_fd1(self, f, d, i0, i1, i2, face):
+        return lib().orc_sommerfeld_fd1(C.byref(self.g), _p(f), d, i0, i1, i2, face)
 
     def apply_inner_only(self, gfs):
         lib().orc_apply_bcs_inner_only(C.byref(self.g), C.byref(self.bc), _p(gfs))

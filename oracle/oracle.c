@@ -40,6 +40,11 @@
  *                          (1/8pi) \oint K_ij n^j dA = P_i and (1/8pi) \oint eps_ijk x^j K^kl n_l dA = S_i
  *                          around a single puncture; lapse closed forms
  *   orc_compute_TRC        atanh written as a log; eta(TRC = 1) = 6.2
+ *   orc_apply_bcs_sommerfeld  the thorn's outer BC: 7-point weights regenerated as Fornberg
+ *                          rationals; exact on polynomials of degree <= 6; d_r r = 1 and
+ *                          d_r of the Cartesian z equal to z/r (Jacobian orientation);
+ *                          f = k/r gives rhs = 0 and an interior rhs K/r^3 is carried to
+ *                          K/r^3 at every outer layer (falloff power 3)
  */
 #include "oracle.h"
 
@@ -562,9 +567,106 @@ void orc_apply_bcs_radiation_and_inner(const orc_grid *g, const orc_bc *bc, cons
 }
 
 /* ---------------------------------------------------------------------------------
+ * The Einstein Toolkit thorn's outer BC (NRPyEllipticET/src/conformally_flat_BBH/
+ * conformally_flat_BBH_apply_bcs_sommerfeld.h). Same Sommerfeld condition as above, with
+ * 6th-order derivatives and the falloff term extrapolated layer by layer from the
+ * neighbour one point inward:
+ *   rhs_p = -c (d_r f_p + (f_p - f_inf)/r_p)
+ *           + (rhs_q + c (d_r f_q + (f_q - f_inf)/r_q)) (r_q/r_p)^n,   q = p + FACE.
+ * ------------------------------------------------------------------------------- */
+/* 7-point one-sided (Fornberg: -49/20, 6, -15/2, 20/3, -15/4, 6/5, -1/6) and centred
+ * (3/4, -3/20, 1/60) first-derivative weights, conformally_flat_BBH_apply_bcs_sommerfeld.h:33-46 */
+static const double SOM_UP[7] = {-49.0 / 20.0, 6.0, -15.0 / 2.0, 20.0 / 3.0, -15.0 / 4.0, 6.0 / 5.0, -1.0 / 6.0};
+static const double SOM_C1 = 3.0 / 4.0, SOM_C2 = 3.0 / 20.0, SOM_C3 = 1.0 / 60.0;
+
+/* conformally_flat_BBH_apply_bcs_sommerfeld.h:49-105: one-sided along the face normal, and
+ * wherever idx + NGHOSTS >= Ng (backward) or idx - NGHOSTS <= 0 (forward); centred otherwise.
+ * The one-sided sum runs from the point itself outward, the centred one from +3 down to -3. */
+double orc_sommerfeld_fd1(const orc_grid *g, const double *f, int dir, int i0, int i1, int i2, int face) {
+  const int idx = dir == 0 ? i0 : i1;
+  const long st = dir == 0 ? 1 : g->Ng[0];
+  const long p = IDX3(g, i0, i1, i2);
+  if (abs(face) == 1 || idx + NG >= g->Ng[dir] || idx - NG <= 0) {
+    int sh = face;
+    if (idx + NG >= g->Ng[dir]) sh = -1;
+    if (idx - NG <= 0) sh = +1;
+    double s = SOM_UP[0] * f[p];
+    for (int k = 1; k < 7; k++) s = fma(SOM_UP[k], f[p + k * sh * st], s);
+    return (sh * s) * g->invdx[dir];
+  }
+  double s = SOM_C3 * f[p + 3 * st];
+  s = fma(-SOM_C2, f[p + 2 * st], s);
+  s = fma(SOM_C1, f[p + st], s);
+  s = fma(-SOM_C1, f[p - st], s);
+  s = fma(SOM_C2, f[p - 2 * st], s);
+  s = fma(-SOM_C3, f[p - 3 * st], s);
+  return s * g->invdx[dir];
+}
+
+/* d_r f = dxx0/dr d_0 f + dxx1/dr d_1 f (conformally_flat_BBH_contraction_term, :16-137; r does
+ * not depend on xx2). The thorn's closed form for the two Jacobian factors is the inverse of
+ * d(r, cos theta)/d(xx0, xx1), the same quantity radial_geometry restates from the map. */
+static double sommerfeld_dr(const orc_grid *g, const double *f, int i0, int i1, int i2, const int face[3],
+                            const double geo[3]) {
+  const double d0 = orc_sommerfeld_fd1(g, f, 0, i0, i1, i2, face[0]);
+  const double d1 = orc_sommerfeld_fd1(g, f, 1, i0, i1, i2, face[1]);
+  return geo[1] * d0 + geo[2] * d1;
+}
+
+/* conformally_flat_BBH_apply_bcs_sommerfeld.h:139-222. The thorn copies inner ghost points
+ * layer by layer after that layer's outer points; every inner source is an interior point or
+ * a pure outer point of the same or an inner layer (the SinhSymTP maps keep i0 or move it into
+ * the interior), so one copy after all layers gives the same values. */
+void orc_apply_bcs_sommerfeld(const orc_grid *g, const orc_bc *bc, const double *xx0, const double *xx1,
+                              const double *xx2, const double speed[2], const double finf[2],
+                              const double falloff[2], const double *gfs, double *rhs) {
+  (void)xx2;
+  const long nt = NTOT(g);
+  for (int gz = 0; gz < NG; gz++)
+    for (int dirn = 0; dirn < 3; dirn++) {
+      const int slot = dirn + 3 * gz;
+      for (int k = 0; k < bc->num_outer[slot]; k++) {
+        const orc_outer *o = &bc->outer[slot][k];
+        const int face[3] = {o->FACEX0, o->FACEX1, o->FACEX2};
+        const int j0 = o->i0 + face[0], j1 = o->i1 + face[1], j2 = o->i2 + face[2];
+        double geo[3], geo_q[3];
+        radial_geometry(g, xx0[o->i0], xx1[o->i1], geo);
+        radial_geometry(g, xx0[j0], xx1[j1], geo_q);
+        const long p = IDX3(g, o->i0, o->i1, o->i2), q = IDX3(g, j0, j1, j2);
+        for (int gf = 0; gf < 2; gf++) {
+          const double *f = gfs + gf * nt;
+          double *rh = rhs + gf * nt;
+          const double c = speed[gf];
+          const double invr = 1.0 / geo[0];
+          double val = -c * (sommerfeld_dr(g, f, o->i0, o->i1, o->i2, face, geo) + invr * (f[p] - finf[gf]));
+          if (falloff[gf] > 0) {
+            const double invr_q = 1.0 / geo_q[0];
+            const double extrap = c * (sommerfeld_dr(g, f, j0, j1, j2, face, geo_q) + invr_q * (f[q] - finf[gf]));
+            val += (rh[q] + extrap) * pow(geo_q[0] * invr, falloff[gf]);
+          }
+          rh[p] = val;
+        }
+      }
+    }
+  orc_apply_bcs_inner_only(g, bc, rhs);
+}
+
+static void outer_bc(const orc_grid *g, const orc_bc *bc, const double *xx0, const double *xx1,
+                     const double *xx2, const double *gfs, double *rhs) {
+  if (g->outer_bc == ORC_BC_SOMMERFELD6) {
+    const double finf[2] = {0.0, 0.0}, falloff[2] = {3.0, 3.0};  /* :10-11 */
+    orc_apply_bcs_sommerfeld(g, bc, xx0, xx1, xx2, g->cOB, finf, falloff, gfs, rhs);
+  } else {
+    orc_apply_bcs_radiation_and_inner(g, bc, xx0, xx1, xx2, gfs, rhs);
+  }
+}
+
+/* ---------------------------------------------------------------------------------
  * RK4 method of lines, k_odd/k_even scheme (MoL_step_forward_in_time.c:7-178; Butcher
  * table RK_Butcher_Table_Dictionary.py:86-91). Stage updates cover every point,
  * ghosts included, with the reference FMA placement (:44-45, :88-89, :129-130, :170).
+ * With g->outer_bc == ORC_BC_SOMMERFELD6 the thorn's BC takes the radiation BC's place, as in
+ * conformally_flat_BBH_MoL_step_forward_in_time.c:27,53,80,107.
  * ------------------------------------------------------------------------------- */
 void orc_mol_step(const orc_grid *g, const orc_rfm *rfm, const orc_bc *bc, const double *xx0,
                   const double *xx1, const double *xx2, const double *aux, double *y_n, double *rt,
@@ -572,7 +674,7 @@ void orc_mol_step(const orc_grid *g, const orc_rfm *rfm, const orc_bc *bc, const
   const long n = 2 * NTOT(g);
   /* k1 */
   orc_rhs_eval(g, rfm, aux, y_n, k_odd);
-  orc_apply_bcs_radiation_and_inner(g, bc, xx0, xx1, xx2, y_n, k_odd);
+  outer_bc(g, bc, xx0, xx1, xx2, y_n, k_odd);
   for (long i = 0; i < n; i++) {
     const double k = k_odd[i];
     rt[i] = (1.0 / 6.0) * (dt * k);
@@ -580,7 +682,7 @@ void orc_mol_step(const orc_grid *g, const orc_rfm *rfm, const orc_bc *bc, const
   }
   /* k2 */
   orc_rhs_eval(g, rfm, aux, k_odd, k_even);
-  orc_apply_bcs_radiation_and_inner(g, bc, xx0, xx1, xx2, k_odd, k_even);
+  outer_bc(g, bc, xx0, xx1, xx2, k_odd, k_even);
   for (long i = 0; i < n; i++) {
     const double k = k_even[i];
     rt[i] = fma(1.0 / 3.0, dt * k, rt[i]);
@@ -588,7 +690,7 @@ void orc_mol_step(const orc_grid *g, const orc_rfm *rfm, const orc_bc *bc, const
   }
   /* k3 */
   orc_rhs_eval(g, rfm, aux, k_even, k_odd);
-  orc_apply_bcs_radiation_and_inner(g, bc, xx0, xx1, xx2, k_even, k_odd);
+  outer_bc(g, bc, xx0, xx1, xx2, k_even, k_odd);
   for (long i = 0; i < n; i++) {
     const double k = k_odd[i];
     rt[i] = fma(1.0 / 3.0, dt * k, rt[i]);
@@ -596,7 +698,7 @@ void orc_mol_step(const orc_grid *g, const orc_rfm *rfm, const orc_bc *bc, const
   }
   /* k4 */
   orc_rhs_eval(g, rfm, aux, k_odd, k_even);
-  orc_apply_bcs_radiation_and_inner(g, bc, xx0, xx1, xx2, k_odd, k_even);
+  outer_bc(g, bc, xx0, xx1, xx2, k_odd, k_even);
   for (long i = 0; i < n; i++) y_n[i] = y_n[i] + fma(1.0 / 6.0, dt * k_even[i], rt[i]);
 }
 

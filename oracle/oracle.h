@@ -32,7 +32,13 @@ typedef struct {
   double P[2][3];    /* linear momenta                                           */
   double S[2][3];    /* spins                                                    */
   double cOB[2];     /* wavespeed used by the radiation BC for u and v           */
+  int outer_bc;      /* ORC_BC_RADIATION (default) or ORC_BC_SOMMERFELD6          */
 } orc_grid;
+
+/* Outer boundary condition of orc_mol_step. */
+#define ORC_BC_RADIATION 0   /* NRPyElliptic: 11-point 10th-order stencils, k/r^3 extrapolation */
+#define ORC_BC_SOMMERFELD6 1 /* Einstein Toolkit thorn: 7-point 6th-order stencils, radial
+                                falloff power 3, wave speed cOB (the thorn hard-codes it)      */
 
 /* Same member order as the reference rfm_struct (NRPy_basic_defines.h:135-151). */
 typedef struct {
@@ -74,6 +80,15 @@ void   orc_apply_bcs_inner_only(const orc_grid *g, const orc_bc *bc, double *gfs
 void   orc_apply_bcs_radiation_and_inner(const orc_grid *g, const orc_bc *bc, const double *xx0,
                                          const double *xx1, const double *xx2, const double *gfs,
                                          double *rhs);
+/* The Einstein Toolkit thorn's outer BC (conformally_flat_BBH_apply_bcs_sommerfeld.h:139-222)
+ * followed by the inner parity copy on the RHS; speed, f_inf and the radial falloff power per
+ * evolved gridfunction. */
+void   orc_apply_bcs_sommerfeld(const orc_grid *g, const orc_bc *bc, const double *xx0, const double *xx1,
+                                const double *xx2, const double speed[2], const double finf[2],
+                                const double falloff[2], const double *gfs, double *rhs);
+/* 7-point first derivative of the thorn along dir (0: x0, 1: x1) at (i0,i1,i2) with face
+ * direction `face` (conformally_flat_BBH_apply_bcs_sommerfeld.h:16-105); exposed for tests. */
+double orc_sommerfeld_fd1(const orc_grid *g, const double *f, int dir, int i0, int i1, int i2, int face);
 void   orc_mol_step(const orc_grid *g, const orc_rfm *rfm, const orc_bc *bc, const double *xx0,
                     const double *xx1, const double *xx2, const double *aux, double *y_n, double *rt,
                     double *k_odd, double *k_even, double dt);

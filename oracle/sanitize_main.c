@@ -58,6 +58,14 @@ int main(void) {
   orc_rhs_eval(&g, &rfm, aux, yn, ko);
   orc_apply_bcs_radiation_and_inner(&g, &bc, xx[0], xx[1], xx[2], yn, ko);
   for (int s = 0; s < 3; s++) orc_mol_step(&g, &rfm, &bc, xx[0], xx[1], xx[2], aux, yn, rt, ko, ke, dt);
+  /* the thorn's Sommerfeld BC, alone and inside the RK4 step */
+  {
+    const double sp[2] = {4.4793122676309114e+06, 4.4793122676309114e+06}, fi[2] = {0, 0}, fo[2] = {3, 3};
+    orc_apply_bcs_sommerfeld(&g, &bc, xx[0], xx[1], xx[2], sp, fi, fo, yn, ko);
+    g.outer_bc = ORC_BC_SOMMERFELD6;
+    orc_mol_step(&g, &rfm, &bc, xx[0], xx[1], xx[2], aux, yn, rt, ko, ke, dt);
+    g.outer_bc = ORC_BC_RADIATION;
+  }
   orc_residual(&g, &rfm, aux, yn, res);
   const double l2 = orc_l2_norm(&g, 100.0, xx[0], xx[1], xx[2], res);
   double w = 0;
