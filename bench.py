@@ -350,6 +350,14 @@ def main():
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
     r = S.solve()
     nn = nn_seeded_time_to_residual(P, S, r)
+    # the Einstein Toolkit thorn's solver variant on its default grid (N0, N1, N2 = 128, 128, 16):
+    # the same relaxation with the thorn's 6th-order Sommerfeld outer BC and wave speed
+    S_et = P.solver(steps_per_graph=100, outer_bc="sommerfeld6")
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    r_et = S_et.solve()
+    S_et.close()
+    et_thorn = {"outer_bc": "sommerfeld6", "wave_speed": runtime.ET_SOMMERFELD_SPEED, "steps": r_et.steps,
+                "seconds": r_et.seconds, "log10_l2": r_et.log10_l2, "converged": bool(r_et.converged)}
     # per-stage kernel durations (back-to-back launches, HIP events on the solver stream)
     stage_us = {f"stage{k}": round(S.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)}
     dropin_us = dropin_call_us(P)
@@ -387,7 +395,7 @@ def main():
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
                              "log10_l2": r.log10_l2, "converged": bool(r.converged),
                              "reference_steps": 3144, "check_interval_steps": 1,
-                             "nn_seeded_mixed_vs_plain_fp64": nn},
+                             "nn_seeded_mixed_vs_plain_fp64": nn, "et_thorn_sommerfeld_bc": et_thorn},
     }
     print(json.dumps(line), flush=True)
 
