@@ -246,8 +246,9 @@ struct DeviceBc {
   std::vector<innerpt_bc_struct> probe;  // sampled host records used to validate the cache
   int *dst = nullptr, *src = nullptr;
   int8_t *par0 = nullptr;
-  // radiation lists per slot (dirn + 3*gz), built against one xx grid
-  const double *xx_key0 = nullptr, *xx_key1 = nullptr;
+  // radiation lists per slot (dirn + 3*gz), built for one grid: keyed on the values the
+  // geometry depends on (AMAX, SINHWAA, bScale, xx0, xx1), not on the xx pointers
+  std::vector<double> geo_key;
   RadPoint *rad[NGHOSTS * 3] = {};
   int nrad[NGHOSTS * 3] = {};
   bool rows = false;  // x0 face only, same rows in every layer: one launch for all layers
@@ -319,7 +320,10 @@ static int sommerfeld_shift(int idx, int face, int G) {
 }
 
 static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const bc_struct *bc, REAL *xx[3]) {
-  if (d.xx_key0 == xx[0] && d.xx_key1 == xx[1]) return;
+  std::vector<double> key = {params->AMAX, params->SINHWAA, params->bScale};
+  key.insert(key.end(), xx[0], xx[0] + params->Nxx_plus_2NGHOSTS0);
+  key.insert(key.end(), xx[1], xx[1] + params->Nxx_plus_2NGHOSTS1);
+  if (key == d.geo_key) return;
   const GridDims g = dims_from_params(params);
   for (int s = 0; s < NGHOSTS * 3; s++) {
     (void)hipFree(d.rad[s]);
@@ -362,8 +366,7 @@ static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const
       if (a.FACEX0 != -1 || b.FACEX0 != -1 || a.i1 != b.i1 || a.i2 != b.i2 || a.i0 != b.i0 + 1) d.rows = false;
     }
   }
-  d.xx_key0 = xx[0];
-  d.xx_key1 = xx[1];
+  d.geo_key = std::move(key);
 }
 
 // ---------------------------------------------------------------------------------------

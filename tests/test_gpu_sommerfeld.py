@@ -130,3 +130,28 @@ def test_fp32_phase_and_batch_with_the_thorn_bc(converged):
     assert np.array_equal(ub, us)
     scale = np.max(np.abs(us))
     assert np.max(np.abs(u32 - us)) <= 1e-6 * scale
+
+
+def test_bc_geometry_follows_the_grid_values():
+    """The outer-BC point lists are rebuilt when the geometry changes behind the same xx
+    pointers (AMAX changed in place): both drop-in BCs then match the oracle on the new grid."""
+    cfg = SMALL[0]
+    P = runtime.Problem(cfg)
+    O = oracle.OracleProblem(cfg, outer_bc="sommerfeld6")
+    rng = np.random.default_rng(5)
+    st = rng.uniform(-0.1, 0.1, 2 * P.ntot)
+    rhs = rng.uniform(-1, 1, 2 * P.ntot)
+    c = P.gd.params.uu_wavespeed_at_OB
+    for amax in (P.gd.params.AMAX, 0.5 * P.gd.params.AMAX):
+        print(f"bc geometry with AMAX {amax}", flush=True)
+        P.gd.params.AMAX = amax
+        O.g.AMAX = amax
+        O.g.cOB[0] = O.g.cOB[1] = c
+        want = rhs.copy()
+        O.apply_radiation_and_inner(st, want)
+        assert rel_maxnorm(P.apply_radiation_and_inner(st, rhs), want) <= 1e-13
+        want = rhs.copy()
+        O.apply_sommerfeld(st, want)
+        got = P.apply_sommerfeld(st, rhs, speed=(c, c))
+        assert rel_maxnorm(got, want) <= 1e-13
+    P.close()
