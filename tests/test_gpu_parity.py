@@ -19,10 +19,12 @@ SMALL = [config.shipped_free_parameters((16, 16, 6)), config.unequal_mass_q4((32
          config.random_batch(1, seed=42, grids=False, Nxx=(24, 16, 6))[0],
          config.random_batch(1, seed=5, grids=False, Nxx=(16, 40, 8))[0],
          config.random_batch(1, seed=9, grids=False, Nxx=(33, 16, 6))[0],
-         config.random_batch(1, seed=42, grids=False, Nxx=(96, 128, 16))[0]]
+         config.random_batch(1, seed=42, grids=False, Nxx=(96, 128, 16))[0],
+         config.random_batch(1, seed=7, grids=False, Nxx=(160, 128, 16))[0]]
 # 33: the tile holding the last interior i0 holds only that column (the one-sided BC stencil
-# reaches into the previous tile)
-IDS = ["shipped_16x16x6", "q4_32x32x8", "rand_24x16x6", "rand_16x40x8", "rand_33x16x6", "rand_96x128x16"]
+# reaches into the previous tile); 160x128x16: the largest grid of the batched workloads
+IDS = ["shipped_16x16x6", "q4_32x32x8", "rand_24x16x6", "rand_16x40x8", "rand_33x16x6", "rand_96x128x16",
+       "max_160x128x16"]
 
 
 def interior_mask(Ng):
