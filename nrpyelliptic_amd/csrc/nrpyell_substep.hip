@@ -63,12 +63,17 @@ __device__ __forceinline__ void fd_sums(const T m[6], const T p[6], T uu, T &s2,
   s1 = fma((T)FD1_W5, p[5] - m[5], s1);
 }
 
-// Fixed-order sum of the per-wave partials of a block.
+// Fixed-order sum of the NW per-wave partials of a block (reads s[0..NW-1] only).
 template <int NW>
 __device__ __forceinline__ double wave_sum(const double *s) {
-  double acc = (s[0] + s[1]) + (s[2] + s[3]);
-  if (NW == 8) acc += (s[4] + s[5]) + (s[6] + s[7]);
-  return acc;
+  static_assert(NW == 2 || NW == 4 || NW == 8, "blocks of 128, 256 or 512 threads");
+  if constexpr (NW == 2) {
+    return s[0] + s[1];
+  } else {
+    double acc = (s[0] + s[1]) + (s[2] + s[3]);
+    if constexpr (NW == 8) acc += (s[4] + s[5]) + (s[6] + s[7]);
+    return acc;
+  }
 }
 
 // RHS at one point (rhs_eval.c:86-110): ku = v - eta u and kv = c^2 (lap u + AA^2/(8 (psi+u)^7))
