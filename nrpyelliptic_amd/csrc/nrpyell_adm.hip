@@ -146,108 +146,110 @@ extern "C" int nrpyell_Initialize_ADMBase(const griddata_struct *gd, const nrpye
                                           REAL *uuGF, REAL *alp, REAL *betax, REAL *betay, REAL *betaz, REAL *gxx,
                                           REAL *gxy, REAL *gxz, REAL *gyy, REAL *gyz, REAL *gzz, REAL *kxx,
                                           REAL *kxy, REAL *kxz, REAL *kyy, REAL *kyz, REAL *kzz) {
-  if (!gd || !opt || !cctk_lsh || !x || !y || !z || !gd->gridfuncs.y_n_gfs || !gd->xx[0] || !gd->xx[1] ||
-      !gd->xx[2]) {
-    set_last_error("Initialize_ADMBase: null argument");
-    return NRPYELL_EINVAL;
-  }
-  if (opt->orbital_plane != NRPYELL_ORBITAL_PLANE_XY && opt->orbital_plane != NRPYELL_ORBITAL_PLANE_XZ) {
-    set_last_error("Unsupported orbital plane. Available options are \"xy\" (\"yx\") or \"xz\" (\"zx\").");
-    return NRPYELL_EINVAL;
-  }
-  if (opt->initial_lapse != NRPYELL_LAPSE_PSI_N && opt->initial_lapse != NRPYELL_LAPSE_AVERAGED) {
-    set_last_error("NRPyEllipticET does not support this lapse initial data");
-    return NRPYELL_EINVAL;
-  }
-  if (opt->interpolation_order < 1 || opt->interpolation_order > 6) {
-    set_last_error("interpolation_order must be 1..6 (Lagrange tensor product)");
-    return NRPYELL_EINVAL;
-  }
-  REAL *outs[NRPYELL_ADM_NOUT] = {uuGF, alp, betax, betay, betaz, gxx, gxy, gxz, gyy,
-                                  gyz, gzz, kxx, kxy, kxz, kyy, kyz, kzz};
-  for (int c = 0; c < NRPYELL_ADM_NOUT; c++)
-    if (!outs[c]) {
-      set_last_error("Initialize_ADMBase: null output array");
+  return api_call([&]() -> int {
+    if (!gd || !opt || !cctk_lsh || !x || !y || !z || !gd->gridfuncs.y_n_gfs || !gd->xx[0] || !gd->xx[1] ||
+        !gd->xx[2]) {
+      set_last_error("Initialize_ADMBase: null argument");
       return NRPYELL_EINVAL;
     }
-  const long n = (long)cctk_lsh[0] * cctk_lsh[1] * cctk_lsh[2];
-  if (cctk_lsh[0] < 0 || cctk_lsh[1] < 0 || cctk_lsh[2] < 0) {
-    set_last_error("Initialize_ADMBase: negative cctk_lsh");
-    return NRPYELL_EINVAL;
-  }
-  if (n == 0) return NRPYELL_OK;
+    if (opt->orbital_plane != NRPYELL_ORBITAL_PLANE_XY && opt->orbital_plane != NRPYELL_ORBITAL_PLANE_XZ) {
+      set_last_error("Unsupported orbital plane. Available options are \"xy\" (\"yx\") or \"xz\" (\"zx\").");
+      return NRPYELL_EINVAL;
+    }
+    if (opt->initial_lapse != NRPYELL_LAPSE_PSI_N && opt->initial_lapse != NRPYELL_LAPSE_AVERAGED) {
+      set_last_error("NRPyEllipticET does not support this lapse initial data");
+      return NRPYELL_EINVAL;
+    }
+    if (opt->interpolation_order < 1 || opt->interpolation_order > 6) {
+      set_last_error("interpolation_order must be 1..6 (Lagrange tensor product)");
+      return NRPYELL_EINVAL;
+    }
+    REAL *outs[NRPYELL_ADM_NOUT] = {uuGF, alp, betax, betay, betaz, gxx, gxy, gxz, gyy,
+                                    gyz, gzz, kxx, kxy, kxz, kyy, kyz, kzz};
+    for (int c = 0; c < NRPYELL_ADM_NOUT; c++)
+      if (!outs[c]) {
+        set_last_error("Initialize_ADMBase: null output array");
+        return NRPYELL_EINVAL;
+      }
+    const long n = (long)cctk_lsh[0] * cctk_lsh[1] * cctk_lsh[2];
+    if (cctk_lsh[0] < 0 || cctk_lsh[1] < 0 || cctk_lsh[2] < 0) {
+      set_last_error("Initialize_ADMBase: negative cctk_lsh");
+      return NRPYELL_EINVAL;
+    }
+    if (n == 0) return NRPYELL_OK;
 
-  const paramstruct *p = &gd->params;
-  AdmArgs a;
-  a.dims[0] = p->Nxx_plus_2NGHOSTS0;
-  a.dims[1] = p->Nxx_plus_2NGHOSTS1;
-  a.dims[2] = p->Nxx_plus_2NGHOSTS2;
-  for (int d = 0; d < 3; d++) {
-    if (a.dims[d] < opt->interpolation_order + 1) {
-      set_last_error("Initialize_ADMBase: grid smaller than the interpolation molecule");
+    const paramstruct *p = &gd->params;
+    AdmArgs a;
+    a.dims[0] = p->Nxx_plus_2NGHOSTS0;
+    a.dims[1] = p->Nxx_plus_2NGHOSTS1;
+    a.dims[2] = p->Nxx_plus_2NGHOSTS2;
+    for (int d = 0; d < 3; d++) {
+      if (a.dims[d] < opt->interpolation_order + 1) {
+        set_last_error("Initialize_ADMBase: grid smaller than the interpolation molecule");
+        return NRPYELL_EINVAL;
+      }
+      a.origin[d] = gd->xx[d][0];
+      a.delta[d] = gd->xx[d][1] - gd->xx[d][0];
+    }
+    a.AMAX = p->AMAX;
+    a.SINHWAA = p->SINHWAA;
+    a.bScale = p->bScale;
+    a.xy = opt->orbital_plane == NRPYELL_ORBITAL_PLANE_XY;
+    a.lapse = opt->initial_lapse == NRPYELL_LAPSE_AVERAGED;
+    a.lapse_n = opt->lapse_exponent_n;
+    a.eps = opt->NRPy_epsilon;
+    for (int d = 0; d < 3; d++) a.shift[d] = opt->position_shift[d];
+    // The solve ran in the NRPy frame; for the "xy" plane the relaxation driver had mapped the
+    // parameter file's punctures (x,y,z)_NRPy = (y,z,x)_ETK
+    // (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:84-105): undo it, then shift
+    // (Initialize_ADMBase.c:35-42).
+    const double pos[2][3] = {{p->puncture_0_x, p->puncture_0_y, p->puncture_0_z},
+                              {p->puncture_1_x, p->puncture_1_y, p->puncture_1_z}};
+    const double P[2][3] = {{p->P0_x, p->P0_y, p->P0_z}, {p->P1_x, p->P1_y, p->P1_z}};
+    const double S[2][3] = {{p->S0_x, p->S0_y, p->S0_z}, {p->S1_x, p->S1_y, p->S1_z}};
+    const int perm[3] = {a.xy ? 2 : 0, a.xy ? 0 : 1, a.xy ? 1 : 2};  // ETK axis d = NRPy axis perm[d]
+    a.etk.m[0] = p->bare_mass_0;
+    a.etk.m[1] = p->bare_mass_1;
+    for (int k = 0; k < 2; k++)
+      for (int d = 0; d < 3; d++) {
+        a.etk.pos[k][d] = pos[k][perm[d]] + a.shift[d];
+        a.etk.P[k][d] = P[k][perm[d]];
+        a.etk.S[k][d] = S[k][perm[d]];
+      }
+    a.u = gd->gridfuncs.y_n_gfs;  // gridfunction UUGF = 0
+    a.x = x;
+    a.y = y;
+    a.z = z;
+    for (int c = 0; c < NRPYELL_ADM_NOUT; c++) a.out[c] = outs[c];
+    a.n = n;
+    int *err = nullptr;
+    HIPCHK(hipMalloc(&err, sizeof(int)));
+    HIPCHK(hipMemset(err, 0, sizeof(int)));
+    a.err = err;
+    const dim3 grd((unsigned)((n + 255) / 256)), blk(256);
+    switch (opt->interpolation_order) {
+      case 1: hipLaunchKernelGGL(admbase_kernel<1>, grd, blk, 0, 0, a); break;
+      case 2: hipLaunchKernelGGL(admbase_kernel<2>, grd, blk, 0, 0, a); break;
+      case 3: hipLaunchKernelGGL(admbase_kernel<3>, grd, blk, 0, 0, a); break;
+      case 5: hipLaunchKernelGGL(admbase_kernel<5>, grd, blk, 0, 0, a); break;
+      case 6: hipLaunchKernelGGL(admbase_kernel<6>, grd, blk, 0, 0, a); break;
+      default: hipLaunchKernelGGL(admbase_kernel<4>, grd, blk, 0, 0, a); break;
+    }
+    HIPCHK(hipGetLastError());
+    int herr = 0;
+    HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipFree(err));
+    if (herr == 1) {
+      set_last_error("NAN found in (xx0,xx1,xx2) after moving the point by NRPy_epsilon; consider increasing "
+                     "NRPy_epsilon");
+      return NRPYELL_ENAN;
+    }
+    if (herr == 2) {
+      set_last_error("Interpolation point outside the NRPyElliptic grid");
       return NRPYELL_EINVAL;
     }
-    a.origin[d] = gd->xx[d][0];
-    a.delta[d] = gd->xx[d][1] - gd->xx[d][0];
-  }
-  a.AMAX = p->AMAX;
-  a.SINHWAA = p->SINHWAA;
-  a.bScale = p->bScale;
-  a.xy = opt->orbital_plane == NRPYELL_ORBITAL_PLANE_XY;
-  a.lapse = opt->initial_lapse == NRPYELL_LAPSE_AVERAGED;
-  a.lapse_n = opt->lapse_exponent_n;
-  a.eps = opt->NRPy_epsilon;
-  for (int d = 0; d < 3; d++) a.shift[d] = opt->position_shift[d];
-  // The solve ran in the NRPy frame; for the "xy" plane the relaxation driver had mapped the
-  // parameter file's punctures (x,y,z)_NRPy = (y,z,x)_ETK
-  // (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:84-105): undo it, then shift
-  // (Initialize_ADMBase.c:35-42).
-  const double pos[2][3] = {{p->puncture_0_x, p->puncture_0_y, p->puncture_0_z},
-                            {p->puncture_1_x, p->puncture_1_y, p->puncture_1_z}};
-  const double P[2][3] = {{p->P0_x, p->P0_y, p->P0_z}, {p->P1_x, p->P1_y, p->P1_z}};
-  const double S[2][3] = {{p->S0_x, p->S0_y, p->S0_z}, {p->S1_x, p->S1_y, p->S1_z}};
-  const int perm[3] = {a.xy ? 2 : 0, a.xy ? 0 : 1, a.xy ? 1 : 2};  // ETK axis d = NRPy axis perm[d]
-  a.etk.m[0] = p->bare_mass_0;
-  a.etk.m[1] = p->bare_mass_1;
-  for (int k = 0; k < 2; k++)
-    for (int d = 0; d < 3; d++) {
-      a.etk.pos[k][d] = pos[k][perm[d]] + a.shift[d];
-      a.etk.P[k][d] = P[k][perm[d]];
-      a.etk.S[k][d] = S[k][perm[d]];
-    }
-  a.u = gd->gridfuncs.y_n_gfs;  // gridfunction UUGF = 0
-  a.x = x;
-  a.y = y;
-  a.z = z;
-  for (int c = 0; c < NRPYELL_ADM_NOUT; c++) a.out[c] = outs[c];
-  a.n = n;
-  int *err = nullptr;
-  HIPCHK(hipMalloc(&err, sizeof(int)));
-  HIPCHK(hipMemset(err, 0, sizeof(int)));
-  a.err = err;
-  const dim3 grd((unsigned)((n + 255) / 256)), blk(256);
-  switch (opt->interpolation_order) {
-    case 1: hipLaunchKernelGGL(admbase_kernel<1>, grd, blk, 0, 0, a); break;
-    case 2: hipLaunchKernelGGL(admbase_kernel<2>, grd, blk, 0, 0, a); break;
-    case 3: hipLaunchKernelGGL(admbase_kernel<3>, grd, blk, 0, 0, a); break;
-    case 5: hipLaunchKernelGGL(admbase_kernel<5>, grd, blk, 0, 0, a); break;
-    case 6: hipLaunchKernelGGL(admbase_kernel<6>, grd, blk, 0, 0, a); break;
-    default: hipLaunchKernelGGL(admbase_kernel<4>, grd, blk, 0, 0, a); break;
-  }
-  HIPCHK(hipGetLastError());
-  int herr = 0;
-  HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
-  HIPCHK(hipFree(err));
-  if (herr == 1) {
-    set_last_error("NAN found in (xx0,xx1,xx2) after moving the point by NRPy_epsilon; consider increasing "
-                   "NRPy_epsilon");
-    return NRPYELL_ENAN;
-  }
-  if (herr == 2) {
-    set_last_error("Interpolation point outside the NRPyElliptic grid");
-    return NRPYELL_EINVAL;
-  }
-  return NRPYELL_OK;
+    return NRPYELL_OK;
+  });
 }
 
 // conformally_flat_BBH_set_optimum_eta_damping.c:3-16: eq. (B6) of arXiv:2111.02424.

@@ -88,75 +88,91 @@ static T *upload(const T *h, size_t n) {
 // Reference surface: memory and setup
 // ---------------------------------------------------------------------------------------
 extern "C" void MoL_malloc_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
-  const long nt = dims_from_params(params).ntot;
-  gf->y_n_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
-  gf->diagnostic_output_gfs = gf->y_nplus1_running_total_gfs;  // MoL_malloc_y_n_gfs.c:14
+  ref_call("MoL_malloc_y_n_gfs", [&]() {
+    const long nt = dims_from_params(params).ntot;
+    gf->y_n_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+    gf->diagnostic_output_gfs = gf->y_nplus1_running_total_gfs;  // MoL_malloc_y_n_gfs.c:14
+  });
 }
 extern "C" void MoL_free_memory_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
-  (void)params;
-  HIPCHK(hipFree(gf->y_n_gfs));
-  gf->y_n_gfs = nullptr;
+  ref_call("MoL_free_memory_y_n_gfs", [&]() {
+    (void)params;
+    HIPCHK(hipFree(gf->y_n_gfs));
+    gf->y_n_gfs = nullptr;
+  });
 }
 extern "C" void MoL_malloc_non_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
-  const long nt = dims_from_params(params).ntot;
-  gf->y_nplus1_running_total_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
-  gf->k_odd_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
-  gf->k_even_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
-  gf->auxevol_gfs = dalloc<double>(NUM_AUXEVOL_GFS * nt);
-  // every array starts defined (the reference's malloc'd buffers are fully written before use)
-  HIPCHK(hipMemset(gf->y_nplus1_running_total_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
-  HIPCHK(hipMemset(gf->k_odd_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
-  HIPCHK(hipMemset(gf->k_even_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
-  HIPCHK(hipMemset(gf->auxevol_gfs, 0, NUM_AUXEVOL_GFS * nt * sizeof(double)));
-  gf->diagnostic_output_gfs = gf->y_nplus1_running_total_gfs;  // MoL_malloc_non_y_n_gfs.c:17
+  ref_call("MoL_malloc_non_y_n_gfs", [&]() {
+    const long nt = dims_from_params(params).ntot;
+    gf->y_nplus1_running_total_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+    gf->k_odd_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+    gf->k_even_gfs = dalloc<double>(NUM_EVOL_GFS * nt);
+    gf->auxevol_gfs = dalloc<double>(NUM_AUXEVOL_GFS * nt);
+    // every array starts defined (the reference's malloc'd buffers are fully written before use)
+    HIPCHK(hipMemset(gf->y_nplus1_running_total_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+    HIPCHK(hipMemset(gf->k_odd_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+    HIPCHK(hipMemset(gf->k_even_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+    HIPCHK(hipMemset(gf->auxevol_gfs, 0, NUM_AUXEVOL_GFS * nt * sizeof(double)));
+    gf->diagnostic_output_gfs = gf->y_nplus1_running_total_gfs;  // MoL_malloc_non_y_n_gfs.c:17
+  });
 }
 extern "C" void MoL_free_memory_non_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
-  (void)params;
-  HIPCHK(hipFree(gf->y_nplus1_running_total_gfs));
-  HIPCHK(hipFree(gf->k_odd_gfs));
-  HIPCHK(hipFree(gf->k_even_gfs));
-  HIPCHK(hipFree(gf->auxevol_gfs));
-  gf->y_nplus1_running_total_gfs = gf->k_odd_gfs = gf->k_even_gfs = gf->auxevol_gfs = nullptr;
-  gf->diagnostic_output_gfs = nullptr;
+  ref_call("MoL_free_memory_non_y_n_gfs", [&]() {
+    (void)params;
+    HIPCHK(hipFree(gf->y_nplus1_running_total_gfs));
+    HIPCHK(hipFree(gf->k_odd_gfs));
+    HIPCHK(hipFree(gf->k_even_gfs));
+    HIPCHK(hipFree(gf->auxevol_gfs));
+    gf->y_nplus1_running_total_gfs = gf->k_odd_gfs = gf->k_even_gfs = gf->auxevol_gfs = nullptr;
+    gf->diagnostic_output_gfs = nullptr;
+  });
 }
 
 extern "C" void rfm_precompute_rfmstruct_malloc(const paramstruct *params, rfm_struct *r) {
-  const long G0 = params->Nxx_plus_2NGHOSTS0, G1 = params->Nxx_plus_2NGHOSTS1;
-  r->f0_of_xx0 = dalloc<double>(G0); r->f0_of_xx0__D0 = dalloc<double>(G0);
-  r->f0_of_xx0__DD00 = dalloc<double>(G0); r->f0_of_xx0__DDD000 = dalloc<double>(G0);
-  r->f1_of_xx1 = dalloc<double>(G1); r->f1_of_xx1__D1 = dalloc<double>(G1); r->f1_of_xx1__DD11 = dalloc<double>(G1);
-  r->f2_of_xx0_xx1 = dalloc<double>(G0 * G1); r->f2_of_xx0_xx1__D0 = dalloc<double>(G0 * G1);
-  r->f2_of_xx0_xx1__D1 = dalloc<double>(G0 * G1); r->f2_of_xx0_xx1__DD00 = dalloc<double>(G0 * G1);
-  r->f2_of_xx0_xx1__DD11 = dalloc<double>(G0 * G1);
-  r->f3_of_xx0 = dalloc<double>(G0); r->f3_of_xx0__D0 = dalloc<double>(G0); r->f3_of_xx0__DD00 = dalloc<double>(G0);
+  ref_call("rfm_precompute_rfmstruct_malloc", [&]() {
+    const long G0 = params->Nxx_plus_2NGHOSTS0, G1 = params->Nxx_plus_2NGHOSTS1;
+    r->f0_of_xx0 = dalloc<double>(G0); r->f0_of_xx0__D0 = dalloc<double>(G0);
+    r->f0_of_xx0__DD00 = dalloc<double>(G0); r->f0_of_xx0__DDD000 = dalloc<double>(G0);
+    r->f1_of_xx1 = dalloc<double>(G1); r->f1_of_xx1__D1 = dalloc<double>(G1); r->f1_of_xx1__DD11 = dalloc<double>(G1);
+    r->f2_of_xx0_xx1 = dalloc<double>(G0 * G1); r->f2_of_xx0_xx1__D0 = dalloc<double>(G0 * G1);
+    r->f2_of_xx0_xx1__D1 = dalloc<double>(G0 * G1); r->f2_of_xx0_xx1__DD00 = dalloc<double>(G0 * G1);
+    r->f2_of_xx0_xx1__DD11 = dalloc<double>(G0 * G1);
+    r->f3_of_xx0 = dalloc<double>(G0); r->f3_of_xx0__D0 = dalloc<double>(G0); r->f3_of_xx0__DD00 = dalloc<double>(G0);
+  });
 }
 extern "C" void rfm_precompute_rfmstruct_define(const paramstruct *params, REAL *xx[3], rfm_struct *r) {
-  const GridDims g = dims_from_params(params);
-  double *x0 = upload(xx[0], g.G0), *x1 = upload(xx[1], g.G1);
-  launch_rfm(g, params->AMAX, params->SINHWAA, params->bScale, x0, x1, *r, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  HIPCHK(hipFree(x0));
-  HIPCHK(hipFree(x1));
+  ref_call("rfm_precompute_rfmstruct_define", [&]() {
+    const GridDims g = dims_from_params(params);
+    double *x0 = upload(xx[0], g.G0), *x1 = upload(xx[1], g.G1);
+    launch_rfm(g, params->AMAX, params->SINHWAA, params->bScale, x0, x1, *r, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(x0));
+    HIPCHK(hipFree(x1));
+  });
 }
 extern "C" void rfm_precompute_rfmstruct_freemem(const paramstruct *params, rfm_struct *r) {
-  (void)params;
-  double **all[] = {&r->f0_of_xx0, &r->f0_of_xx0__D0, &r->f0_of_xx0__DD00, &r->f0_of_xx0__DDD000, &r->f1_of_xx1,
-                    &r->f1_of_xx1__D1, &r->f1_of_xx1__DD11, &r->f2_of_xx0_xx1, &r->f2_of_xx0_xx1__D0,
-                    &r->f2_of_xx0_xx1__D1, &r->f2_of_xx0_xx1__DD00, &r->f2_of_xx0_xx1__DD11, &r->f3_of_xx0,
-                    &r->f3_of_xx0__D0, &r->f3_of_xx0__DD00};
-  for (double **p : all) {
-    HIPCHK(hipFree(*p));
-    *p = nullptr;
-  }
+  ref_call("rfm_precompute_rfmstruct_freemem", [&]() {
+    (void)params;
+    double **all[] = {&r->f0_of_xx0, &r->f0_of_xx0__D0, &r->f0_of_xx0__DD00, &r->f0_of_xx0__DDD000, &r->f1_of_xx1,
+                      &r->f1_of_xx1__D1, &r->f1_of_xx1__DD11, &r->f2_of_xx0_xx1, &r->f2_of_xx0_xx1__D0,
+                      &r->f2_of_xx0_xx1__D1, &r->f2_of_xx0_xx1__DD00, &r->f2_of_xx0_xx1__DD11, &r->f3_of_xx0,
+                      &r->f3_of_xx0__D0, &r->f3_of_xx0__DD00};
+    for (double **p : all) {
+      HIPCHK(hipFree(*p));
+      *p = nullptr;
+    }
+  });
 }
 
 extern "C" void initial_guess_all_points(const paramstruct *params, REAL *xx[3], REAL *in_gfs) {
-  // initial_guess_single_point.c:14-15: u = v = 0 at every point, ghosts included
-  (void)xx;
-  const long nt = dims_from_params(params).ntot;
-  HIPCHK(hipMemset(in_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
-  HIPCHK(hipDeviceSynchronize());
+  ref_call("initial_guess_all_points", [&]() {
+    // initial_guess_single_point.c:14-15: u = v = 0 at every point, ghosts included
+    (void)xx;
+    const long nt = dims_from_params(params).ntot;
+    HIPCHK(hipMemset(in_gfs, 0, NUM_EVOL_GFS * nt * sizeof(double)));
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 static Punctures punctures_of(const paramstruct *p) {
@@ -173,36 +189,42 @@ static Punctures punctures_of(const paramstruct *p) {
 }
 
 extern "C" void auxevol_gfs_all_points(const paramstruct *params, REAL *xx[3], REAL *auxevol_gfs) {
-  const GridDims g = dims_from_params(params);
-  double *x0 = upload(xx[0], g.G0), *x1 = upload(xx[1], g.G1), *x2 = upload(xx[2], g.G2);
-  launch_auxevol(g, params->AMAX, params->SINHWAA, params->bScale, punctures_of(params), x0, x1, x2, auxevol_gfs,
-                 default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  HIPCHK(hipFree(x0)); HIPCHK(hipFree(x1)); HIPCHK(hipFree(x2));
+  ref_call("auxevol_gfs_all_points", [&]() {
+    const GridDims g = dims_from_params(params);
+    double *x0 = upload(xx[0], g.G0), *x1 = upload(xx[1], g.G1), *x2 = upload(xx[2], g.G2);
+    launch_auxevol(g, params->AMAX, params->SINHWAA, params->bScale, punctures_of(params), x0, x1, x2, auxevol_gfs,
+                   default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(x0)); HIPCHK(hipFree(x1)); HIPCHK(hipFree(x2));
+  });
 }
 
 extern "C" void wavespeed_gf_all_points(const paramstruct *params, const REAL CFL_FACTOR, const REAL dt, REAL *xx[3],
                                         REAL *auxevol_gfs) {
-  const GridDims g = dims_from_params(params);
-  std::vector<double> ds((size_t)g.G0 * g.G1, 0.0);
-  host_min_ds_2d(params, xx, ds.data());
-  // c = CFL*dsmin/dt (wavespeed_gf_all_points.c:40-41), rounded on the host per (i0,i1)
-  for (double &v : ds) v = CFL_FACTOR * v / dt;
-  double *d = upload(ds.data(), ds.size());
-  launch_wavespeed(g, d, 1.0, dt, auxevol_gfs, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  HIPCHK(hipFree(d));
+  ref_call("wavespeed_gf_all_points", [&]() {
+    const GridDims g = dims_from_params(params);
+    std::vector<double> ds((size_t)g.G0 * g.G1, 0.0);
+    host_min_ds_2d(params, xx, ds.data());
+    // c = CFL*dsmin/dt (wavespeed_gf_all_points.c:40-41), rounded on the host per (i0,i1)
+    for (double &v : ds) v = CFL_FACTOR * v / dt;
+    double *d = upload(ds.data(), ds.size());
+    launch_wavespeed(g, d, 1.0, dt, auxevol_gfs, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipFree(d));
+  });
 }
 
 extern "C" REAL compute_wavespeed_at_OB(const paramstruct *params, REAL *auxevol_gfs) {
-  // compute_wavespeed_at_OB.c:10
-  const GridDims g = dims_from_params(params);
-  double c = 0.0;
-  const long p = idx3(g, g.G0 - NRPYELL_NG - 1, NRPYELL_NG, g.G2 / 2);
-  HIPCHK(hipMemcpy(&c, auxevol_gfs + WAVESPEEDGF * g.ntot + p, sizeof(double), hipMemcpyDeviceToHost));
-  return c;
+  return ref_call("compute_wavespeed_at_OB", [&]() -> REAL {
+    // compute_wavespeed_at_OB.c:10
+    const GridDims g = dims_from_params(params);
+    double c = 0.0;
+    const long p = idx3(g, g.G0 - NRPYELL_NG - 1, NRPYELL_NG, g.G2 / 2);
+    HIPCHK(hipMemcpy(&c, auxevol_gfs + WAVESPEEDGF * g.ntot + p, sizeof(double), hipMemcpyDeviceToHost));
+    return c;
+  });
 }
 
 extern "C" int nrpyell_bcstruct_set_up_checked(const paramstruct *params, REAL *xx[3], bc_struct *bcstruct) {
@@ -374,76 +396,86 @@ static void ensure_radiation_lists(DeviceBc &d, const paramstruct *params, const
 // ---------------------------------------------------------------------------------------
 extern "C" void rhs_eval(const paramstruct *params, const rfm_struct *rfmstruct, const REAL *auxevol_gfs,
                          const REAL *in_gfs, REAL *rhs_gfs) {
-  const GridDims g = dims_from_params(params);
-  launch_rhs_exact(g, rfmstruct, auxevol_gfs, in_gfs, rhs_gfs, params->eta_damping, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
+  ref_call("rhs_eval", [&]() {
+    const GridDims g = dims_from_params(params);
+    launch_rhs_exact(g, rfmstruct, auxevol_gfs, in_gfs, rhs_gfs, params->eta_damping, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 extern "C" void residual_all_points(const paramstruct *params, const rfm_struct *rfmstruct, const REAL *auxevol_gfs,
                                     const REAL *in_gfs, REAL *residual_gf) {
-  const GridDims g = dims_from_params(params);
-  launch_residual_exact(g, rfmstruct, auxevol_gfs, in_gfs, residual_gf, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
+  ref_call("residual_all_points", [&]() {
+    const GridDims g = dims_from_params(params);
+    launch_residual_exact(g, rfmstruct, auxevol_gfs, in_gfs, residual_gf, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 extern "C" void apply_bcs_inner_only(const paramstruct *params, const bc_struct *bcstruct, REAL *gfs) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  DeviceBc &d = device_bc(params, bcstruct);
-  launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, gfs, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
+  ref_call("apply_bcs_inner_only", [&]() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceBc &d = device_bc(params, bcstruct);
+    launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, gfs, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 extern "C" void apply_bcs_outerradiation_and_inner(const paramstruct *params, const bc_struct *bcstruct, REAL *xx[3],
                                                    const REAL custom_wavespeed[NUM_EVOL_GFS],
                                                    const REAL custom_f_infinity[NUM_EVOL_GFS], REAL *gfs,
                                                    REAL *rhs_gfs) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  const GridDims g = dims_from_params(params);
-  DeviceBc &d = device_bc(params, bcstruct);
-  ensure_radiation_lists(d, params, bcstruct, xx);
-  // innermost layer first; within a layer x0, x1, x2 faces (apply_bcs_outerradiation_and_inner.c:431-458)
-  if (d.rows) {
-    RadLayers L;
-    for (int gz = 0; gz < NGHOSTS; gz++) L.l[gz] = d.rad[3 * gz];
-    launch_radiation_rows(g, d.nrad[0], L, gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
-                          custom_f_infinity[0], custom_f_infinity[1], default_stream());
-  } else {
-    for (int s = 0; s < NGHOSTS * 3; s++)
-      launch_radiation_list(g, d.nrad[s], d.rad[s], gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
+  ref_call("apply_bcs_outerradiation_and_inner", [&]() {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const GridDims g = dims_from_params(params);
+    DeviceBc &d = device_bc(params, bcstruct);
+    ensure_radiation_lists(d, params, bcstruct, xx);
+    // innermost layer first; within a layer x0, x1, x2 faces (apply_bcs_outerradiation_and_inner.c:431-458)
+    if (d.rows) {
+      RadLayers L;
+      for (int gz = 0; gz < NGHOSTS; gz++) L.l[gz] = d.rad[3 * gz];
+      launch_radiation_rows(g, d.nrad[0], L, gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
                             custom_f_infinity[0], custom_f_infinity[1], default_stream());
-  }
-  launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
+    } else {
+      for (int s = 0; s < NGHOSTS * 3; s++)
+        launch_radiation_list(g, d.nrad[s], d.rad[s], gfs, rhs_gfs, custom_wavespeed[0], custom_wavespeed[1],
+                              custom_f_infinity[0], custom_f_infinity[1], default_stream());
+    }
+    launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+  });
 }
 
 extern "C" int nrpyell_apply_bcs_sommerfeld(const paramstruct *params, const bc_struct *bcstruct, REAL *xx[3],
                                             const REAL wavespeed[NUM_EVOL_GFS], const REAL f_infinity[NUM_EVOL_GFS],
                                             const REAL radial_falloff_power[NUM_EVOL_GFS], REAL *gfs,
                                             REAL *rhs_gfs) {
-  if (!params || !bcstruct || !xx || !wavespeed || !f_infinity || !radial_falloff_power || !gfs || !rhs_gfs) {
-    set_last_error("nrpyell_apply_bcs_sommerfeld: null argument");
-    return NRPYELL_EINVAL;
-  }
-  std::lock_guard<std::mutex> lk(g_mu);
-  const GridDims g = dims_from_params(params);
-  DeviceBc &d = device_bc(params, bcstruct);
-  ensure_radiation_lists(d, params, bcstruct, xx);
-  // innermost layer first: each layer reads the RHS its inner neighbour layer just received
-  SommerfeldParams sp;
-  for (int k = 0; k < 2; k++) {
-    sp.c[k] = wavespeed[k];
-    sp.finf[k] = f_infinity[k];
-    sp.falloff[k] = radial_falloff_power[k];
-  }
-  for (int s = 0; s < NGHOSTS * 3; s++) launch_sommerfeld_list(g, d.nrad[s], d.som[s], gfs, rhs_gfs, sp, default_stream());
-  launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (!params || !bcstruct || !xx || !wavespeed || !f_infinity || !radial_falloff_power || !gfs || !rhs_gfs) {
+      set_last_error("nrpyell_apply_bcs_sommerfeld: null argument");
+      return NRPYELL_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    const GridDims g = dims_from_params(params);
+    DeviceBc &d = device_bc(params, bcstruct);
+    ensure_radiation_lists(d, params, bcstruct, xx);
+    // innermost layer first: each layer reads the RHS its inner neighbour layer just received
+    SommerfeldParams sp;
+    for (int k = 0; k < 2; k++) {
+      sp.c[k] = wavespeed[k];
+      sp.finf[k] = f_infinity[k];
+      sp.falloff[k] = radial_falloff_power[k];
+    }
+    for (int s = 0; s < NGHOSTS * 3; s++) launch_sommerfeld_list(g, d.nrad[s], d.som[s], gfs, rhs_gfs, sp, default_stream());
+    launch_inner_gather(d.ntot, d.num_inner, d.dst, d.src, d.par0, rhs_gfs, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return NRPYELL_OK;
+  });
 }
 
 // L2 weight tables cached per (grid, radius, coordinates).
@@ -485,64 +517,70 @@ static L2Weights &l2_weights(const paramstruct *p, REAL *const xx[3], double rad
 
 extern "C" REAL L2_norm_of_gf(const paramstruct *params, const int gf_index, const REAL integration_radius,
                               REAL *xx[3], const REAL *in_gf) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  const GridDims g = dims_from_params(params);
-  L2Weights &w = l2_weights(params, xx, integration_radius);
-  const long npart = launch_l2_partial(g, in_gf + (long)gf_index * g.ntot, w.w2d, w.partial, default_stream());
-  launch_l2_finalize(w.partial, npart, w.volume, nullptr, 0, 0.0, w.result, default_stream());
-  HIPCHK(hipGetLastError());
-  double out = 0.0;
-  HIPCHK(hipMemcpy(&out, w.result, sizeof(double), hipMemcpyDeviceToHost));
-  return out;
+  return ref_call("L2_norm_of_gf", [&]() -> REAL {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const GridDims g = dims_from_params(params);
+    L2Weights &w = l2_weights(params, xx, integration_radius);
+    const long npart = launch_l2_partial(g, in_gf + (long)gf_index * g.ntot, w.w2d, w.partial, default_stream());
+    launch_l2_finalize(w.partial, npart, w.volume, nullptr, 0, 0.0, w.result, default_stream());
+    HIPCHK(hipGetLastError());
+    double out = 0.0;
+    HIPCHK(hipMemcpy(&out, w.result, sizeof(double), hipMemcpyDeviceToHost));
+    return out;
+  });
 }
 
 // gridfunction_z_axis.c:6-52: z and gf along the +z axis (i1 = NG), -z axis (i1 = G1-NG-1)
 // and the i0 = NG line, at i2 = G2/2, "%.17e %.17e" per line.
 extern "C" void gridfunction_z_axis(const paramstruct *params, const int gf_index, REAL *xx[3], const REAL *in_gf,
                                     FILE *outfile) {
-  const GridDims g = dims_from_params(params);
-  std::vector<double> h(g.ntot);
-  HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
-  const double A = params->AMAX, b = params->bScale;
-  // z = sqrt(AMAX^2 sinh^2 / sinh^2 + b^2) cos x1, evaluated in the reference's operation order
-  // (its GCC build fuses the "+ b^2" into one fma), so the text output is identical
-  const double iw = 1.0 / params->SINHWAA;
-  const double D = exp(iw) - exp(-iw);
-  auto zc = [&](double x0, double x1) {
-    const double s = exp(iw * x0) - exp(-iw * x0);
-    return sqrt(fma(b, b, (A * A) * (s * s) / (D * D))) * cos(x1);
-  };
-  const int i2 = g.G2 / 2;
-  for (int pass = 0; pass < 2; pass++) {
-    const int i1 = pass == 0 ? NRPYELL_NG : g.G1 - NRPYELL_NG - 1;
-    for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++)
-      fprintf(outfile, "%.17e %.17e\n", zc(xx[0][i0], xx[1][i1]), h[idx3(g, i0, i1, i2)]);
-  }
-  for (int i1 = NRPYELL_NG + 1; i1 < g.G1 - NRPYELL_NG - 1; i1++)
-    fprintf(outfile, "%.17e %.17e\n", zc(xx[0][NRPYELL_NG], xx[1][i1]), h[idx3(g, NRPYELL_NG, i1, i2)]);
+  ref_call("gridfunction_z_axis", [&]() {
+    const GridDims g = dims_from_params(params);
+    std::vector<double> h(g.ntot);
+    HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
+    const double A = params->AMAX, b = params->bScale;
+    // z = sqrt(AMAX^2 sinh^2 / sinh^2 + b^2) cos x1, evaluated in the reference's operation order
+    // (its GCC build fuses the "+ b^2" into one fma), so the text output is identical
+    const double iw = 1.0 / params->SINHWAA;
+    const double D = exp(iw) - exp(-iw);
+    auto zc = [&](double x0, double x1) {
+      const double s = exp(iw * x0) - exp(-iw * x0);
+      return sqrt(fma(b, b, (A * A) * (s * s) / (D * D))) * cos(x1);
+    };
+    const int i2 = g.G2 / 2;
+    for (int pass = 0; pass < 2; pass++) {
+      const int i1 = pass == 0 ? NRPYELL_NG : g.G1 - NRPYELL_NG - 1;
+      for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++)
+        fprintf(outfile, "%.17e %.17e\n", zc(xx[0][i0], xx[1][i1]), h[idx3(g, i0, i1, i2)]);
+    }
+    for (int i1 = NRPYELL_NG + 1; i1 < g.G1 - NRPYELL_NG - 1; i1++)
+      fprintf(outfile, "%.17e %.17e\n", zc(xx[0][NRPYELL_NG], xx[1][i1]), h[idx3(g, NRPYELL_NG, i1, i2)]);
+  });
 }
 
 // gridfunction_xz_plane.c:6-49: x, z, gf on the xz plane: the x < 0 half at i2 = NGHOSTS,
 // then the x > 0 half at i2 = G2/2, "%.17e %.17e %.17e" per line.
 extern "C" void gridfunction_xz_plane(const paramstruct *params, const int gf_index, REAL *xx[3], const REAL *in_gf,
                                       FILE *outfile) {
-  const GridDims g = dims_from_params(params);
-  std::vector<double> h(g.ntot);
-  HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
-  const double A = params->AMAX, b = params->bScale;
-  const double iw = 1.0 / params->SINHWAA;
-  const double D = exp(iw) - exp(-iw);
-  const int i2s[2] = {NRPYELL_NG, g.G2 / 2};
-  for (int k = 0; k < 2; k++) {
-    const int i2 = i2s[k];
-    for (int i1 = NRPYELL_NG; i1 < g.G1 - NRPYELL_NG; i1++)
-      for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++) {
-        const double s = exp(iw * xx[0][i0]) - exp(-iw * xx[0][i0]);
-        const double x = A * s * sin(xx[1][i1]) * cos(xx[2][i2]) / D;
-        const double z = sqrt(fma(b, b, (A * A) * (s * s) / (D * D))) * cos(xx[1][i1]);
-        fprintf(outfile, "%.17e %.17e %.17e\n", x, z, h[idx3(g, i0, i1, i2)]);
-      }
-  }
+  ref_call("gridfunction_xz_plane", [&]() {
+    const GridDims g = dims_from_params(params);
+    std::vector<double> h(g.ntot);
+    HIPCHK(hipMemcpy(h.data(), in_gf + (long)gf_index * g.ntot, g.ntot * sizeof(double), hipMemcpyDeviceToHost));
+    const double A = params->AMAX, b = params->bScale;
+    const double iw = 1.0 / params->SINHWAA;
+    const double D = exp(iw) - exp(-iw);
+    const int i2s[2] = {NRPYELL_NG, g.G2 / 2};
+    for (int k = 0; k < 2; k++) {
+      const int i2 = i2s[k];
+      for (int i1 = NRPYELL_NG; i1 < g.G1 - NRPYELL_NG; i1++)
+        for (int i0 = NRPYELL_NG; i0 < g.G0 - NRPYELL_NG; i0++) {
+          const double s = exp(iw * xx[0][i0]) - exp(-iw * xx[0][i0]);
+          const double x = A * s * sin(xx[1][i1]) * cos(xx[2][i2]) / D;
+          const double z = sqrt(fma(b, b, (A * A) * (s * s) / (D * D))) * cos(xx[1][i1]);
+          fprintf(outfile, "%.17e %.17e %.17e\n", x, z, h[idx3(g, i0, i1, i2)]);
+        }
+    }
+  });
 }
 
 // print_puncture_parameters.c:6-37: the same report on stdout.
@@ -701,104 +739,108 @@ extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
 
 extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell_solve_options *opt,
                                      nrpyell_solver **out) {
-  std::lock_guard<std::mutex> lk(g_mu);
-  auto s = std::make_unique<nrpyell_solver>();
-  s->gd = gd;
-  s->g = dims_from_params(&gd->params);
-  s->dt = dt;
-  if (opt) s->opt = *opt; else nrpyell_default_options(&s->opt);
-  if (s->opt.check_every < 1) s->opt.check_every = 1;
-  if (s->opt.steps_per_graph < s->opt.check_every) s->opt.steps_per_graph = s->opt.check_every;
-  s->opt.steps_per_graph -= s->opt.steps_per_graph % s->opt.check_every;
-  const GridDims &g = s->g;
-  if (g.N0 < 2 * NRPYELL_NG || g.N1 < 2 * NRPYELL_NG || g.N2 < 2 || (g.N2 & 1)) {
-    set_last_error("grid too small for the fused solver (need Nxx0,Nxx1 >= 10 and even Nxx2)");
-    return NRPYELL_EINVAL;
-  }
-  if ((unsigned long)3 * g.ntot * sizeof(double) >= (1ul << 32)) {
-    // the fused kernels address each array with 32-bit byte offsets
-    set_last_error("grid too large for the fused solver: each array must stay under 4 GiB");
-    return NRPYELL_EINVAL;
-  }
-  std::string err;
-  if (ghostmap_verify(g, &gd->bcstruct, &err) != NRPYELL_OK) {
-    set_last_error(err);
-    return NRPYELL_EGHOSTMAP;
-  }
-  if (s->opt.outer_bc != NRPYELL_OUTER_BC_RADIATION && s->opt.outer_bc != NRPYELL_OUTER_BC_SOMMERFELD6) {
-    set_last_error("outer_bc must be NRPYELL_OUTER_BC_RADIATION or NRPYELL_OUTER_BC_SOMMERFELD6");
-    return NRPYELL_EINVAL;
-  }
-  int tx = s->opt.tile_i0 > 0 ? s->opt.tile_i0 : 32;
-  if (tx != 16 && tx != 32 && tx != 64) {
-    set_last_error("tile_i0 must be 16, 32 or 64");
-    return NRPYELL_EINVAL;
-  }
-  const int pl = s->opt.planes > 0 ? s->opt.planes : (tx >= 32 ? 2 : 1);
-  if (pl != 1 && !(pl == 2 && tx >= 32)) {
-    set_last_error("planes must be 1, or 2 with tile_i0 32 or 64");
-    return NRPYELL_EINVAL;
-  }
-  s->tx = tx;
-  s->pl = pl;
-  const int ty = NRPYELL_BLOCK / tx;
-  s->ntile0 = (g.N0 + tx - 1) / tx;
-  s->ntile1 = (g.N1 + ty - 1) / ty;
-  {
-    // the kernels split block indices with div_by: exact while index x divisor <= 2^32
-    const double lim = 4294967296.0, nbc = (double)((g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS);
-    const double nsub = (double)s->ntile0 * s->ntile1 * (g.N2 / pl);
-    if (nsub * (g.N2 / pl) > lim || (double)s->ntile0 * s->ntile1 * s->ntile0 > lim || nbc * nbc * g.N2 > lim) {
-      set_last_error("grid too large for the fused solver's block indexing");
+  return api_call([&]() -> int {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto s = std::make_unique<nrpyell_solver>();
+    s->gd = gd;
+    s->g = dims_from_params(&gd->params);
+    s->dt = dt;
+    if (opt) s->opt = *opt; else nrpyell_default_options(&s->opt);
+    if (s->opt.check_every < 1) s->opt.check_every = 1;
+    if (s->opt.steps_per_graph < s->opt.check_every) s->opt.steps_per_graph = s->opt.check_every;
+    s->opt.steps_per_graph -= s->opt.steps_per_graph % s->opt.check_every;
+    const GridDims &g = s->g;
+    if (g.N0 < 2 * NRPYELL_NG || g.N1 < 2 * NRPYELL_NG || g.N2 < 2 || (g.N2 & 1)) {
+      set_last_error("grid too small for the fused solver (need Nxx0,Nxx1 >= 10 and even Nxx2)");
       return NRPYELL_EINVAL;
     }
-  }
-  HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-  const long n2 = (long)g.G0 * g.G1;
-  s->coef64 = dalloc<double>(5 * n2);
-  launch_coef(g, &gd->rfmstruct, s->coef64, nullptr, s->stream);
-  std::vector<double> geo((size_t)6 * g.G1 * 4);
-  host_outer_geometry(&gd->params, gd->xx, geo.data());
-  s->geo64 = upload(geo.data(), geo.size());
-  std::vector<float> geof(geo.begin(), geo.end());
-  s->geo32 = upload(geof.data(), geof.size());
-  std::vector<double> w(n2);
-  host_l2_weights(&gd->params, gd->xx, s->opt.integration_radius, w.data(), &s->volume);
-  s->w2d = upload(w.data(), w.size());
-  // tables with the wave speed folded in (auxevol and rfm are read here, once per solver)
-  s->kc64 = dalloc<double>(6 * n2);
-  s->kc32 = dalloc<float>(6 * n2);
-  s->src64 = dalloc<double>(2 * g.ntot);
-  int *bad = dalloc<int>(1);
-  HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
-  launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->src64, bad, s->stream);
-  int hbad = 0;
-  HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  (void)hipFree(bad);
-  if (hbad) {
-    set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
-                   "wavespeed_gf_all_points sets it");
-    return NRPYELL_EINVAL;
-  }
-  const long nblk_res = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
-  const long nblk_sub = (long)s->ntile0 * s->ntile1 * (g.N2 / pl);
-  s->partial = dalloc<double>(nblk_res > nblk_sub ? nblk_res : nblk_sub);
-  s->l2_out = dalloc<double>(1);
-  s->dstate = dalloc<SolverDeviceState>(1);
-  HIPCHK(hipHostMalloc((void **)&s->hstate, sizeof(SolverDeviceState), hipHostMallocDefault));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  HIPCHK(hipGetLastError());
-  reset_state(s.get());
-  *out = s.release();
-  return NRPYELL_OK;
+    if ((unsigned long)3 * g.ntot * sizeof(double) >= (1ul << 32)) {
+      // the fused kernels address each array with 32-bit byte offsets
+      set_last_error("grid too large for the fused solver: each array must stay under 4 GiB");
+      return NRPYELL_EINVAL;
+    }
+    std::string err;
+    if (ghostmap_verify(g, &gd->bcstruct, &err) != NRPYELL_OK) {
+      set_last_error(err);
+      return NRPYELL_EGHOSTMAP;
+    }
+    if (s->opt.outer_bc != NRPYELL_OUTER_BC_RADIATION && s->opt.outer_bc != NRPYELL_OUTER_BC_SOMMERFELD6) {
+      set_last_error("outer_bc must be NRPYELL_OUTER_BC_RADIATION or NRPYELL_OUTER_BC_SOMMERFELD6");
+      return NRPYELL_EINVAL;
+    }
+    int tx = s->opt.tile_i0 > 0 ? s->opt.tile_i0 : 32;
+    if (tx != 16 && tx != 32 && tx != 64) {
+      set_last_error("tile_i0 must be 16, 32 or 64");
+      return NRPYELL_EINVAL;
+    }
+    const int pl = s->opt.planes > 0 ? s->opt.planes : (tx >= 32 ? 2 : 1);
+    if (pl != 1 && !(pl == 2 && tx >= 32)) {
+      set_last_error("planes must be 1, or 2 with tile_i0 32 or 64");
+      return NRPYELL_EINVAL;
+    }
+    s->tx = tx;
+    s->pl = pl;
+    const int ty = NRPYELL_BLOCK / tx;
+    s->ntile0 = (g.N0 + tx - 1) / tx;
+    s->ntile1 = (g.N1 + ty - 1) / ty;
+    {
+      // the kernels split block indices with div_by: exact while index x divisor <= 2^32
+      const double lim = 4294967296.0, nbc = (double)((g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS);
+      const double nsub = (double)s->ntile0 * s->ntile1 * (g.N2 / pl);
+      if (nsub * (g.N2 / pl) > lim || (double)s->ntile0 * s->ntile1 * s->ntile0 > lim || nbc * nbc * g.N2 > lim) {
+        set_last_error("grid too large for the fused solver's block indexing");
+        return NRPYELL_EINVAL;
+      }
+    }
+    HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    const long n2 = (long)g.G0 * g.G1;
+    s->coef64 = dalloc<double>(5 * n2);
+    launch_coef(g, &gd->rfmstruct, s->coef64, nullptr, s->stream);
+    std::vector<double> geo((size_t)6 * g.G1 * 4);
+    host_outer_geometry(&gd->params, gd->xx, geo.data());
+    s->geo64 = upload(geo.data(), geo.size());
+    std::vector<float> geof(geo.begin(), geo.end());
+    s->geo32 = upload(geof.data(), geof.size());
+    std::vector<double> w(n2);
+    host_l2_weights(&gd->params, gd->xx, s->opt.integration_radius, w.data(), &s->volume);
+    s->w2d = upload(w.data(), w.size());
+    // tables with the wave speed folded in (auxevol and rfm are read here, once per solver)
+    s->kc64 = dalloc<double>(6 * n2);
+    s->kc32 = dalloc<float>(6 * n2);
+    s->src64 = dalloc<double>(2 * g.ntot);
+    int *bad = dalloc<int>(1);
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
+    launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->src64, bad, s->stream);
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    (void)hipFree(bad);
+    if (hbad) {
+      set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
+                     "wavespeed_gf_all_points sets it");
+      return NRPYELL_EINVAL;
+    }
+    const long nblk_res = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
+    const long nblk_sub = (long)s->ntile0 * s->ntile1 * (g.N2 / pl);
+    s->partial = dalloc<double>(nblk_res > nblk_sub ? nblk_res : nblk_sub);
+    s->l2_out = dalloc<double>(1);
+    s->dstate = dalloc<SolverDeviceState>(1);
+    HIPCHK(hipHostMalloc((void **)&s->hstate, sizeof(SolverDeviceState), hipHostMallocDefault));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    HIPCHK(hipGetLastError());
+    reset_state(s.get());
+    *out = s.release();
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_solver_destroy(nrpyell_solver *s) {
-  if (!s) return NRPYELL_OK;
-  (void)hipStreamSynchronize(s->stream);
-  delete s;
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (!s) return NRPYELL_OK;
+    (void)hipStreamSynchronize(s->stream);
+    delete s;
+    return NRPYELL_OK;
+  });
 }
 
 static void ensure_fp32(nrpyell_solver *s) {
@@ -833,199 +875,211 @@ static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double t
 }
 
 extern "C" int nrpyell_solver_steps(nrpyell_solver *s, int n) {
-  reset_state(s);
-  for (int k = 0; k < n; k++) s->step64(s->stream);
-  s->fill_ghosts(s->stream);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s->stream));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    reset_state(s);
+    for (int k = 0; k < n; k++) s->step64(s->stream);
+    s->fill_ghosts(s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_solver_residual_l2(nrpyell_solver *s, double *log10_l2) {
-  const long npart = launch_residual_l2(s->g, s->gd->gridfuncs.y_n_gfs, false, s->gd->gridfuncs.auxevol_gfs,
-                                        s->coef64, s->w2d, s->partial, nullptr, nullptr, s->stream);
-  launch_l2_finalize(s->partial, npart, s->volume, nullptr, 0, 0.0, s->l2_out, s->stream);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(log10_l2, s->l2_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPCHK(hipStreamSynchronize(s->stream));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    const long npart = launch_residual_l2(s->g, s->gd->gridfuncs.y_n_gfs, false, s->gd->gridfuncs.auxevol_gfs,
+                                          s->coef64, s->w2d, s->partial, nullptr, nullptr, s->stream);
+    launch_l2_finalize(s->partial, npart, s->volume, nullptr, 0, 0.0, s->l2_out, s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(log10_l2, s->l2_out, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res) {
-  memset(res, 0, sizeof(*res));
-  const auto t0 = std::chrono::steady_clock::now();
-  const long nt = s->g.ntot;
-  MoL_gridfunctions_struct &f = s->gd->gridfuncs;
-  reset_state(s);
-  // residual of the initial guess (counts no steps)
-  s->check(s->stream, false, 0, s->opt.log10_target);
-  read_state(s);
-  res->log10_l2_initial = s->hstate->last_log10;
-  int steps = 0;
-  if (s->hstate->done == 0 && s->opt.precision == 32) {
-    ensure_fp32(s);
-    launch_convert_d2f(2 * nt, f.y_n_gfs, s->y32, s->stream);
-    launch_convert_d2f(2 * nt, s->src64, s->src32, s->stream);
-    if (!s->graph32 || s->graph32_target != s->opt.fp32_switch_log10) {
-      if (s->graph32) (void)hipGraphExecDestroy(s->graph32);
-      s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
-      s->graph32_target = s->opt.fp32_switch_log10;
-    }
+  return api_call([&]() -> int {
+    memset(res, 0, sizeof(*res));
+    const auto t0 = std::chrono::steady_clock::now();
+    const long nt = s->g.ntot;
+    MoL_gridfunctions_struct &f = s->gd->gridfuncs;
     reset_state(s);
-    // The relaxation residual oscillates, so the FP32 phase ends on the lowest value seen so
-    // far: at the switch target, after fp32_plateau_steps steps without a new lowest value
-    // (FP32 rounding floor), or when it climbs 2 decades above the lowest (diverging).
-    double best = res->log10_l2_initial;
-    int since_best = 0, seen = 0;
-    while (true) {
-      HIPCHK(hipGraphLaunch(s->graph32, s->stream));
-      read_state(s);
-      const SolverDeviceState &h = *s->hstate;
-      if (h.done || h.steps >= s->opt.max_steps) break;
-      if (h.nhist > seen) {
-        for (int i = seen; i < h.nhist; i++) {
-          if (h.hist_log10[i] < best - 0.01) {
-            best = h.hist_log10[i];
-            since_best = 0;
-          } else {
-            since_best += s->opt.check_every;
-          }
-        }
-        seen = h.nhist;
-      } else if (h.last_log10 < best - 0.01) {  // history full: the graph's last check
-        best = h.last_log10;
-        since_best = 0;
-      } else {
-        since_best += s->opt.steps_per_graph;
-      }
-      if (since_best >= s->opt.fp32_plateau_steps) break;
-      if (h.last_log10 > best + 2.0) break;
-    }
-    steps = s->hstate->steps;
-    res->fp32_steps = steps;
-    launch_convert_f2d(2 * nt, s->y32, f.y_n_gfs, s->stream);
-    reset_state(s);
+    // residual of the initial guess (counts no steps)
     s->check(s->stream, false, 0, s->opt.log10_target);
     read_state(s);
-  }
-  if (s->hstate->done == 0) {
-    if (!s->graph64 || s->graph64_steps != s->opt.steps_per_graph) {
-      if (s->graph64) (void)hipGraphExecDestroy(s->graph64);
-      s->graph64 = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
-      s->graph64_steps = s->opt.steps_per_graph;
-    }
-    reset_state(s);
-    // every step's stage 1 checks the state it starts from: a converged solve took `steps`
-    // steps (the count when its check passed), an unconverged one all `nchecks` launched
-    while (true) {
-      HIPCHK(hipGraphLaunch(s->graph64, s->stream));
+    res->log10_l2_initial = s->hstate->last_log10;
+    int steps = 0;
+    if (s->hstate->done == 0 && s->opt.precision == 32) {
+      ensure_fp32(s);
+      launch_convert_d2f(2 * nt, f.y_n_gfs, s->y32, s->stream);
+      launch_convert_d2f(2 * nt, s->src64, s->src32, s->stream);
+      if (!s->graph32 || s->graph32_target != s->opt.fp32_switch_log10) {
+        if (s->graph32) (void)hipGraphExecDestroy(s->graph32);
+        s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
+        s->graph32_target = s->opt.fp32_switch_log10;
+      }
+      reset_state(s);
+      // The relaxation residual oscillates, so the FP32 phase ends on the lowest value seen so
+      // far: at the switch target, after fp32_plateau_steps steps without a new lowest value
+      // (FP32 rounding floor), or when it climbs 2 decades above the lowest (diverging).
+      double best = res->log10_l2_initial;
+      int since_best = 0, seen = 0;
+      while (true) {
+        HIPCHK(hipGraphLaunch(s->graph32, s->stream));
+        read_state(s);
+        const SolverDeviceState &h = *s->hstate;
+        if (h.done || h.steps >= s->opt.max_steps) break;
+        if (h.nhist > seen) {
+          for (int i = seen; i < h.nhist; i++) {
+            if (h.hist_log10[i] < best - 0.01) {
+              best = h.hist_log10[i];
+              since_best = 0;
+            } else {
+              since_best += s->opt.check_every;
+            }
+          }
+          seen = h.nhist;
+        } else if (h.last_log10 < best - 0.01) {  // history full: the graph's last check
+          best = h.last_log10;
+          since_best = 0;
+        } else {
+          since_best += s->opt.steps_per_graph;
+        }
+        if (since_best >= s->opt.fp32_plateau_steps) break;
+        if (h.last_log10 > best + 2.0) break;
+      }
+      steps = s->hstate->steps;
+      res->fp32_steps = steps;
+      launch_convert_f2d(2 * nt, s->y32, f.y_n_gfs, s->stream);
+      reset_state(s);
+      s->check(s->stream, false, 0, s->opt.log10_target);
       read_state(s);
-      if (s->hstate->done || steps + s->hstate->nchecks >= s->opt.max_steps) break;
     }
-    steps += s->hstate->done ? s->hstate->steps : s->hstate->nchecks;
-  }
-  s->fill_ghosts(s->stream);
-  HIPCHK(hipStreamSynchronize(s->stream));
-  const auto t1 = std::chrono::steady_clock::now();
-  res->steps = steps;
-  res->log10_l2 = s->hstate->last_log10;
-  res->seconds = std::chrono::duration<double>(t1 - t0).count();
-  res->converged = s->hstate->done == 1;
-  if (s->hstate->done == 2) { set_last_error("residual became NaN"); return NRPYELL_ENAN; }
-  if (s->hstate->done == 3) {
-    set_last_error("residual diverged (4 decades above its lowest value)");
-    return NRPYELL_ENOTCONVERGED;
-  }
-  if (!res->converged) { set_last_error("max_steps reached before the residual target"); return NRPYELL_ENOTCONVERGED; }
-  return NRPYELL_OK;
+    if (s->hstate->done == 0) {
+      if (!s->graph64 || s->graph64_steps != s->opt.steps_per_graph) {
+        if (s->graph64) (void)hipGraphExecDestroy(s->graph64);
+        s->graph64 = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
+        s->graph64_steps = s->opt.steps_per_graph;
+      }
+      reset_state(s);
+      // every step's stage 1 checks the state it starts from: a converged solve took `steps`
+      // steps (the count when its check passed), an unconverged one all `nchecks` launched
+      while (true) {
+        HIPCHK(hipGraphLaunch(s->graph64, s->stream));
+        read_state(s);
+        if (s->hstate->done || steps + s->hstate->nchecks >= s->opt.max_steps) break;
+      }
+      steps += s->hstate->done ? s->hstate->steps : s->hstate->nchecks;
+    }
+    s->fill_ghosts(s->stream);
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    res->steps = steps;
+    res->log10_l2 = s->hstate->last_log10;
+    res->seconds = std::chrono::duration<double>(t1 - t0).count();
+    res->converged = s->hstate->done == 1;
+    if (s->hstate->done == 2) { set_last_error("residual became NaN"); return NRPYELL_ENAN; }
+    if (s->hstate->done == 3) {
+      set_last_error("residual diverged (4 decades above its lowest value)");
+      return NRPYELL_ENOTCONVERGED;
+    }
+    if (!res->converged) { set_last_error("max_steps reached before the residual target"); return NRPYELL_ENOTCONVERGED; }
+    return NRPYELL_OK;
+  });
 }
 
 // Exactly n RK4 steps of the production FP64 path (graph replay, stage-1 residual check every
 // step against an unreachable target so nothing stops early), bracketed by HIP events on the
 // solver's stream. Returns the elapsed milliseconds and the final log10 L2.
 extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_out, double *log10_out) {
-  const double keep = s->opt.log10_target;
-  s->opt.log10_target = -1e300;
-  hipGraphExec_t g = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
-  s->opt.log10_target = keep;
-  const int per = s->opt.steps_per_graph;
-  hipGraphExec_t tail = nullptr;
-  if (n % per) {
+  return api_call([&]() -> int {
+    const double keep = s->opt.log10_target;
     s->opt.log10_target = -1e300;
-    tail = capture(s, false, n % per, s->opt.log10_target);
+    hipGraphExec_t g = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
     s->opt.log10_target = keep;
-  }
-  reset_state(s);
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, s->stream));
-  for (int k = 0; k < n / per; k++) HIPCHK(hipGraphLaunch(g, s->stream));
-  if (tail) HIPCHK(hipGraphLaunch(tail, s->stream));
-  HIPCHK(hipEventRecord(e1, s->stream));
-  HIPCHK(hipEventSynchronize(e1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  s->fill_ghosts(s->stream);
-  read_state(s);
-  *ms_out = ms;
-  if (log10_out) *log10_out = s->hstate->last_log10;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  HIPCHK(hipGraphExecDestroy(g));
-  if (tail) HIPCHK(hipGraphExecDestroy(tail));
-  return NRPYELL_OK;
+    const int per = s->opt.steps_per_graph;
+    hipGraphExec_t tail = nullptr;
+    if (n % per) {
+      s->opt.log10_target = -1e300;
+      tail = capture(s, false, n % per, s->opt.log10_target);
+      s->opt.log10_target = keep;
+    }
+    reset_state(s);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s->stream));
+    for (int k = 0; k < n / per; k++) HIPCHK(hipGraphLaunch(g, s->stream));
+    if (tail) HIPCHK(hipGraphLaunch(tail, s->stream));
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    s->fill_ghosts(s->stream);
+    read_state(s);
+    *ms_out = ms;
+    if (log10_out) *log10_out = s->hstate->last_log10;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGraphExecDestroy(g));
+    if (tail) HIPCHK(hipGraphExecDestroy(tail));
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2) {
-  read_state(s);
-  const int n = s->hstate->nhist < max ? s->hstate->nhist : max;
-  for (int k = 0; k < n; k++) {
-    steps[k] = s->hstate->hist_step[k];
-    log10_l2[k] = s->hstate->hist_log10[k];
-  }
-  return n;
+  return api_call([&]() -> int {
+    read_state(s);
+    const int n = s->hstate->nhist < max ? s->hstate->nhist : max;
+    for (int k = 0; k < n; k++) {
+      steps[k] = s->hstate->hist_step[k];
+      log10_l2[k] = s->hstate->hist_log10[k];
+    }
+    return n;
+  });
 }
 
 extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_per_launch) {
-  MoL_gridfunctions_struct &f = s->gd->gridfuncs;
-  reset_state(s);
-  // stage 0 = stage 1 with the fused residual-L2 check (the form the solver runs)
-  const bool fuse = stage == 0;
-  if (fuse) stage = 1;
-  // the kernels write y_n only in stage 4; time on scratch copies so the state is untouched
-  SubstepArgs<double> a =
-      s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, s->src64,
-                      s->kc64, s->geo64);
-  if (fuse) {
-    a.l2_partial = s->partial;
-    a.state = s->dstate;
-    a.l2_volume = s->volume;
-    a.l2_target = -1e300;  // never "converges": every launch does the full work
-  }
-  SubstepArgs<double> b = a;
-  double *tmp = nullptr;
-  if (stage == 4) {
-    tmp = dalloc<double>(2 * s->g.ntot);
-    HIPCHK(hipMemcpyAsync(tmp, f.y_n_gfs, 2 * s->g.ntot * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
-    b.out = tmp;
-    b.yn = tmp;
-  }
-  for (int k = 0; k < 3; k++) launch_substep<double>(b, stage, s->stream);
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, s->stream));
-  for (int k = 0; k < n; k++) launch_substep<double>(b, stage, s->stream);
-  HIPCHK(hipEventRecord(e1, s->stream));
-  HIPCHK(hipEventSynchronize(e1));
-  float ms = 0;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  *us_per_launch = 1000.0 * ms / n;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (tmp) (void)hipFree(tmp);
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    MoL_gridfunctions_struct &f = s->gd->gridfuncs;
+    reset_state(s);
+    // stage 0 = stage 1 with the fused residual-L2 check (the form the solver runs)
+    const bool fuse = stage == 0;
+    if (fuse) stage = 1;
+    // the kernels write y_n only in stage 4; time on scratch copies so the state is untouched
+    SubstepArgs<double> a =
+        s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs, f.y_nplus1_running_total_gfs, s->src64,
+                        s->kc64, s->geo64);
+    if (fuse) {
+      a.l2_partial = s->partial;
+      a.state = s->dstate;
+      a.l2_volume = s->volume;
+      a.l2_target = -1e300;  // never "converges": every launch does the full work
+    }
+    SubstepArgs<double> b = a;
+    double *tmp = nullptr;
+    if (stage == 4) {
+      tmp = dalloc<double>(2 * s->g.ntot);
+      HIPCHK(hipMemcpyAsync(tmp, f.y_n_gfs, 2 * s->g.ntot * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+      b.out = tmp;
+      b.yn = tmp;
+    }
+    for (int k = 0; k < 3; k++) launch_substep<double>(b, stage, s->stream);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, s->stream));
+    for (int k = 0; k < n; k++) launch_substep<double>(b, stage, s->stream);
+    HIPCHK(hipEventRecord(e1, s->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *us_per_launch = 1000.0 * ms / n;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (tmp) (void)hipFree(tmp);
+    return NRPYELL_OK;
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1040,31 +1094,33 @@ struct StepCacheEntry {
 static std::map<const griddata_struct *, StepCacheEntry> g_step_cache;
 
 extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL dt) {
-  nrpyell_solver *s = nullptr;
-  {
-    auto it = g_step_cache.find(griddata);
-    if (it != g_step_cache.end() && it->second.yn == griddata->gridfuncs.y_n_gfs &&
-        it->second.aux == griddata->gridfuncs.auxevol_gfs && it->second.rfm0 == griddata->rfmstruct.f0_of_xx0 &&
-        it->second.inner == griddata->bcstruct.inner_bc_array) {
-      s = it->second.s;
-    } else {
-      if (it != g_step_cache.end()) {
-        nrpyell_solver_destroy(it->second.s);
-        g_step_cache.erase(it);
+  ref_call("MoL_step_forward_in_time", [&]() {
+    nrpyell_solver *s = nullptr;
+    {
+      auto it = g_step_cache.find(griddata);
+      if (it != g_step_cache.end() && it->second.yn == griddata->gridfuncs.y_n_gfs &&
+          it->second.aux == griddata->gridfuncs.auxevol_gfs && it->second.rfm0 == griddata->rfmstruct.f0_of_xx0 &&
+          it->second.inner == griddata->bcstruct.inner_bc_array) {
+        s = it->second.s;
+      } else {
+        if (it != g_step_cache.end()) {
+          nrpyell_solver_destroy(it->second.s);
+          g_step_cache.erase(it);
+        }
+        nrpyell_solve_options o;
+        nrpyell_default_options(&o);
+        if (nrpyell_solver_create(griddata, dt, &o, &s) != NRPYELL_OK) fatal("MoL_step_forward_in_time", g_last_error);
+        g_step_cache[griddata] = {s, griddata->gridfuncs.y_n_gfs, griddata->gridfuncs.auxevol_gfs,
+                                  griddata->rfmstruct.f0_of_xx0, griddata->bcstruct.inner_bc_array};
       }
-      nrpyell_solve_options o;
-      nrpyell_default_options(&o);
-      if (nrpyell_solver_create(griddata, dt, &o, &s) != NRPYELL_OK) fatal("MoL_step_forward_in_time", g_last_error);
-      g_step_cache[griddata] = {s, griddata->gridfuncs.y_n_gfs, griddata->gridfuncs.auxevol_gfs,
-                                griddata->rfmstruct.f0_of_xx0, griddata->bcstruct.inner_bc_array};
     }
-  }
-  s->dt = dt;
-  HIPCHK(hipMemsetAsync(&s->dstate->done, 0, sizeof(int), s->stream));
-  s->step64(s->stream);
-  s->fill_ghosts(s->stream);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s->stream));
+    s->dt = dt;
+    HIPCHK(hipMemsetAsync(&s->dstate->done, 0, sizeof(int), s->stream));
+    s->step64(s->stream);
+    s->fill_ghosts(s->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->stream));
+  });
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1190,101 +1246,107 @@ struct nrpyell_batch {
 
 extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, const int *tile_i0, const int *planes,
                                     int n, const nrpyell_solve_options *opt, nrpyell_batch **out) {
-  if (n <= 0) { set_last_error("empty batch"); return NRPYELL_EINVAL; }
-  auto b = std::make_unique<nrpyell_batch>();
-  b->n = n;
-  if (opt) b->opt = *opt; else nrpyell_default_options(&b->opt);
-  b->opt.precision = 64;
-  for (int k = 0; k < n; k++) {
-    nrpyell_solve_options o = b->opt;
-    o.tile_i0 = tile_i0 ? tile_i0[k] : b->opt.tile_i0;
-    o.planes = planes ? planes[k] : b->opt.planes;
-    nrpyell_solver *s = nullptr;
-    const int rc = nrpyell_solver_create(gds[k], dts[k], &o, &s);
-    if (rc != NRPYELL_OK) return rc;
-    b->solvers.push_back(s);
-  }
-  HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-  std::map<std::pair<int, int>, std::vector<int>> by_tile;
-  for (int k = 0; k < n; k++) by_tile[{b->solvers[k]->tx, b->solvers[k]->pl}].push_back(k);
-  for (auto &kv : by_tile) {
-    nrpyell_batch::Group g;
-    g.tx = kv.first.first;
-    g.pl = kv.first.second;
-    g.members = kv.second;
-    const int m = (int)g.members.size();
-    for (int stage = 1; stage <= 4; stage++) {
-      std::vector<SubstepArgs<double>> args(m);
-      for (int j = 0; j < m; j++) {
-        nrpyell_solver *s = b->solvers[g.members[j]];
-        MoL_gridfunctions_struct &f = s->gd->gridfuncs;
-        SubstepArgs<double> a = s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs,
-                                                f.y_nplus1_running_total_gfs, s->src64, s->kc64, s->geo64);
-        if (stage <= 2) {
-          a.l2_partial = s->partial;
-          a.state = s->dstate;
-          a.l2_volume = s->volume;
-          a.l2_target = b->opt.log10_target;
-        }
-        args[j] = a;
-      }
-      g.d_args[stage] = upload(args.data(), args.size());
+  return api_call([&]() -> int {
+    if (n <= 0) { set_last_error("empty batch"); return NRPYELL_EINVAL; }
+    auto b = std::make_unique<nrpyell_batch>();
+    b->n = n;
+    if (opt) b->opt = *opt; else nrpyell_default_options(&b->opt);
+    b->opt.precision = 64;
+    for (int k = 0; k < n; k++) {
+      nrpyell_solve_options o = b->opt;
+      o.tile_i0 = tile_i0 ? tile_i0[k] : b->opt.tile_i0;
+      o.planes = planes ? planes[k] : b->opt.planes;
+      nrpyell_solver *s = nullptr;
+      const int rc = nrpyell_solver_create(gds[k], dts[k], &o, &s);
+      if (rc != NRPYELL_OK) return rc;
+      b->solvers.push_back(s);
     }
-    b->groups.push_back(g);
-  }
-  std::vector<SolverDeviceState *> st(n);
-  for (int k = 0; k < n; k++) st[k] = b->solvers[k]->dstate;
-  b->d_states = upload(st.data(), st.size());
-  b->d_heads = dalloc<double>(4 * n);
-  HIPCHK(hipHostMalloc((void **)&b->h_heads, sizeof(double) * 4 * n, hipHostMallocDefault));
-  b->rebuild(std::vector<char>(n, 1));
-  *out = b.release();
-  return NRPYELL_OK;
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    std::map<std::pair<int, int>, std::vector<int>> by_tile;
+    for (int k = 0; k < n; k++) by_tile[{b->solvers[k]->tx, b->solvers[k]->pl}].push_back(k);
+    for (auto &kv : by_tile) {
+      nrpyell_batch::Group g;
+      g.tx = kv.first.first;
+      g.pl = kv.first.second;
+      g.members = kv.second;
+      const int m = (int)g.members.size();
+      for (int stage = 1; stage <= 4; stage++) {
+        std::vector<SubstepArgs<double>> args(m);
+        for (int j = 0; j < m; j++) {
+          nrpyell_solver *s = b->solvers[g.members[j]];
+          MoL_gridfunctions_struct &f = s->gd->gridfuncs;
+          SubstepArgs<double> a = s->args<double>(stage, f.y_n_gfs, f.k_odd_gfs, f.k_even_gfs,
+                                                  f.y_nplus1_running_total_gfs, s->src64, s->kc64, s->geo64);
+          if (stage <= 2) {
+            a.l2_partial = s->partial;
+            a.state = s->dstate;
+            a.l2_volume = s->volume;
+            a.l2_target = b->opt.log10_target;
+          }
+          args[j] = a;
+        }
+        g.d_args[stage] = upload(args.data(), args.size());
+      }
+      b->groups.push_back(g);
+    }
+    std::vector<SolverDeviceState *> st(n);
+    for (int k = 0; k < n; k++) st[k] = b->solvers[k]->dstate;
+    b->d_states = upload(st.data(), st.size());
+    b->d_heads = dalloc<double>(4 * n);
+    HIPCHK(hipHostMalloc((void **)&b->h_heads, sizeof(double) * 4 * n, hipHostMallocDefault));
+    b->rebuild(std::vector<char>(n, 1));
+    *out = b.release();
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, double *seconds) {
-  const auto t0 = std::chrono::steady_clock::now();
-  for (auto *s : b->solvers) reset_state(s);
-  if (b->in_graph != b->n) b->rebuild(std::vector<char>(b->n, 1));
-  int launched = 0;
-  std::vector<char> active(b->n, 1);
-  while (true) {
-    HIPCHK(hipGraphLaunch(b->graph, b->stream));
-    launched += b->opt.steps_per_graph;
-    b->read_heads();
-    int n_active = 0;
-    for (int k = 0; k < b->n; k++) {
-      active[k] = b->h_heads[4 * k] == 0.0;
-      n_active += active[k];
+  return api_call([&]() -> int {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (auto *s : b->solvers) reset_state(s);
+    if (b->in_graph != b->n) b->rebuild(std::vector<char>(b->n, 1));
+    int launched = 0;
+    std::vector<char> active(b->n, 1);
+    while (true) {
+      HIPCHK(hipGraphLaunch(b->graph, b->stream));
+      launched += b->opt.steps_per_graph;
+      b->read_heads();
+      int n_active = 0;
+      for (int k = 0; k < b->n; k++) {
+        active[k] = b->h_heads[4 * k] == 0.0;
+        n_active += active[k];
+      }
+      if (n_active == 0 || launched >= b->opt.max_steps) break;
+      // compact when a quarter of the launched configurations have finished
+      if (4 * n_active <= 3 * b->in_graph) b->rebuild(active);
     }
-    if (n_active == 0 || launched >= b->opt.max_steps) break;
-    // compact when a quarter of the launched configurations have finished
-    if (4 * n_active <= 3 * b->in_graph) b->rebuild(active);
-  }
-  for (auto *s : b->solvers) s->fill_ghosts(b->stream);
-  HIPCHK(hipStreamSynchronize(b->stream));
-  const auto t1 = std::chrono::steady_clock::now();
-  *seconds = std::chrono::duration<double>(t1 - t0).count();
-  int rc = NRPYELL_OK;
-  for (int k = 0; k < b->n; k++) {
-    nrpyell_solve_result &r = res[k];
-    memset(&r, 0, sizeof(r));
-    r.converged = b->h_heads[4 * k] == 1.0;
-    // converged, NaN or diverged: the steps when its check decided; running: all launched
-    r.steps = b->h_heads[4 * k] != 0.0 ? (int)b->h_heads[4 * k + 1] : launched;
-    r.log10_l2 = b->h_heads[4 * k + 2];
-    r.seconds = *seconds;
-    if (b->h_heads[4 * k] == 2.0) rc = NRPYELL_ENAN;
-    else if (!r.converged && rc == NRPYELL_OK) rc = NRPYELL_ENOTCONVERGED;
-  }
-  if (rc == NRPYELL_ENAN) set_last_error("a configuration's residual became NaN");
-  if (rc == NRPYELL_ENOTCONVERGED) set_last_error("some configurations diverged or reached max_steps");
-  return rc;
+    for (auto *s : b->solvers) s->fill_ghosts(b->stream);
+    HIPCHK(hipStreamSynchronize(b->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    *seconds = std::chrono::duration<double>(t1 - t0).count();
+    int rc = NRPYELL_OK;
+    for (int k = 0; k < b->n; k++) {
+      nrpyell_solve_result &r = res[k];
+      memset(&r, 0, sizeof(r));
+      r.converged = b->h_heads[4 * k] == 1.0;
+      // converged, NaN or diverged: the steps when its check decided; running: all launched
+      r.steps = b->h_heads[4 * k] != 0.0 ? (int)b->h_heads[4 * k + 1] : launched;
+      r.log10_l2 = b->h_heads[4 * k + 2];
+      r.seconds = *seconds;
+      if (b->h_heads[4 * k] == 2.0) rc = NRPYELL_ENAN;
+      else if (!r.converged && rc == NRPYELL_OK) rc = NRPYELL_ENOTCONVERGED;
+    }
+    if (rc == NRPYELL_ENAN) set_last_error("a configuration's residual became NaN");
+    if (rc == NRPYELL_ENOTCONVERGED) set_last_error("some configurations diverged or reached max_steps");
+    return rc;
+  });
 }
 
 extern "C" int nrpyell_batch_destroy(nrpyell_batch *b) {
-  if (!b) return NRPYELL_OK;
-  (void)hipStreamSynchronize(b->stream);
-  delete b;
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (!b) return NRPYELL_OK;
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+    return NRPYELL_OK;
+  });
 }

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
 #include <string>
 
 #include "nrpyelliptic_amd.h"
@@ -187,9 +188,35 @@ int ghostmap_verify(const GridDims &g, const bc_struct *bc, std::string *err);
 // error plumbing (nrpyell_capi.hip)
 void set_last_error(const std::string &s);
 [[noreturn]] void fatal(const char *what, const std::string &msg);
-#define HIPCHK(x)                                                                                 \
-  do {                                                                                            \
-    hipError_t e_ = (x);                                                                          \
-    if (e_ != hipSuccess) fatal(#x, std::string(hipGetErrorString(e_)) + " at " + __FILE__ + ":" + \
-                                        std::to_string(__LINE__));                                \
+// A failed HIP runtime call inside the library. The nrpyell_* entry points turn it into
+// NRPYELL_EHIP with the message in nrpyell_last_error() (api_call); the reference-surface entry
+// points, void like the reference's, print it and exit(1) as the reference does (ref_call).
+struct HipFailure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+#define HIPCHK(x)                                                                                  \
+  do {                                                                                             \
+    hipError_t e_ = (x);                                                                           \
+    if (e_ != hipSuccess)                                                                          \
+      throw HipFailure(std::string(#x) + " failed: " + hipGetErrorString(e_) + " at " + __FILE__ + \
+                       ":" + std::to_string(__LINE__));                                            \
   } while (0)
+
+template <typename F>
+int api_call(F &&f) noexcept {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return NRPYELL_EHIP;
+  }
+}
+
+template <typename F>
+auto ref_call(const char *name, F &&f) noexcept -> decltype(f()) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    fatal(name, e.what());
+  }
+}

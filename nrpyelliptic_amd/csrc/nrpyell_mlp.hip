@@ -440,9 +440,14 @@ struct nrpyell_mlp {
   int K(int l) const { return l == 1 ? MLP_K0 : MLP_W; }
   void ensure(int M) {
     if (M <= cap) return;
-    for (auto *h : Hs) (void)hipFree(h);
-    for (auto *z : Zs) (void)hipFree(z);
-    (void)hipFree(dZa); (void)hipFree(dZb); (void)hipFree(y); (void)hipFree(tb);
+    // release first; a failed allocation below (HipFailure) leaves cap 0 and null or valid
+    // pointers only, so the next call allocates again
+    cap = 0;
+    for (auto *&h : Hs) { (void)hipFree(h); h = nullptr; }
+    for (auto *&z : Zs) { (void)hipFree(z); z = nullptr; }
+    (void)hipFree(dZa); (void)hipFree(dZb); (void)hipFree(y); (void)hipFree(tb); (void)hipFree(dy);
+    dZa = dZb = nullptr;
+    y = tb = dy = nullptr;
     Hs[0] = mlp_alloc<bf16_t>((size_t)M * MLP_K0);
     for (int l = 1; l <= MLP_HIDDEN; l++) {
       Hs[l] = mlp_alloc<bf16_t>((size_t)M * MLP_W);
@@ -452,7 +457,6 @@ struct nrpyell_mlp {
     dZb = mlp_alloc<bf16_t>((size_t)M * MLP_W);
     y = mlp_alloc<float>(M);
     tb = mlp_alloc<float>(M);
-    (void)hipFree(dy);
     dy = mlp_alloc<float>(M);
     cap = M;
   }
@@ -505,77 +509,85 @@ struct nrpyell_mlp {
 static void backward_full(nrpyell_mlp *s, int M) { s->backward(M); }
 
 extern "C" int nrpyell_mlp_create(unsigned long long seed, nrpyell_mlp **out) {
-  auto s = std::make_unique<nrpyell_mlp>();
-  s->w_off.assign(7, 0);
-  s->b_off.assign(7, 0);
-  s->w_size.assign(7, 0);
-  long off = 0;
-  for (int l = 1; l <= MLP_HIDDEN; l++) {
-    s->w_off[l] = off;
-    s->w_size[l] = (long)MLP_W * s->K(l);
-    off += s->w_size[l];
-    s->b_off[l] = off;
+  return api_call([&]() -> int {
+    auto s = std::make_unique<nrpyell_mlp>();
+    s->w_off.assign(7, 0);
+    s->b_off.assign(7, 0);
+    s->w_size.assign(7, 0);
+    long off = 0;
+    for (int l = 1; l <= MLP_HIDDEN; l++) {
+      s->w_off[l] = off;
+      s->w_size[l] = (long)MLP_W * s->K(l);
+      off += s->w_size[l];
+      s->b_off[l] = off;
+      off += MLP_W;
+    }
+    s->w_off[6] = off;
+    s->w_size[6] = MLP_W;
     off += MLP_W;
-  }
-  s->w_off[6] = off;
-  s->w_size[6] = MLP_W;
-  off += MLP_W;
-  s->b_off[6] = off;
-  off += 1;
-  s->n_params = off;
-  // He-style initialisation (uniform, splitmix64 stream), biases zero
-  std::vector<float> h(off, 0.0f);
-  unsigned long long z = seed;
-  auto rnd = [&]() {
-    z += 0x9E3779B97F4A7C15ull;
-    unsigned long long x = z;
-    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-    x ^= x >> 31;
-    return (double)(x >> 11) / 9007199254740992.0;
-  };
-  for (int l = 1; l <= 6; l++) {
-    const int fan_in = l == 1 ? MLP_NFEAT : MLP_W;
-    const double a = sqrt(6.0 / fan_in) * (l == 6 ? 0.1 : 1.0);
-    for (long k = 0; k < s->w_size[l]; k++) h[s->w_off[l] + k] = (float)((2 * rnd() - 1) * a);
-  }
-  s->p = mlp_alloc<float>(off);
-  s->g = mlp_alloc<float>(off);
-  s->m = mlp_alloc<float>(off);
-  s->v = mlp_alloc<float>(off);
-  s->pw = mlp_alloc<bf16_t>(off);
-  s->loss = mlp_alloc<double>(1);
-  HIPCHK(hipMemcpy(s->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(s->m, 0, off * sizeof(float)));
-  HIPCHK(hipMemset(s->v, 0, off * sizeof(float)));
-  HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
-  s->refresh_bf16();
-  HIPCHK(hipStreamSynchronize(s->st));
-  *out = s.release();
-  return NRPYELL_OK;
+    s->b_off[6] = off;
+    off += 1;
+    s->n_params = off;
+    // He-style initialisation (uniform, splitmix64 stream), biases zero
+    std::vector<float> h(off, 0.0f);
+    unsigned long long z = seed;
+    auto rnd = [&]() {
+      z += 0x9E3779B97F4A7C15ull;
+      unsigned long long x = z;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      x ^= x >> 31;
+      return (double)(x >> 11) / 9007199254740992.0;
+    };
+    for (int l = 1; l <= 6; l++) {
+      const int fan_in = l == 1 ? MLP_NFEAT : MLP_W;
+      const double a = sqrt(6.0 / fan_in) * (l == 6 ? 0.1 : 1.0);
+      for (long k = 0; k < s->w_size[l]; k++) h[s->w_off[l] + k] = (float)((2 * rnd() - 1) * a);
+    }
+    s->p = mlp_alloc<float>(off);
+    s->g = mlp_alloc<float>(off);
+    s->m = mlp_alloc<float>(off);
+    s->v = mlp_alloc<float>(off);
+    s->pw = mlp_alloc<bf16_t>(off);
+    s->loss = mlp_alloc<double>(1);
+    HIPCHK(hipMemcpy(s->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(s->m, 0, off * sizeof(float)));
+    HIPCHK(hipMemset(s->v, 0, off * sizeof(float)));
+    HIPCHK(hipStreamCreateWithFlags(&s->st, hipStreamNonBlocking));
+    s->refresh_bf16();
+    HIPCHK(hipStreamSynchronize(s->st));
+    *out = s.release();
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_mlp_destroy(nrpyell_mlp *s) {
-  if (s) {
-    (void)hipStreamSynchronize(s->st);
-    delete s;
-  }
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (s) {
+      (void)hipStreamSynchronize(s->st);
+      delete s;
+    }
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" long nrpyell_mlp_num_params(nrpyell_mlp *s) { return s->n_params; }
 
 extern "C" int nrpyell_mlp_get_params(nrpyell_mlp *s, float *host) {
-  HIPCHK(hipStreamSynchronize(s->st));
-  HIPCHK(hipMemcpy(host, s->p, s->n_params * sizeof(float), hipMemcpyDeviceToHost));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    HIPCHK(hipStreamSynchronize(s->st));
+    HIPCHK(hipMemcpy(host, s->p, s->n_params * sizeof(float), hipMemcpyDeviceToHost));
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_mlp_set_params(nrpyell_mlp *s, const float *host) {
-  HIPCHK(hipMemcpy(s->p, host, s->n_params * sizeof(float), hipMemcpyHostToDevice));
-  s->refresh_bf16();
-  HIPCHK(hipStreamSynchronize(s->st));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    HIPCHK(hipMemcpy(s->p, host, s->n_params * sizeof(float), hipMemcpyHostToDevice));
+    s->refresh_bf16();
+    HIPCHK(hipStreamSynchronize(s->st));
+    return NRPYELL_OK;
+  });
 }
 
 static int check_rows(int M) {
@@ -588,33 +600,37 @@ static int check_rows(int M) {
 
 // Forward pass on M rows of device BF16 features [M x 32]; y (device, FP32) gets M outputs.
 extern "C" int nrpyell_mlp_forward(nrpyell_mlp *s, const void *x, int M, float *y_out) {
-  if (check_rows(M)) return NRPYELL_EINVAL;
-  MlpOffsets off;
-  for (int l = 1; l <= MLP_HIDDEN + 1; l++) {
-    off.w[l] = s->w_off[l];
-    off.b[l] = s->b_off[l];
-  }
-  hipLaunchKernelGGL(mlp_forward_fused_kernel, dim3((M + FI_R - 1) / FI_R), dim3(256), 0, s->st, (const bf16_t *)x,
-                     s->pw, s->p, off, M, y_out);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(s->st));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (check_rows(M)) return NRPYELL_EINVAL;
+    MlpOffsets off;
+    for (int l = 1; l <= MLP_HIDDEN + 1; l++) {
+      off.w[l] = s->w_off[l];
+      off.b[l] = s->b_off[l];
+    }
+    hipLaunchKernelGGL(mlp_forward_fused_kernel, dim3((M + FI_R - 1) / FI_R), dim3(256), 0, s->st, (const bf16_t *)x,
+                       s->pw, s->p, off, M, y_out);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s->st));
+    return NRPYELL_OK;
+  });
 }
 
 // Loss (mean squared error) and its gradient for M rows; grads copied to host when non-null.
 extern "C" int nrpyell_mlp_loss_grad(nrpyell_mlp *s, const void *x, const float *t, int M, double *loss,
                                      float *grads_host) {
-  if (check_rows(M)) return NRPYELL_EINVAL;
-  s->ensure(M);
-  HIPCHK(hipMemcpyAsync(s->Hs[0], x, (size_t)M * MLP_K0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, s->st));
-  HIPCHK(hipMemcpyAsync(s->tb, t, (size_t)M * sizeof(float), hipMemcpyDeviceToDevice, s->st));
-  s->forward(M);
-  backward_full(s, M);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, s->st));
-  if (grads_host) HIPCHK(hipMemcpyAsync(grads_host, s->g, s->n_params * sizeof(float), hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (check_rows(M)) return NRPYELL_EINVAL;
+    s->ensure(M);
+    HIPCHK(hipMemcpyAsync(s->Hs[0], x, (size_t)M * MLP_K0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, s->st));
+    HIPCHK(hipMemcpyAsync(s->tb, t, (size_t)M * sizeof(float), hipMemcpyDeviceToDevice, s->st));
+    s->forward(M);
+    backward_full(s, M);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, s->st));
+    if (grads_host) HIPCHK(hipMemcpyAsync(grads_host, s->g, s->n_params * sizeof(float), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    return NRPYELL_OK;
+  });
 }
 
 static void adam(nrpyell_mlp *s, float lr) {
@@ -631,46 +647,52 @@ static void adam(nrpyell_mlp *s, float lr) {
 // losses (host, nsteps entries, may be null) receives each step's minibatch loss.
 extern "C" int nrpyell_mlp_train(nrpyell_mlp *s, const void *X, const float *t, long N, int batch, int nsteps,
                                  float lr, unsigned long long seed, double *losses) {
-  if (check_rows(batch) || N <= 0) return NRPYELL_EINVAL;
-  s->ensure(batch);
-  std::vector<double> hl;
-  double *dl = mlp_alloc<double>(nsteps > 0 ? nsteps : 1);
-  for (int k = 0; k < nsteps; k++) {
-    hipLaunchKernelGGL(gather_rows_kernel, dim3(batch), dim3(32), 0, s->st, (const bf16_t *)X, t, N, batch,
-                       seed + 0x632BE59BD9B4E019ull * (unsigned long long)(k + 1), s->Hs[0], s->tb);
-    s->forward(batch);
-    backward_full(s, batch);
-    HIPCHK(hipMemcpyAsync(dl + k, s->loss, sizeof(double), hipMemcpyDeviceToDevice, s->st));
-    const float frac = nsteps > 1 ? (float)k / (float)(nsteps - 1) : 0.0f;
-    const float lrk = lr * (0.01f + 0.99f * 0.5f * (1.0f + cosf(3.14159265f * frac)));
-    adam(s, lrk);
-  }
-  HIPCHK(hipGetLastError());
-  if (losses) HIPCHK(hipMemcpyAsync(losses, dl, nsteps * sizeof(double), hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipStreamSynchronize(s->st));
-  (void)hipFree(dl);
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (check_rows(batch) || N <= 0) return NRPYELL_EINVAL;
+    s->ensure(batch);
+    std::vector<double> hl;
+    double *dl = mlp_alloc<double>(nsteps > 0 ? nsteps : 1);
+    for (int k = 0; k < nsteps; k++) {
+      hipLaunchKernelGGL(gather_rows_kernel, dim3(batch), dim3(32), 0, s->st, (const bf16_t *)X, t, N, batch,
+                         seed + 0x632BE59BD9B4E019ull * (unsigned long long)(k + 1), s->Hs[0], s->tb);
+      s->forward(batch);
+      backward_full(s, batch);
+      HIPCHK(hipMemcpyAsync(dl + k, s->loss, sizeof(double), hipMemcpyDeviceToDevice, s->st));
+      const float frac = nsteps > 1 ? (float)k / (float)(nsteps - 1) : 0.0f;
+      const float lrk = lr * (0.01f + 0.99f * 0.5f * (1.0f + cosf(3.14159265f * frac)));
+      adam(s, lrk);
+    }
+    HIPCHK(hipGetLastError());
+    if (losses) HIPCHK(hipMemcpyAsync(losses, dl, nsteps * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    HIPCHK(hipStreamSynchronize(s->st));
+    (void)hipFree(dl);
+    return NRPYELL_OK;
+  });
 }
 
 // n_take rows drawn uniformly (with replacement) from device rows (x_src [n_src x 64],
 // t_src [n_src]) into x_dst/t_dst: the per-configuration sample of the training set.
 extern "C" int nrpyell_mlp_sample_rows(const void *x_src, const float *t_src, long n_src, long n_take,
                                        unsigned long long seed, void *x_dst, float *t_dst) {
-  if (n_src <= 0 || n_take <= 0 || n_take > (1L << 30)) {
-    set_last_error("sample_rows: bad sizes");
-    return NRPYELL_EINVAL;
-  }
-  hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)n_take), dim3(64), 0, 0, (const bf16_t *)x_src, t_src, n_src,
-                     (int)n_take, seed, (bf16_t *)x_dst, t_dst);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    if (n_src <= 0 || n_take <= 0 || n_take > (1L << 30)) {
+      set_last_error("sample_rows: bad sizes");
+      return NRPYELL_EINVAL;
+    }
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)n_take), dim3(64), 0, 0, (const bf16_t *)x_src, t_src, n_src,
+                       (int)n_take, seed, (bf16_t *)x_dst, t_dst);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return NRPYELL_OK;
+  });
 }
 
 extern "C" int nrpyell_mlp_adam_step(nrpyell_mlp *s, float lr) {
-  adam(s, lr);
-  HIPCHK(hipStreamSynchronize(s->st));
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    adam(s, lr);
+    HIPCHK(hipStreamSynchronize(s->st));
+    return NRPYELL_OK;
+  });
 }
 
 static FeatureParams feature_params(const paramstruct *p) {
@@ -700,29 +722,33 @@ extern "C" long nrpyell_mlp_feature_rows(const griddata_struct *gd) {
 
 // Features (device BF16 [rows x 32], padding rows zero), per-row u scale and grid index.
 extern "C" int nrpyell_mlp_features(const griddata_struct *gd, void *X, double *scale, long *idx) {
-  const GridDims g = dims_from_params(&gd->params);
-  const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
-  double *x0 = mlp_alloc<double>(g.G0), *x1 = mlp_alloc<double>(g.G1), *x2 = mlp_alloc<double>(g.G2);
-  HIPCHK(hipMemcpy(x0, gd->xx[0], g.G0 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(x1, gd->xx[1], g.G1 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(x2, gd->xx[2], g.G2 * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemset(X, 0, rows * MLP_K0 * sizeof(bf16_t)));
-  hipLaunchKernelGGL(features_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, g, feature_params(&gd->params), x0, x1,
-                     x2, (bf16_t *)X, scale, idx);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  (void)hipFree(x0); (void)hipFree(x1); (void)hipFree(x2);
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    const GridDims g = dims_from_params(&gd->params);
+    const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
+    double *x0 = mlp_alloc<double>(g.G0), *x1 = mlp_alloc<double>(g.G1), *x2 = mlp_alloc<double>(g.G2);
+    HIPCHK(hipMemcpy(x0, gd->xx[0], g.G0 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x1, gd->xx[1], g.G1 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(x2, gd->xx[2], g.G2 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(X, 0, rows * MLP_K0 * sizeof(bf16_t)));
+    hipLaunchKernelGGL(features_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, g, feature_params(&gd->params), x0, x1,
+                       x2, (bf16_t *)X, scale, idx);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    (void)hipFree(x0); (void)hipFree(x1); (void)hipFree(x2);
+    return NRPYELL_OK;
+  });
 }
 
 // Training targets t = u/scale from the solved y_n of a griddata at the feature rows.
 extern "C" int nrpyell_mlp_targets(const griddata_struct *gd, const double *scale, const long *idx, float *t) {
-  const GridDims g = dims_from_params(&gd->params);
-  const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2;
-  hipLaunchKernelGGL(target_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, gd->gridfuncs.y_n_gfs, scale, idx, t);
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  return NRPYELL_OK;
+  return api_call([&]() -> int {
+    const GridDims g = dims_from_params(&gd->params);
+    const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2;
+    hipLaunchKernelGGL(target_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, n, gd->gridfuncs.y_n_gfs, scale, idx, t);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return NRPYELL_OK;
+  });
 }
 
 // Seed griddata's y_n from the network: u = e^t scale at interior and pure outer points,
@@ -730,39 +756,41 @@ extern "C" int nrpyell_mlp_targets(const griddata_struct *gd, const double *scal
 // state of du/dt = v - eta u), parity images written in every ghost point. k_odd_gfs is the
 // scratch of the smoothing.
 extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd, int smooth_passes) {
-  if (smooth_passes < 0 || smooth_passes > 64) {
-    set_last_error("mlp_seed: smooth_passes must be in [0, 64]");
-    return NRPYELL_EINVAL;
-  }
-  const GridDims g = dims_from_params(&gd->params);
-  const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
-  bf16_t *X = mlp_alloc<bf16_t>(rows * MLP_K0);
-  double *scale = mlp_alloc<double>(rows);
-  long *idx = mlp_alloc<long>(rows);
-  float *y = mlp_alloc<float>(rows);
-  nrpyell_mlp_features(gd, X, scale, idx);
-  const int chunk = 1 << 18;  // rows per forward pass (multiple of 64)
-  for (long r0 = 0; r0 < rows; r0 += chunk) {
-    const int M = (int)((rows - r0) < chunk ? (rows - r0) : chunk);
-    nrpyell_mlp_forward(s, X + r0 * MLP_K0, M, y + r0);
-  }
-  double *yn = gd->gridfuncs.y_n_gfs, *tmp = gd->gridfuncs.k_odd_gfs;
-  hipLaunchKernelGGL(seed_state_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, g, n, y, scale, idx,
-                     gd->params.eta_damping, yn);
-  const dim3 sg((g.G0 - NRPYELL_NG + 63) / 64, g.N1, g.N2);
-  for (int k = 0; k < smooth_passes; k++) {
-    hipLaunchKernelGGL(smooth_kernel<0>, sg, dim3(64), 0, 0, g, yn, tmp);
-    hipLaunchKernelGGL(smooth_kernel<1>, sg, dim3(64), 0, 0, g, tmp, yn);
-    hipLaunchKernelGGL(smooth_kernel<2>, sg, dim3(64), 0, 0, g, yn, tmp);
-    HIPCHK(hipMemcpyAsync(yn, tmp, sizeof(double) * g.ntot, hipMemcpyDeviceToDevice, 0));
-  }
-  if (smooth_passes > 0) {
-    launch_fill_ghost_planes<double>(g, yn, 1, 0);
-    hipLaunchKernelGGL(v_from_u_kernel, dim3((g.ntot + 255) / 256), dim3(256), 0, 0, g.ntot,
+  return api_call([&]() -> int {
+    if (smooth_passes < 0 || smooth_passes > 64) {
+      set_last_error("mlp_seed: smooth_passes must be in [0, 64]");
+      return NRPYELL_EINVAL;
+    }
+    const GridDims g = dims_from_params(&gd->params);
+    const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
+    bf16_t *X = mlp_alloc<bf16_t>(rows * MLP_K0);
+    double *scale = mlp_alloc<double>(rows);
+    long *idx = mlp_alloc<long>(rows);
+    float *y = mlp_alloc<float>(rows);
+    nrpyell_mlp_features(gd, X, scale, idx);
+    const int chunk = 1 << 18;  // rows per forward pass (multiple of 64)
+    for (long r0 = 0; r0 < rows; r0 += chunk) {
+      const int M = (int)((rows - r0) < chunk ? (rows - r0) : chunk);
+      nrpyell_mlp_forward(s, X + r0 * MLP_K0, M, y + r0);
+    }
+    double *yn = gd->gridfuncs.y_n_gfs, *tmp = gd->gridfuncs.k_odd_gfs;
+    hipLaunchKernelGGL(seed_state_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, g, n, y, scale, idx,
                        gd->params.eta_damping, yn);
-  }
-  HIPCHK(hipGetLastError());
-  HIPCHK(hipDeviceSynchronize());
-  (void)hipFree(X); (void)hipFree(scale); (void)hipFree(idx); (void)hipFree(y);
-  return NRPYELL_OK;
+    const dim3 sg((g.G0 - NRPYELL_NG + 63) / 64, g.N1, g.N2);
+    for (int k = 0; k < smooth_passes; k++) {
+      hipLaunchKernelGGL(smooth_kernel<0>, sg, dim3(64), 0, 0, g, yn, tmp);
+      hipLaunchKernelGGL(smooth_kernel<1>, sg, dim3(64), 0, 0, g, tmp, yn);
+      hipLaunchKernelGGL(smooth_kernel<2>, sg, dim3(64), 0, 0, g, yn, tmp);
+      HIPCHK(hipMemcpyAsync(yn, tmp, sizeof(double) * g.ntot, hipMemcpyDeviceToDevice, 0));
+    }
+    if (smooth_passes > 0) {
+      launch_fill_ghost_planes<double>(g, yn, 1, 0);
+      hipLaunchKernelGGL(v_from_u_kernel, dim3((g.ntot + 255) / 256), dim3(256), 0, 0, g.ntot,
+                         gd->params.eta_damping, yn);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    (void)hipFree(X); (void)hipFree(scale); (void)hipFree(idx); (void)hipFree(y);
+    return NRPYELL_OK;
+  });
 }

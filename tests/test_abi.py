@@ -165,3 +165,51 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(runtime, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(runtime.NrpyellError):
         runtime.lib()
+
+
+_NO_DEVICE_PROBE = r"""
+import ctypes as C, sys
+L = C.CDLL(sys.argv[1])
+L.nrpyell_last_error.restype = C.c_char_p
+p = C.c_void_p()
+if L.nrpyell_malloc_device(C.byref(p), C.c_size_t(64)) == 0:
+    print("device-present"); sys.exit(0)
+h = C.c_void_p()
+rc = L.nrpyell_mlp_create(C.c_ulonglong(1), C.byref(h))
+print("mlp_create", rc, L.nrpyell_last_error().decode())
+sys.stdout.flush()
+if sys.argv[2] == "reference":
+    # a void reference-surface entry point: prints and exits(1) like the reference
+    sys.path.insert(0, sys.argv[3])
+    from nrpyelliptic_amd import abi
+    prm = abi.paramstruct()
+    L.set_Cparameters_to_default(C.byref(prm))
+    L.rhs_eval(C.byref(prm), None, None, None, None)
+    print("rhs_eval returned")
+"""
+
+
+def _run_no_device(mode):
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, "-c", _NO_DEVICE_PROBE, runtime.LIB_PATH, mode, ROOT], capture_output=True,
+                       text=True, timeout=300)
+    if "device-present" in r.stdout:
+        pytest.skip("a GPU is visible: the HIP failure cannot be forced from the host")
+    return r
+
+
+def test_hip_failures_return_ehip_from_the_extension_api():
+    """A failing HIP call inside an nrpyell_* entry point (here: no device, so hipMalloc fails)
+    comes back as NRPYELL_EHIP with the message, and the process goes on."""
+    r = _run_no_device("api")
+    assert r.returncode == 0, r.stderr
+    assert "mlp_create -2 hipMalloc" in r.stdout and "failed" in r.stdout
+
+
+def test_hip_failures_exit_from_the_reference_surface():
+    """The reference's void functions keep the reference's error convention: print, exit(1)."""
+    r = _run_no_device("reference")
+    assert "mlp_create -2" in r.stdout
+    assert r.returncode == 1 and "rhs_eval returned" not in r.stdout
+    assert "libnrpyell_hip: rhs_eval failed" in r.stderr
