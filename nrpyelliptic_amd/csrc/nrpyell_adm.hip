@@ -222,8 +222,8 @@ extern "C" int nrpyell_Initialize_ADMBase(const griddata_struct *gd, const nrpye
     a.z = z;
     for (int c = 0; c < NRPYELL_ADM_NOUT; c++) a.out[c] = outs[c];
     a.n = n;
-    int *err = nullptr;
-    HIPCHK(hipMalloc(&err, sizeof(int)));
+    DevBuf<int> errb(1);
+    int *err = errb.p;
     HIPCHK(hipMemset(err, 0, sizeof(int)));
     a.err = err;
     const dim3 grd((unsigned)((n + 255) / 256)), blk(256);
@@ -238,7 +238,6 @@ extern "C" int nrpyell_Initialize_ADMBase(const griddata_struct *gd, const nrpye
     HIPCHK(hipGetLastError());
     int herr = 0;
     HIPCHK(hipMemcpy(&herr, err, sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(hipFree(err));
     if (herr == 1) {
       set_last_error("NAN found in (xx0,xx1,xx2) after moving the point by NRPy_epsilon; consider increasing "
                      "NRPy_epsilon");

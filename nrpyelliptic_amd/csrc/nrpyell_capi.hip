@@ -808,13 +808,13 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     s->kc64 = dalloc<double>(6 * n2);
     s->kc32 = dalloc<float>(6 * n2);
     s->src64 = dalloc<double>(2 * g.ntot);
-    int *bad = dalloc<int>(1);
+    DevBuf<int> badb(1);
+    int *bad = badb.p;
     HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
     launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->src64, bad, s->stream);
     int hbad = 0;
     HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    (void)hipFree(bad);
     if (hbad) {
       set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
                      "wavespeed_gf_all_points sets it");

@@ -202,6 +202,16 @@ struct HipFailure : std::runtime_error {
                        ":" + std::to_string(__LINE__));                                            \
   } while (0)
 
+// Device buffer owned by one call: freed on every return path, HipFailure included.
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  explicit DevBuf(size_t n) { HIPCHK(hipMalloc((void **)&p, (n > 0 ? n : 1) * sizeof(T))); }
+  ~DevBuf() { (void)hipFree(p); }
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+};
+
 template <typename F>
 int api_call(F &&f) noexcept {
   try {

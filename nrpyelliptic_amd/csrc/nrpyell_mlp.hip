@@ -651,7 +651,8 @@ extern "C" int nrpyell_mlp_train(nrpyell_mlp *s, const void *X, const float *t, 
     if (check_rows(batch) || N <= 0) return NRPYELL_EINVAL;
     s->ensure(batch);
     std::vector<double> hl;
-    double *dl = mlp_alloc<double>(nsteps > 0 ? nsteps : 1);
+    DevBuf<double> dlb(nsteps > 0 ? nsteps : 1);
+    double *dl = dlb.p;
     for (int k = 0; k < nsteps; k++) {
       hipLaunchKernelGGL(gather_rows_kernel, dim3(batch), dim3(32), 0, s->st, (const bf16_t *)X, t, N, batch,
                          seed + 0x632BE59BD9B4E019ull * (unsigned long long)(k + 1), s->Hs[0], s->tb);
@@ -665,7 +666,6 @@ extern "C" int nrpyell_mlp_train(nrpyell_mlp *s, const void *X, const float *t, 
     HIPCHK(hipGetLastError());
     if (losses) HIPCHK(hipMemcpyAsync(losses, dl, nsteps * sizeof(double), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
-    (void)hipFree(dl);
     return NRPYELL_OK;
   });
 }
@@ -725,7 +725,8 @@ extern "C" int nrpyell_mlp_features(const griddata_struct *gd, void *X, double *
   return api_call([&]() -> int {
     const GridDims g = dims_from_params(&gd->params);
     const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
-    double *x0 = mlp_alloc<double>(g.G0), *x1 = mlp_alloc<double>(g.G1), *x2 = mlp_alloc<double>(g.G2);
+    DevBuf<double> b0(g.G0), b1(g.G1), b2(g.G2);
+    double *x0 = b0.p, *x1 = b1.p, *x2 = b2.p;
     HIPCHK(hipMemcpy(x0, gd->xx[0], g.G0 * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x1, gd->xx[1], g.G1 * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(x2, gd->xx[2], g.G2 * sizeof(double), hipMemcpyHostToDevice));
@@ -734,7 +735,6 @@ extern "C" int nrpyell_mlp_features(const griddata_struct *gd, void *X, double *
                        x2, (bf16_t *)X, scale, idx);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
-    (void)hipFree(x0); (void)hipFree(x1); (void)hipFree(x2);
     return NRPYELL_OK;
   });
 }
@@ -763,15 +763,20 @@ extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd, int smooth_
     }
     const GridDims g = dims_from_params(&gd->params);
     const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
-    bf16_t *X = mlp_alloc<bf16_t>(rows * MLP_K0);
-    double *scale = mlp_alloc<double>(rows);
-    long *idx = mlp_alloc<long>(rows);
-    float *y = mlp_alloc<float>(rows);
-    nrpyell_mlp_features(gd, X, scale, idx);
+    DevBuf<bf16_t> Xb(rows * MLP_K0);
+    DevBuf<double> scaleb(rows);
+    DevBuf<long> idxb(rows);
+    DevBuf<float> yb(rows);
+    bf16_t *X = Xb.p;
+    double *scale = scaleb.p;
+    long *idx = idxb.p;
+    float *y = yb.p;
+    int rc = nrpyell_mlp_features(gd, X, scale, idx);
+    if (rc != NRPYELL_OK) return rc;
     const int chunk = 1 << 18;  // rows per forward pass (multiple of 64)
     for (long r0 = 0; r0 < rows; r0 += chunk) {
       const int M = (int)((rows - r0) < chunk ? (rows - r0) : chunk);
-      nrpyell_mlp_forward(s, X + r0 * MLP_K0, M, y + r0);
+      if ((rc = nrpyell_mlp_forward(s, X + r0 * MLP_K0, M, y + r0)) != NRPYELL_OK) return rc;
     }
     double *yn = gd->gridfuncs.y_n_gfs, *tmp = gd->gridfuncs.k_odd_gfs;
     hipLaunchKernelGGL(seed_state_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, g, n, y, scale, idx,
@@ -790,7 +795,6 @@ extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd, int smooth_
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
-    (void)hipFree(X); (void)hipFree(scale); (void)hipFree(idx); (void)hipFree(y);
     return NRPYELL_OK;
   });
 }
