@@ -228,8 +228,63 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
                 "nn_s": r.seconds, "nn_used": bool(used), "log10_res0_zero": r0, "log10_res0_nn": r1,
                 "converged": bool(plain.converged and r.converged)}
 
+    # damping strength tuned per grid (EtaTuner) on a calibration configuration of the grid
+    # (seed 901: neither the bench problem, the held-out set nor the training set)
+    from nrpyelliptic_amd.autotune import EtaTuner
+    tuner = EtaTuner(cache_path=None)
+    t0 = time.perf_counter()
+    calib = runtime.Problem(config.random_batch(1, seed=901, grids=False, Nxx=(128, 128, 16))[0])
+    f_nn = tuner.choose(calib, lambda Q: learned.seed_with_fallback(net, Q), "nn")
+    f_zero = tuner.choose(calib, lambda Q: Q.set("y_n_gfs", np.zeros(2 * Q.ntot)), "zero")
+    calib.close()
+    tune_s = time.perf_counter() - t0
+
+    def tuned_solves(prob, plain):
+        """NN-seeded FP32->FP64 solve at f_nn x eta and plain FP64 at f_zero x eta, against
+        the plain FP64 solve at the configured eta; eta restored afterwards."""
+        eta0 = prob.gd.params.eta_damping
+        try:
+            prob.gd.params.eta_damping = f_nn * eta0
+            used, _, _ = learned.seed_with_fallback(net, prob)
+            S = prob.solver(precision=32, log10_target=-10.0, max_steps=20000)
+            rn = S.solve()
+            S.close()
+            prob.gd.params.eta_damping = f_zero * eta0
+            prob.set("y_n_gfs", np.zeros(2 * prob.ntot))
+            S = prob.solver(log10_target=-10.0, max_steps=20000)
+            rz = S.solve()
+            S.close()
+        finally:
+            prob.gd.params.eta_damping = eta0
+        return {"plain_steps": plain.steps, "plain_s": plain.seconds, "nn_steps": rn.steps, "nn_fp32_steps": rn.fp32_steps,
+                "nn_s": rn.seconds, "nn_used": bool(used), "plain_tuned_steps": rz.steps, "plain_tuned_s": rz.seconds,
+                "converged": bool(plain.converged and rn.converged and rz.converged)}
+
     bench_cfg = nn_solve(P, r_plain)
-    rows = []
+    bench_tuned = tuned_solves(P, r_plain)
+    # accuracy of the two paths on the bench problem: max |u - u*| with u* converged to 1e-14
+    interior = np.zeros(P.Ng[::-1], dtype=bool)
+    interior[5:-5, 5:-5, 5:-5] = True
+    interior = interior.ravel()
+    eta0 = P.gd.params.eta_damping
+    errs = {}
+    try:
+        for name, eta, nn, tgt in (("star", eta0, False, -14.0), ("plain", eta0, False, -10.0),
+                                   ("nn_tuned", f_nn * eta0, True, -10.0)):
+            P.gd.params.eta_damping = eta
+            if nn:
+                learned.seed_with_fallback(net, P)
+            else:
+                P.set("y_n_gfs", np.zeros(2 * P.ntot))
+            S = P.solver(precision=32 if nn else 64, log10_target=tgt, max_steps=40000)
+            S.solve()
+            S.close()
+            errs[name] = P.get("y_n_gfs")[: P.ntot][interior].copy()
+    finally:
+        P.gd.params.eta_damping = eta0
+    bench_tuned["max_abs_u_error_plain"] = float(np.max(np.abs(errs["plain"] - errs["star"])))
+    bench_tuned["max_abs_u_error_nn_tuned"] = float(np.max(np.abs(errs["nn_tuned"] - errs["star"])))
+    rows, rows_t = [], []
     for cfg in config.random_batch(n_held_out, seed=900, grids=False, Nxx=(128, 128, 16)):
         Q = runtime.Problem(cfg)
         S = Q.solver(log10_target=-10.0, max_steps=10000)
@@ -241,12 +296,26 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
         S.close()
         if rz is not None and rz.converged:
             rows.append(nn_solve(Q, rz))
+            rows_t.append(tuned_solves(Q, rz))
         Q.close()
     ok = [r for r in rows if r["converged"]]
+    okt = [r for r in rows_t if r["converged"]]
     return {"bench_problem": bench_cfg, "speedup_bench_problem": bench_cfg["plain_s"] / bench_cfg["nn_s"],
             "held_out": {"n": len(ok), "speedup_time": sum(r["plain_s"] for r in ok) / max(sum(r["nn_s"] for r in ok), 1e-12),
                          "speedup_steps": sum(r["plain_steps"] for r in ok) / max(sum(r["nn_steps"] for r in ok), 1),
-                         "nn_used_frac": sum(r["nn_used"] for r in ok) / max(len(ok), 1)}}
+                         "nn_used_frac": sum(r["nn_used"] for r in ok) / max(len(ok), 1)},
+            "eta_tuned": {
+                "factor_nn": f_nn, "factor_zero": f_zero, "calibration_seconds": tune_s,
+                "calibration": "seed 901 configuration, 128x128x16: fewest steps to -10 among the factors whose "
+                               "u is within half the reference path's error of a solve to -14",
+                "bench_problem": bench_tuned,
+                "speedup_bench_problem": bench_tuned["plain_s"] / bench_tuned["nn_s"],
+                "speedup_bench_problem_vs_plain_tuned": bench_tuned["plain_tuned_s"] / bench_tuned["nn_s"],
+                "held_out": {"n": len(okt),
+                             "speedup_time": sum(r["plain_s"] for r in okt) / max(sum(r["nn_s"] for r in okt), 1e-12),
+                             "speedup_steps": sum(r["plain_steps"] for r in okt) / max(sum(r["nn_steps"] for r in okt), 1),
+                             "speedup_time_vs_plain_tuned": sum(r["plain_tuned_s"] for r in okt)
+                             / max(sum(r["nn_s"] for r in okt), 1e-12)}}}
 
 
 def batched_solve(n=64, seed=42):

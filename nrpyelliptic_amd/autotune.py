@@ -1,4 +1,7 @@
-"""Tile-width autotuner for the fused RK4 substep kernel.
+"""Autotuners of the relaxation: the fused kernel's tile shape (TileTuner) and the damping
+strength eta (EtaTuner).
+
+Tile width of the fused RK4 substep kernel:
 
 The fused kernel puts i0 across the lanes of a 256-thread block (tile TX x 256/TX) and gives
 each thread one or two i2 planes; which shape keeps HBM busiest depends on N0 (ragged last
@@ -14,7 +17,9 @@ from __future__ import annotations
 
 import json
 import os
-from typing import Dict, Iterable, Optional, Sequence
+from typing import Callable, Dict, Iterable, Optional, Sequence
+
+import numpy as np
 
 from .runtime import NrpyellError, Problem, Solver
 
@@ -85,3 +90,115 @@ class TileTuner:
         with open(tmp, "w") as f:
             json.dump(self.table, f, indent=1, sort_keys=True)
         os.replace(tmp, self.cache_path)
+
+
+# ---------------------------------------------------------------------------------------
+# Damping strength of the relaxation
+# ---------------------------------------------------------------------------------------
+ETA_FACTORS = (1.0, 1.4, 2.0, 2.8, 4.0, 5.6)  # multiples of the configured eta
+
+
+def _eta_key(prob: Problem, start: str, target: float) -> str:
+    p = prob.gd.params
+    return f"{p.Nxx0}x{p.Nxx1}x{p.Nxx2}_A{p.AMAX:g}_w{p.SINHWAA:g}_{start}_t{target:g}"
+
+
+class EtaTuner:
+    """Chooses the damping strength eta of the hyperbolic relaxation per grid and start, at no
+    loss of accuracy.
+
+    In u_tt + eta u_t = c^2 (lap u + ...) a spatial mode of eigenvalue lambda decays at rate
+    eta/2 while eta < 2 c sqrt(lambda) (underdamped) and at c^2 lambda / eta above it. A larger
+    eta therefore reaches the residual target in fewer steps, but past some point only because
+    the residual (masked to r < integration radius) no longer sees the slow, long-wavelength
+    error that is left: the converged u gets worse. A start that carries less low-frequency
+    error, such as the network's guess, stays accurate up to a larger eta.
+
+    `choose(prob, start, name)` calibrates on `prob` (a configuration of the grid, not the one
+    to be measured): u* = a solve from the zero guess at the configured eta to target - 4; the
+    reference error e_ref = max |u - u*| of the zero-guess solve at the configured eta to the
+    target (the reference driver's path); then, for every factor of the configured eta, a
+    solve from the initial state `start(prob)` writes to the target. The factor with the
+    fewest steps among those whose error is at most `margin` x e_ref is cached per (grid,
+    start, target), 1.0 if none qualifies; configurations of the grid with other punctures
+    and foci reuse it. The problem's eta and state are restored afterwards. `cache_path=None`
+    keeps the table in memory only."""
+
+    def __init__(self, cache_path: Optional[str] = None, factors: Sequence[float] = ETA_FACTORS,
+                 target: float = -10.0, max_steps: int = 40000, margin: float = 0.5):
+        self.cache_path = cache_path
+        self.factors = tuple(factors)
+        self.target = target
+        self.max_steps = max_steps
+        self.margin = margin
+        self.measured = 0  # calibrations run (not served from the table)
+        self.table: Dict[str, dict] = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.table = json.load(f)
+
+    def _solve(self, prob: Problem, eta: float, start: Callable[[Problem], object], target: float):
+        """(steps, interior u) of one FP64 solve, or None if it does not converge."""
+        prob.gd.params.eta_damping = eta
+        start(prob)  # after eta: a seed writes v = eta u
+        s = Solver(prob, log10_target=target, max_steps=self.max_steps)
+        try:
+            r = s.solve()
+        except NrpyellError:
+            return None  # NaN at this eta
+        finally:
+            s.close()
+        if not r.converged:
+            return None
+        return int(r.steps), prob.get("y_n_gfs")[: prob.ntot][_interior(prob)]
+
+    def measure(self, prob: Problem, start: Callable[[Problem], object]) -> dict:
+        """Reference error and (steps, error) per converging factor."""
+        eta0 = prob.gd.params.eta_damping
+        saved = prob.get("y_n_gfs")
+        zero = lambda q: q.set("y_n_gfs", np.zeros(2 * q.ntot))
+        try:
+            star = self._solve(prob, eta0, zero, self.target - 4.0)
+            ref = self._solve(prob, eta0, zero, self.target)
+            if star is None or ref is None:
+                return {}
+            u_star = star[1]
+            out = {"reference_steps": ref[0], "reference_error": float(np.max(np.abs(ref[1] - u_star))),
+                   "factors": {}}
+            for f in self.factors:
+                got = self._solve(prob, f * eta0, start, self.target)
+                if got is not None:
+                    out["factors"][f] = (got[0], float(np.max(np.abs(got[1] - u_star))))
+        finally:
+            prob.gd.params.eta_damping = eta0
+            prob.set("y_n_gfs", saved)
+        self.measured += 1
+        return out
+
+    def choose(self, prob: Problem, start: Callable[[Problem], object], start_name: str) -> float:
+        """Factor of the configured eta: fewest steps at no loss of accuracy (1.0 by default)."""
+        k = _eta_key(prob, start_name, self.target)
+        if k not in self.table:
+            m = self.measure(prob, start)
+            best = 1.0
+            if m:
+                ok = {f: v[0] for f, v in m["factors"].items() if v[1] <= self.margin * m["reference_error"]}
+                if ok:
+                    best = min(ok, key=lambda f: (ok[f], f))
+            self.table[k] = {"factor": best,
+                             "reference_steps": m.get("reference_steps"), "reference_error": m.get("reference_error"),
+                             "factors": {f"{f:g}": {"steps": v[0], "error": v[1]}
+                                         for f, v in (m.get("factors") or {}).items()}}
+            if self.cache_path:
+                tmp = self.cache_path + ".tmp"
+                with open(tmp, "w") as fh:
+                    json.dump(self.table, fh, indent=1, sort_keys=True)
+                os.replace(tmp, self.cache_path)
+        return float(self.table[k]["factor"])
+
+
+def _interior(prob: Problem) -> np.ndarray:
+    G0, G1, G2 = prob.Ng
+    m = np.zeros((G2, G1, G0), dtype=bool)
+    m[5:-5, 5:-5, 5:-5] = True
+    return m.ravel()
