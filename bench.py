@@ -246,6 +246,8 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
         try:
             prob.gd.params.eta_damping = f_nn * eta0
             used, _, _ = learned.seed_with_fallback(net, prob)
+            if not used:  # fallback to the zero guess (u = v = 0): the zero guess's damping
+                prob.gd.params.eta_damping = f_zero * eta0
             S = prob.solver(precision=32, log10_target=-10.0, max_steps=20000)
             rn = S.solve()
             S.close()
@@ -273,7 +275,8 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
                                    ("nn_tuned", f_nn * eta0, True, -10.0)):
             P.gd.params.eta_damping = eta
             if nn:
-                learned.seed_with_fallback(net, P)
+                if not learned.seed_with_fallback(net, P)[0]:
+                    P.gd.params.eta_damping = f_zero * eta0
             else:
                 P.set("y_n_gfs", np.zeros(2 * P.ntot))
             S = P.solver(precision=32 if nn else 64, log10_target=tgt, max_steps=40000)
