@@ -244,10 +244,7 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
         the plain FP64 solve at the configured eta; eta restored afterwards."""
         eta0 = prob.gd.params.eta_damping
         try:
-            prob.gd.params.eta_damping = f_nn * eta0
-            used, _, _ = learned.seed_with_fallback(net, prob)
-            if not used:  # fallback to the zero guess (u = v = 0): the zero guess's damping
-                prob.gd.params.eta_damping = f_zero * eta0
+            used, _ = learned.seed_with_tuned_damping(net, prob, f_nn, f_zero)
             S = prob.solver(precision=32, log10_target=-10.0, max_steps=20000)
             rn = S.solve()
             S.close()
@@ -273,11 +270,11 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
     try:
         for name, eta, nn, tgt in (("star", eta0, False, -14.0), ("plain", eta0, False, -10.0),
                                    ("nn_tuned", f_nn * eta0, True, -10.0)):
-            P.gd.params.eta_damping = eta
             if nn:
-                if not learned.seed_with_fallback(net, P)[0]:
-                    P.gd.params.eta_damping = f_zero * eta0
+                P.gd.params.eta_damping = eta0
+                learned.seed_with_tuned_damping(net, P, f_nn, f_zero)
             else:
+                P.gd.params.eta_damping = eta
                 P.set("y_n_gfs", np.zeros(2 * P.ntot))
             S = P.solver(precision=32 if nn else 64, log10_target=tgt, max_steps=40000)
             S.solve()

@@ -73,6 +73,20 @@ def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem, smooth_passes: 
     return used, r0, r1
 
 
+def seed_with_tuned_damping(net: InitialGuessMLP, P: runtime.Problem, f_nn: float, f_zero: float = 1.0,
+                            smooth_passes: int = 2):
+    """The start of the fast path: eta = f_nn x the configured eta (a seed writes v = eta u, so
+    eta is set first), then seed_with_fallback; when the zero guess is kept, eta = f_zero x the
+    configured eta (factors from autotune.EtaTuner for the "nn" and "zero" starts). Returns
+    (used, configured eta); the caller restores P.gd.params.eta_damping after its solve."""
+    eta0 = P.gd.params.eta_damping
+    P.gd.params.eta_damping = f_nn * eta0
+    used, _, _ = seed_with_fallback(net, P, smooth_passes)
+    if not used:  # u = v = 0: any eta is consistent with the state
+        P.gd.params.eta_damping = f_zero * eta0
+    return used, eta0
+
+
 def _solve(S):
     try:
         return S.solve()

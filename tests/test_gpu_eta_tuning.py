@@ -54,11 +54,11 @@ def test_tuned_solve_is_faster_and_no_less_accurate(tuned):
     eta0 = P.gd.params.eta_damping
 
     def solve(eta, nn, target):
-        P.gd.params.eta_damping = eta
         if nn:
-            used, _, _ = learned.seed_with_fallback(net, P)
-            assert used
+            used, eta_conf = learned.seed_with_tuned_damping(net, P, eta / eta0)
+            assert used and eta_conf == eta0 and abs(P.gd.params.eta_damping - eta) <= 1e-12 * eta
         else:
+            P.gd.params.eta_damping = eta
             P.set("y_n_gfs", np.zeros(2 * P.ntot))
         S = P.solver(log10_target=target, max_steps=40000)
         r = S.solve()
