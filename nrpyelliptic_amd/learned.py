@@ -95,10 +95,12 @@ def _solve(S):
 
 
 def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_steps=10000, smooth_passes=2,
-             log=print):
-    """Per configuration: steps and seconds to `target` from the zero guess (FP64) and from the
-    NN guess with the fallback (FP64 or mixed precision), single-configuration solver, and the
-    max |du| between the two converged states."""
+             log=print, eta_factors=None):
+    """Per configuration: steps and seconds to `target` from the zero guess (FP64, configured
+    eta) and from the NN guess with the fallback (FP64 or mixed precision), single-configuration
+    solver, and the max |du| between the two converged states. `eta_factors` = (f_nn, f_zero)
+    from autotune.EtaTuner runs the NN-seeded solve with the damping tuned to its start
+    (seed_with_tuned_damping); None keeps the configured eta."""
     rows = []
     for k, cfg in enumerate(configs):
         P = runtime.Problem(cfg)
@@ -106,9 +108,17 @@ def evaluate(net: InitialGuessMLP, configs, precision=64, target=-10.0, max_step
         P.set("y_n_gfs", np.zeros(2 * P.ntot))
         rz = _solve(S)
         u_zero = P.get("y_n_gfs")[: P.ntot]
-        used, r0, r1 = seed_with_fallback(net, P, smooth_passes)
+        eta0 = P.gd.params.eta_damping
+        if eta_factors is None:
+            used, r0, r1 = seed_with_fallback(net, P, smooth_passes)
+        else:
+            P.set("y_n_gfs", np.zeros(2 * P.ntot))
+            r0 = P.log10_residual()
+            used, _ = seed_with_tuned_damping(net, P, eta_factors[0], eta_factors[1], smooth_passes)
+            r1 = P.log10_residual()
         S2 = P.solver(log10_target=target, precision=precision, max_steps=max_steps)
         rn = _solve(S2)
+        P.gd.params.eta_damping = eta0
         ok = rz is not None and rn is not None and bool(rz.converged and rn.converged)
         du = float(np.max(np.abs(P.get("y_n_gfs")[: P.ntot] - u_zero))) if ok else None
         rows.append({"config": cfg.name, "zero_steps": rz.steps if rz else None, "zero_s": rz.seconds if rz else None,
@@ -162,6 +172,16 @@ def main():
     eval_cfgs = config.random_batch(args.eval_configs, seed=900, grids=False, Nxx=(128, 128, 16))
     rows64 = evaluate(net, eval_cfgs, precision=64, target=args.target, smooth_passes=args.smooth, log=log)
     rows32 = evaluate(net, eval_cfgs, precision=32, target=args.target, smooth_passes=args.smooth, log=log)
+    # the damping tuned to each start on a calibration configuration (seed 901)
+    from .autotune import EtaTuner
+    calib = runtime.Problem(config.random_batch(1, seed=901, grids=False, Nxx=(128, 128, 16))[0])
+    et = EtaTuner(cache_path=None, target=args.target)
+    factors = (et.choose(calib, lambda Q: seed_with_fallback(net, Q, args.smooth), "nn"),
+               et.choose(calib, lambda Q: Q.set("y_n_gfs", np.zeros(2 * Q.ntot)), "zero"))
+    calib.close()
+    log(f"eta factors (nn, zero): {factors}")
+    rows32t = evaluate(net, eval_cfgs, precision=32, target=args.target, smooth_passes=args.smooth, log=log,
+                       eta_factors=factors)
 
     def summary(rows):
         n_all = len(rows)
@@ -179,11 +199,13 @@ def main():
 
     rep = {"train_configs": args.train_configs, "converged_train": stats["converged"], "rows": N,
            "final_loss": float(losses[-50:].mean()),
-           "fp64": summary(rows64), "mixed": summary(rows32)}
+           "fp64": summary(rows64), "mixed": summary(rows32), "eta_factors": list(factors),
+           "mixed_tuned_eta": summary(rows32t)}
     log(json.dumps(rep))
     if args.report:
         with open(args.report, "w") as f:
-            json.dump({"summary": rep, "fp64": rows64, "mixed": rows32, "losses": losses[::10].tolist()}, f)
+            json.dump({"summary": rep, "fp64": rows64, "mixed": rows32, "mixed_tuned_eta": rows32t,
+                       "losses": losses[::10].tolist()}, f)
 
 
 if __name__ == "__main__":

@@ -49,3 +49,31 @@ def test_fallback_keeps_the_zero_guess_when_the_network_is_worse():
     assert not used and r1 >= r0
     assert not np.any(P.get("y_n_gfs"))  # the zero guess is back in place
     P.close()
+
+
+def test_tuned_damping_falls_back_with_the_zero_guess():
+    """A rejected network guess leaves u = v = 0 and the zero guess's damping."""
+    net = InitialGuessMLP(seed=3)
+    p = net.get_params()
+    p[-1] = 8.0
+    net.set_params(p)
+    P = runtime.Problem(config.random_batch(1, seed=900, grids=False, Nxx=(64, 64, 8))[0])
+    eta0 = P.gd.params.eta_damping
+    used, eta_conf = learned.seed_with_tuned_damping(net, P, 2.8, 1.0)
+    assert not used and eta_conf == eta0 and P.gd.params.eta_damping == eta0
+    assert not np.any(P.get("y_n_gfs"))
+    P.close()
+
+
+def test_evaluate_with_tuned_damping():
+    """learned.evaluate with the damping tuned to each start (the factors EtaTuner gives on the
+    128x128x16 grid): the NN-seeded mixed solve of a held-out configuration takes under half
+    the zero guess's steps and lands on the same u, and the configured eta is left in place."""
+    net = InitialGuessMLP(seed=7)
+    net.load(WEIGHTS)
+    cfg = config.random_batch(1, seed=900, grids=False, Nxx=(128, 128, 16))[0]
+    rows = learned.evaluate(net, [cfg], precision=32, eta_factors=(2.8, 1.0), log=lambda s: None)
+    r = rows[0]
+    print("tuned evaluate:", r, flush=True)
+    assert r["converged"] and r["nn_used"] and r["nn_steps"] < 0.5 * r["zero_steps"]
+    assert r["max_du"] <= 1e-7
