@@ -298,6 +298,7 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
             rows.append(nn_solve(Q, rz))
             rows_t.append(tuned_solves(Q, rz))
         Q.close()
+    batch = batched_nn_tuned(net, f_nn, f_zero)
     ok = [r for r in rows if r["converged"]]
     okt = [r for r in rows_t if r["converged"]]
     return {"bench_problem": bench_cfg, "speedup_bench_problem": bench_cfg["plain_s"] / bench_cfg["nn_s"],
@@ -315,7 +316,8 @@ def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
                              "speedup_time": sum(r["plain_s"] for r in okt) / max(sum(r["nn_s"] for r in okt), 1e-12),
                              "speedup_steps": sum(r["plain_steps"] for r in okt) / max(sum(r["nn_steps"] for r in okt), 1),
                              "speedup_time_vs_plain_tuned": sum(r["plain_tuned_s"] for r in okt)
-                             / max(sum(r["nn_s"] for r in okt), 1e-12)}}}
+                             / max(sum(r["nn_s"] for r in okt), 1e-12)},
+                "batched_32_seed900": batch}}
 
 
 def batched_solve(n=64, seed=42):
@@ -338,6 +340,36 @@ def batched_solve(n=64, seed=42):
            "gp_substeps_per_s": pts / sec, "mean_steps": sum(r.steps for _, r in conv) / max(len(conv), 1),
            "autotune_seconds": tune_s, "tiles": sorted({f"{t[0]}x{t[1]}" for t in tiles})}
     B.close()
+    for P in probs:
+        P.close()
+    return out
+
+
+def batched_nn_tuned(net, f_nn, f_zero, n=32, seed=900):
+    """n held-out configurations (seed 900, 128x128x16) relaxed together to log10 L2 <= -10 by
+    the batched FP64 solver: from the zero guess at the configured eta, then from the network's
+    guess at the damping tuned to that start (each configuration's own eta)."""
+    from nrpyelliptic_amd import config, learned, runtime
+    probs = [runtime.Problem(c) for c in config.random_batch(n, seed=seed, grids=False, Nxx=(128, 128, 16))]
+    eta0 = [P.gd.params.eta_damping for P in probs]
+    try:
+        B = runtime.BatchSolver(probs, log10_target=-10.0, max_steps=20000)
+        rz, sz = B.solve()
+        B.close()
+        used = [learned.seed_with_tuned_damping(net, P, f_nn, f_zero)[0] for P in probs]
+        B = runtime.BatchSolver(probs, log10_target=-10.0, max_steps=20000)
+        rn, sn = B.solve()
+        B.close()
+    finally:
+        for P, e in zip(probs, eta0):
+            P.gd.params.eta_damping = e
+    both = [k for k in range(n) if rz[k].converged and rn[k].converged]
+    out = {"configurations": n, "converged_zero": sum(bool(r.converged) for r in rz),
+           "converged_nn_tuned": sum(bool(r.converged) for r in rn), "converged_both": len(both),
+           "nn_used": int(sum(used)),
+           "zero_guess_seconds": sz, "nn_tuned_seconds": sn, "speedup_time": sz / max(sn, 1e-12),
+           "mean_steps_zero": sum(rz[k].steps for k in both) / max(len(both), 1),
+           "mean_steps_nn_tuned": sum(rn[k].steps for k in both) / max(len(both), 1)}
     for P in probs:
         P.close()
     return out
