@@ -76,3 +76,29 @@ def test_tuned_solve_is_faster_and_no_less_accurate(tuned):
     assert n_ref == 3144 and n_nn <= 0.5 * n_ref
     assert e_nn <= e_ref
     P.close()
+
+
+def test_batched_solver_with_tuned_damping(tuned):
+    """Four held-out configurations relaxed together: from the network's guess at the tuned
+    damping (each configuration its own eta) the same configurations converge as from the
+    zero guess, each in under half the steps, to the same u."""
+    net, f_nn, f_zero = tuned
+    probs = [runtime.Problem(c) for c in config.random_batch(4, seed=900, grids=False, Nxx=(128, 128, 16))]
+    eta0 = [P.gd.params.eta_damping for P in probs]
+    B = runtime.BatchSolver(probs, log10_target=-10.0, max_steps=20000)
+    rz, _ = B.solve()
+    B.close()
+    u_zero = [P.get("y_n_gfs")[: P.ntot].copy() for P in probs]
+    for P in probs:
+        learned.seed_with_tuned_damping(net, P, f_nn, f_zero)
+    B = runtime.BatchSolver(probs, log10_target=-10.0, max_steps=20000)
+    rn, _ = B.solve()
+    B.close()
+    print("batched steps zero / tuned:", [(a.steps, b.steps) for a, b in zip(rz, rn)], flush=True)
+    for P, e, a, b, u in zip(probs, eta0, rz, rn, u_zero):
+        P.gd.params.eta_damping = e
+        assert bool(a.converged) == bool(b.converged)
+        if a.converged:
+            assert b.steps <= 0.5 * a.steps
+            assert np.max(np.abs(P.get("y_n_gfs")[: P.ntot] - u)) <= 1e-7
+        P.close()
