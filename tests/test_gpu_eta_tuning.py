@@ -4,7 +4,8 @@ the configured one (that start carries less low-frequency error); on the bench p
 tuned solve converges in at most half the steps of the reference path (zero guess, configured
 eta) and its u is at least as close to a solve converged to 1e-14 as the reference path's.
 The calibration leaves the problem's eta and state as they were, and a second request is
-served from the table."""
+served from the table. The batched solver at the tuned damping converges on the same
+configurations as from the zero guess, in under half the steps, to the same u."""
 import os
 
 import numpy as np
