@@ -1,0 +1,65 @@
+"""The damping tuner's selection rule (autotune.EtaTuner.choose), on synthetic calibration
+results (no GPU): fewest steps among the factors whose error is at most margin x the reference
+path's error; ties go to the smaller factor; 1.0 when no factor qualifies or the calibration
+failed; a faster but less accurate factor is never chosen; results are cached per (grid, start,
+target) and written to the JSON table."""
+import json
+import types
+
+from nrpyelliptic_amd import autotune
+
+
+def _prob(n=(128, 128, 16)):
+    p = types.SimpleNamespace(Nxx0=n[0], Nxx1=n[1], Nxx2=n[2], AMAX=1e6, SINHWAA=0.07)
+    return types.SimpleNamespace(gd=types.SimpleNamespace(params=p))
+
+
+def _tuner(result, **kw):
+    t = autotune.EtaTuner(cache_path=None, **kw)
+    calls = []
+
+    def measure(prob, start):
+        calls.append(prob)
+        return result
+    t.measure = measure
+    return t, calls
+
+
+def test_fastest_factor_within_the_accuracy_margin():
+    res = {"reference_steps": 3144, "reference_error": 1e-8,
+           "factors": {1.0: (2857, 2e-9), 2.0: (1467, 3e-9), 2.8: (1032, 4.7e-9), 4.0: (774, 2.5e-7)}}
+    t, _ = _tuner(res)
+    assert t.choose(_prob(), None, "nn") == 2.8  # 4.0 is faster but 25x less accurate
+
+
+def test_margin_boundary_and_ties_go_to_the_smaller_factor():
+    res = {"reference_steps": 100, "reference_error": 1e-8,
+           "factors": {1.0: (90, 5e-9), 1.4: (80, 5.0000001e-9), 2.0: (90, 1e-9)}}
+    t, _ = _tuner(res)
+    # 1.0 sits exactly on the 0.5 margin (accepted), 1.4 just above it; 1.0 and 2.0 tie at 90
+    assert t.choose(_prob(), None, "x") == 1.0
+    t2, _ = _tuner(res, margin=0.6)
+    assert t2.choose(_prob(), None, "x") == 1.4
+
+
+def test_no_qualifying_factor_or_failed_calibration_keeps_the_configured_eta():
+    t, _ = _tuner({"reference_steps": 100, "reference_error": 1e-8, "factors": {2.0: (50, 1e-7)}})
+    assert t.choose(_prob(), None, "zero") == 1.0
+    t2, _ = _tuner({})
+    assert t2.choose(_prob(), None, "zero") == 1.0
+
+
+def test_cached_per_grid_start_and_target(tmp_path):
+    res = {"reference_steps": 100, "reference_error": 1e-8, "factors": {2.0: (50, 1e-9)}}
+    path = str(tmp_path / "eta.json")
+    t, calls = _tuner(res)
+    t.cache_path = path
+    P = _prob()
+    assert t.choose(P, None, "nn") == 2.0 and t.choose(P, None, "nn") == 2.0 and len(calls) == 1
+    t.choose(P, None, "zero")
+    t.choose(_prob((96, 96, 8)), None, "nn")
+    assert len(calls) == 3
+    table = json.load(open(path))
+    assert len(table) == 3 and all("factor" in v and "reference_error" in v for v in table.values())
+    t3 = autotune.EtaTuner(cache_path=path)  # a new tuner reads the table back
+    assert t3.choose(P, None, "nn") == 2.0 and t3.measured == 0
