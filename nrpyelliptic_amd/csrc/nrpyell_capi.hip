@@ -31,6 +31,8 @@ void set_last_error(const std::string &s) { g_last_error = s; }
 extern "C" const char *nrpyell_last_error(void) { return g_last_error.c_str(); }
 
 static hipStream_t default_stream() { return 0; }
+// drops the cached MoL_step_forward_in_time solvers that use this buffer (defined below)
+static void step_cache_forget_buffer(const void *p);
 
 // ---------------------------------------------------------------------------------------
 // plumbing
@@ -97,6 +99,7 @@ extern "C" void MoL_malloc_y_n_gfs(const paramstruct *params, MoL_gridfunctions_
 extern "C" void MoL_free_memory_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
   ref_call("MoL_free_memory_y_n_gfs", [&]() {
     (void)params;
+    step_cache_forget_buffer(gf->y_n_gfs);
     HIPCHK(hipFree(gf->y_n_gfs));
     gf->y_n_gfs = nullptr;
   });
@@ -119,6 +122,10 @@ extern "C" void MoL_malloc_non_y_n_gfs(const paramstruct *params, MoL_gridfuncti
 extern "C" void MoL_free_memory_non_y_n_gfs(const paramstruct *params, MoL_gridfunctions_struct *gf) {
   ref_call("MoL_free_memory_non_y_n_gfs", [&]() {
     (void)params;
+    step_cache_forget_buffer(gf->y_nplus1_running_total_gfs);
+    step_cache_forget_buffer(gf->k_odd_gfs);
+    step_cache_forget_buffer(gf->k_even_gfs);
+    step_cache_forget_buffer(gf->auxevol_gfs);
     HIPCHK(hipFree(gf->y_nplus1_running_total_gfs));
     HIPCHK(hipFree(gf->k_odd_gfs));
     HIPCHK(hipFree(gf->k_even_gfs));
@@ -154,6 +161,7 @@ extern "C" void rfm_precompute_rfmstruct_define(const paramstruct *params, REAL 
 extern "C" void rfm_precompute_rfmstruct_freemem(const paramstruct *params, rfm_struct *r) {
   ref_call("rfm_precompute_rfmstruct_freemem", [&]() {
     (void)params;
+    step_cache_forget_buffer(r->f0_of_xx0);
     double **all[] = {&r->f0_of_xx0, &r->f0_of_xx0__D0, &r->f0_of_xx0__DD00, &r->f0_of_xx0__DDD000, &r->f1_of_xx1,
                       &r->f1_of_xx1__D1, &r->f1_of_xx1__DD11, &r->f2_of_xx0_xx1, &r->f2_of_xx0_xx1__D0,
                       &r->f2_of_xx0_xx1__D1, &r->f2_of_xx0_xx1__DD00, &r->f2_of_xx0_xx1__DD11, &r->f3_of_xx0,
@@ -250,6 +258,7 @@ extern "C" int nrpyell_ghostmap_check(const paramstruct *params, const bc_struct
 }
 
 extern "C" void nrpyell_bcstruct_free(bc_struct *bc) {
+  step_cache_forget_buffer(bc->inner_bc_array);
   free(bc->inner_bc_array);
   bc->inner_bc_array = nullptr;
   for (int s = 0; s < NGHOSTS * 3; s++) {
@@ -626,15 +635,29 @@ struct nrpyell_solver {
   // FP32 shadow state
   float *y32 = nullptr, *ko32 = nullptr, *ke32 = nullptr, *rt32 = nullptr, *src32 = nullptr;
   hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
-  int graph64_steps = 0, graph32_steps = 0;
-  double graph32_target = 0.0;
+  // what a captured graph bakes into its kernel arguments; a graph is recaptured when any of it
+  // changes between solves (eta_damping or the outer-boundary wave speeds edited in the params)
+  struct GraphKey {
+    double eta = 0, dt = 0, cu = 0, cv = 0, target = 0;
+    int steps = 0;
+    bool same(const GraphKey &o) const {
+      return eta == o.eta && dt == o.dt && cu == o.cu && cv == o.cv && target == o.target && steps == o.steps;
+    }
+  };
+  GraphKey key64, key32;
+  // residual history of the FP32 phase of the last mixed-precision solve (its checks count FP32
+  // steps); the FP64 phase's entries follow it, offset by fp32_offset steps
+  std::vector<int> hist32_step;
+  std::vector<double> hist32_log10;
+  int fp32_offset = 0;
   double *cached_aux = nullptr;
+  int *bad = nullptr;         // set by the fold kernel when the wave speed varies along xx2
   ~nrpyell_solver() {
     if (graph64) (void)hipGraphExecDestroy(graph64);
     if (graph32) (void)hipGraphExecDestroy(graph32);
     (void)hipFree(coef64); (void)hipFree(kc64); (void)hipFree(kc32); (void)hipFree(src64);
     (void)hipFree(geo64); (void)hipFree(geo32); (void)hipFree(w2d); (void)hipFree(partial);
-    (void)hipFree(l2_out); (void)hipFree(dstate);
+    (void)hipFree(l2_out); (void)hipFree(dstate); (void)hipFree(bad);
     if (hstate) (void)hipHostFree(hstate);
     (void)hipFree(y32); (void)hipFree(ko32); (void)hipFree(ke32); (void)hipFree(rt32); (void)hipFree(src32);
     if (stream) (void)hipStreamDestroy(stream);
@@ -696,6 +719,28 @@ struct nrpyell_solver {
       launch_substep<double>(a, s, st);
     }
   }
+  GraphKey graph_key(bool fp32, double target) const {
+    GraphKey k;
+    if (fp32) {
+      const SubstepArgs<float> a = args<float>(1, y32, ko32, ke32, rt32, src32, kc32, geo32);
+      k.eta = a.eta; k.dt = a.dt; k.cu = a.cOB_u; k.cv = a.cOB_v;
+    } else {
+      const SubstepArgs<double> a = args<double>(1, nullptr, nullptr, nullptr, nullptr, src64, kc64, geo64);
+      k.eta = a.eta; k.dt = a.dt; k.cu = a.cOB_u; k.cv = a.cOB_v;
+    }
+    k.target = target;
+    k.steps = opt.steps_per_graph;
+    return k;
+  }
+  // Metric coefficients from the rfm arrays and the c^2-folded coefficient and source tables
+  // from auxevol: the only reads of those arrays. Raises `bad` (cleared first) when the wave
+  // speed varies along xx2.
+  void refresh_tables(hipStream_t st) {
+    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), st));
+    launch_coef(g, &gd->rfmstruct, coef64, nullptr, st);
+    launch_fold(g, gd->gridfuncs.auxevol_gfs, coef64, w2d, kc64, kc32, src64, bad, st);
+    HIPCHK(hipGetLastError());
+  }
   // The fused kernels keep only the interior phi planes current; this fills the ghost planes
   // of y_n from them before the state is handed back.
   void fill_ghosts(hipStream_t st) { launch_fill_ghost_planes<double>(g, gd->gridfuncs.y_n_gfs, 2, st); }
@@ -712,6 +757,9 @@ struct nrpyell_solver {
 };
 
 static void reset_state(nrpyell_solver *s) {
+  s->hist32_step.clear();
+  s->hist32_log10.clear();
+  s->fp32_offset = 0;
   memset(s->hstate, 0, sizeof(SolverDeviceState));
   HIPCHK(hipMemcpyAsync(s->dstate, s->hstate, sizeof(SolverDeviceState), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -795,7 +843,6 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     const long n2 = (long)g.G0 * g.G1;
     s->coef64 = dalloc<double>(5 * n2);
-    launch_coef(g, &gd->rfmstruct, s->coef64, nullptr, s->stream);
     std::vector<double> geo((size_t)6 * g.G1 * 4);
     host_outer_geometry(&gd->params, gd->xx, geo.data());
     s->geo64 = upload(geo.data(), geo.size());
@@ -808,12 +855,10 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     s->kc64 = dalloc<double>(6 * n2);
     s->kc32 = dalloc<float>(6 * n2);
     s->src64 = dalloc<double>(2 * g.ntot);
-    DevBuf<int> badb(1);
-    int *bad = badb.p;
-    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), s->stream));
-    launch_fold(g, gd->gridfuncs.auxevol_gfs, s->coef64, s->w2d, s->kc64, s->kc32, s->src64, bad, s->stream);
+    s->bad = dalloc<int>(1);
+    s->refresh_tables(s->stream);
     int hbad = 0;
-    HIPCHK(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(&hbad, s->bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     if (hbad) {
       set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
@@ -913,10 +958,12 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       ensure_fp32(s);
       launch_convert_d2f(2 * nt, f.y_n_gfs, s->y32, s->stream);
       launch_convert_d2f(2 * nt, s->src64, s->src32, s->stream);
-      if (!s->graph32 || s->graph32_target != s->opt.fp32_switch_log10) {
+      const nrpyell_solver::GraphKey k32 = s->graph_key(true, s->opt.fp32_switch_log10);
+      if (!s->graph32 || !s->key32.same(k32)) {
         if (s->graph32) (void)hipGraphExecDestroy(s->graph32);
+        s->graph32 = nullptr;
         s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
-        s->graph32_target = s->opt.fp32_switch_log10;
+        s->key32 = k32;
       }
       reset_state(s);
       // The relaxation residual oscillates, so the FP32 phase ends on the lowest value seen so
@@ -950,16 +997,29 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       }
       steps = s->hstate->steps;
       res->fp32_steps = steps;
+      std::vector<int> h32s(s->hstate->hist_step, s->hstate->hist_step + s->hstate->nhist);
+      std::vector<double> h32l(s->hstate->hist_log10, s->hstate->hist_log10 + s->hstate->nhist);
       launch_convert_f2d(2 * nt, s->y32, f.y_n_gfs, s->stream);
       reset_state(s);
       s->check(s->stream, false, 0, s->opt.log10_target);
       read_state(s);
+      // the FP64 phase records its own check at the switch step
+      while (!h32s.empty() && h32s.back() >= steps) {
+        h32s.pop_back();
+        h32l.pop_back();
+      }
+      s->hist32_step = std::move(h32s);
+      s->hist32_log10 = std::move(h32l);
     }
+    const std::vector<int> keep_s = s->hist32_step;
+    const std::vector<double> keep_l = s->hist32_log10;
     if (s->hstate->done == 0) {
-      if (!s->graph64 || s->graph64_steps != s->opt.steps_per_graph) {
+      const nrpyell_solver::GraphKey k64 = s->graph_key(false, s->opt.log10_target);
+      if (!s->graph64 || !s->key64.same(k64)) {
         if (s->graph64) (void)hipGraphExecDestroy(s->graph64);
+        s->graph64 = nullptr;
         s->graph64 = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
-        s->graph64_steps = s->opt.steps_per_graph;
+        s->key64 = k64;
       }
       reset_state(s);
       // every step's stage 1 checks the state it starts from: a converged solve took `steps`
@@ -974,6 +1034,9 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
     s->fill_ghosts(s->stream);
     HIPCHK(hipStreamSynchronize(s->stream));
     const auto t1 = std::chrono::steady_clock::now();
+    s->hist32_step = keep_s;
+    s->hist32_log10 = keep_l;
+    s->fp32_offset = res->fp32_steps;
     res->steps = steps;
     res->log10_l2 = s->hstate->last_log10;
     res->seconds = std::chrono::duration<double>(t1 - t0).count();
@@ -1030,10 +1093,14 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
 extern "C" int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2) {
   return api_call([&]() -> int {
     read_state(s);
-    const int n = s->hstate->nhist < max ? s->hstate->nhist : max;
-    for (int k = 0; k < n; k++) {
-      steps[k] = s->hstate->hist_step[k];
-      log10_l2[k] = s->hstate->hist_log10[k];
+    int n = 0;
+    for (size_t k = 0; k < s->hist32_step.size() && n < max; k++, n++) {
+      steps[n] = s->hist32_step[k];
+      log10_l2[n] = s->hist32_log10[k];
+    }
+    for (int k = 0; k < s->hstate->nhist && n < max; k++, n++) {
+      steps[n] = s->fp32_offset + s->hstate->hist_step[k];
+      log10_l2[n] = s->hstate->hist_log10[k];
     }
     return n;
   });
@@ -1085,33 +1152,78 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
 // ---------------------------------------------------------------------------------------
 // MoL_step_forward_in_time: one RK4 step through a solver cached per griddata.
 // ---------------------------------------------------------------------------------------
+// The solver reads, at creation, the grid (dimensions, SinhSymTP parameters and the host xx
+// arrays: outer-BC geometry, ghost map check) and the buffer addresses; an entry is reused only
+// while all of them are unchanged. The rfm and auxevol arrays are read again on every call, as
+// the reference's MoL step reads them, so rewriting them in place (new punctures through
+// auxevol_gfs_all_points, or any other writer) takes effect at the next step. The free
+// functions drop the entries that point at the memory they release.
 struct StepCacheEntry {
   nrpyell_solver *s;
-  const double *yn, *aux;
+  MoL_gridfunctions_struct gf;
   const void *rfm0;
   const innerpt_bc_struct *inner;
+  int ninner;
+  std::vector<double> grid;
 };
 static std::map<const griddata_struct *, StepCacheEntry> g_step_cache;
+
+static std::vector<double> step_grid_key(const griddata_struct *gd) {
+  const paramstruct &p = gd->params;
+  std::vector<double> k = {(double)p.Nxx0, (double)p.Nxx1, (double)p.Nxx2, p.AMAX, p.SINHWAA, p.bScale,
+                           p.dxx0, p.dxx1, p.dxx2, p.xxmin0, p.xxmin1, p.xxmin2};
+  const int G[3] = {p.Nxx_plus_2NGHOSTS0, p.Nxx_plus_2NGHOSTS1, p.Nxx_plus_2NGHOSTS2};
+  for (int d = 0; d < 3; d++) k.insert(k.end(), gd->xx[d], gd->xx[d] + G[d]);
+  return k;
+}
+
+template <typename Pred>
+static void step_cache_evict(Pred pred) {
+  for (auto it = g_step_cache.begin(); it != g_step_cache.end();) {
+    if (pred(it->second)) {
+      nrpyell_solver_destroy(it->second.s);
+      it = g_step_cache.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+static void step_cache_forget_buffer(const void *p) {
+  if (!p) return;
+  step_cache_evict([p](const StepCacheEntry &e) {
+    return e.gf.y_n_gfs == p || e.gf.k_odd_gfs == p || e.gf.k_even_gfs == p ||
+           e.gf.y_nplus1_running_total_gfs == p || e.gf.auxevol_gfs == p || e.rfm0 == p || e.inner == p;
+  });
+}
 
 extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL dt) {
   ref_call("MoL_step_forward_in_time", [&]() {
     nrpyell_solver *s = nullptr;
+    const std::vector<double> key = step_grid_key(griddata);
+    const MoL_gridfunctions_struct &gf = griddata->gridfuncs;
     {
       auto it = g_step_cache.find(griddata);
-      if (it != g_step_cache.end() && it->second.yn == griddata->gridfuncs.y_n_gfs &&
-          it->second.aux == griddata->gridfuncs.auxevol_gfs && it->second.rfm0 == griddata->rfmstruct.f0_of_xx0 &&
-          it->second.inner == griddata->bcstruct.inner_bc_array) {
-        s = it->second.s;
-      } else {
-        if (it != g_step_cache.end()) {
-          nrpyell_solver_destroy(it->second.s);
+      if (it != g_step_cache.end()) {
+        const StepCacheEntry &e = it->second;
+        if (e.gf.y_n_gfs == gf.y_n_gfs && e.gf.k_odd_gfs == gf.k_odd_gfs && e.gf.k_even_gfs == gf.k_even_gfs &&
+            e.gf.y_nplus1_running_total_gfs == gf.y_nplus1_running_total_gfs && e.gf.auxevol_gfs == gf.auxevol_gfs &&
+            e.rfm0 == griddata->rfmstruct.f0_of_xx0 && e.inner == griddata->bcstruct.inner_bc_array &&
+            e.ninner == griddata->bcstruct.bc_info.num_inner_boundary_points && e.grid == key) {
+          s = e.s;
+        } else {
+          nrpyell_solver_destroy(e.s);
           g_step_cache.erase(it);
         }
+      }
+      if (!s) {
         nrpyell_solve_options o;
         nrpyell_default_options(&o);
         if (nrpyell_solver_create(griddata, dt, &o, &s) != NRPYELL_OK) fatal("MoL_step_forward_in_time", g_last_error);
-        g_step_cache[griddata] = {s, griddata->gridfuncs.y_n_gfs, griddata->gridfuncs.auxevol_gfs,
-                                  griddata->rfmstruct.f0_of_xx0, griddata->bcstruct.inner_bc_array};
+        g_step_cache[griddata] = {s, gf, griddata->rfmstruct.f0_of_xx0, griddata->bcstruct.inner_bc_array,
+                                  griddata->bcstruct.bc_info.num_inner_boundary_points, key};
+      } else {
+        s->refresh_tables(s->stream);
       }
     }
     s->dt = dt;
@@ -1119,7 +1231,13 @@ extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL d
     s->step64(s->stream);
     s->fill_ghosts(s->stream);
     HIPCHK(hipGetLastError());
+    int hbad = 0;
+    HIPCHK(hipMemcpyAsync(&hbad, s->bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
+    if (hbad)
+      fatal("MoL_step_forward_in_time",
+            "the wave speed gridfunction varies along xx2; the fused step needs c = c(xx0, xx1) as "
+            "wavespeed_gf_all_points sets it");
   });
 }
 

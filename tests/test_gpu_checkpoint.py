@@ -51,3 +51,42 @@ def test_residual_log_format(tmp_path):
         assert int(a) == k and float(b) == float("%10.4e" % (k * P.dt)) and float(c) == v
     S.close()
     P.close()
+
+
+def test_mixed_precision_log_spans_both_phases():
+    """After an FP32 -> FP64 solve the history holds the FP32 phase's checks and then the FP64
+    phase's, numbered by the steps taken since the start: steps strictly increase and the log
+    ends at the step the solve stopped on."""
+    P = runtime.Problem(config.axisymmetric_spinning())
+    S = P.solver(precision=32, log10_target=-8.0, max_steps=5000, check_every=10, fp32_switch_log10=-5.0)
+    r = S.solve()
+    assert r.converged and 0 < r.fp32_steps < r.steps
+    steps, vals = S.history()
+    assert np.all(np.diff(steps) > 0)
+    assert steps[-1] == r.steps and vals[-1] == r.log10_l2
+    assert np.any(steps < r.fp32_steps) and np.any(steps > r.fp32_steps)
+    S.close()
+    P.close()
+
+
+def test_changed_damping_between_solves_is_used():
+    """eta_damping edited in the params between two solves on one handle takes effect (the
+    captured graph is rebuilt): the second solve equals a fresh solver's at the new eta."""
+    cfg = config.axisymmetric_spinning()
+    P = runtime.Problem(cfg)
+    S = P.solver(log10_target=-7.0, max_steps=5000)
+    S.solve()
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    P.gd.params.eta_damping = 2.0 * cfg.eta
+    r2 = S.solve()
+    u2 = P.get("y_n_gfs")
+    S.close()
+    P.close()
+    Q = runtime.Problem(cfg)
+    Q.gd.params.eta_damping = 2.0 * cfg.eta
+    T = Q.solver(log10_target=-7.0, max_steps=5000)
+    r3 = T.solve()
+    assert r2.steps == r3.steps
+    assert np.array_equal(u2, Q.get("y_n_gfs"))
+    T.close()
+    Q.close()
