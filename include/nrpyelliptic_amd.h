@@ -222,6 +222,20 @@ void print_puncture_parameters(const paramstruct *NRPYELL_RESTRICT params);
 #define NRPYELL_ENOTCONVERGED -4
 #define NRPYELL_ENAN -5
 
+/* L2_norm_of_gf that allocates nothing. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code around HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and result as L2_norm_of_gf (the log10 L2
+ * goes to *log10_l2); the volume weights are computed on the device each call (device
+ * exp/sin/cos: agrees with L2_norm_of_gf to rounding, not bit for bit). Every device buffer it
+ * uses lives in one fixed array of NRPYELL_NOALLOC_SCRATCH_BYTES per GPU, held by the library
+ * until the process ends; a grid that needs more
+ * ((G0 + G1 + G0*G1 + ceil(Nxx0/64)*ceil(Nxx1/4)*Nxx2 + 1) doubles, G = Nxx + 10) is refused with
+ * NRPYELL_EINVAL. Calls that overlap on one GPU share that array and race: serialise them.
+ * Synchronous; NRPYELL_OK, NRPYELL_EINVAL or NRPYELL_EHIP. */
+#define NRPYELL_NOALLOC_SCRATCH_BYTES (2u << 20)
+int nrpyell_L2_norm_of_gf_noalloc(const paramstruct *params, const int gf_index, const REAL integration_radius,
+                                  REAL *xx[3], const REAL *in_gf, REAL *log10_l2);
+
 /* Library/device plumbing used by the Python host layer and tests. */
 int nrpyell_device_synchronize(void);
 int nrpyell_malloc_device(void **ptr, size_t bytes);

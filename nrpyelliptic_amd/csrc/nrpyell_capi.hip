@@ -539,6 +539,71 @@ extern "C" REAL L2_norm_of_gf(const paramstruct *params, const int gf_index, con
   });
 }
 
+// ---------------------------------------------------------------------------------------
+// Allocation-free L2 norm. Every device buffer it needs (coordinate values, weight table,
+// partial sums, result) lives in `scratch`, one fixed array per GPU that the library holds
+// until the process ends. Its device address is looked up once per device; host code uses
+// that address, never the symbol itself.
+// ---------------------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NRPYELL_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+#define NRPYELL_MAX_DEVICES 64
+static std::mutex g_scratch_mu;
+static double *g_scratch_ptr[NRPYELL_MAX_DEVICES];
+
+static int scratch_on_current_device(double **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NRPYELL_MAX_DEVICES) {
+    set_last_error("device index beyond the library's table of scratch addresses");
+    return NRPYELL_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = (double *)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return NRPYELL_OK;
+}
+
+extern "C" int nrpyell_L2_norm_of_gf_noalloc(const paramstruct *params, const int gf_index,
+                                             const REAL integration_radius, REAL *xx[3], const REAL *in_gf,
+                                             REAL *log10_l2) {
+  return api_call([&]() -> int {
+    if (!params || !xx || !xx[0] || !xx[1] || !in_gf || !log10_l2 || gf_index < 0) {
+      set_last_error("nrpyell_L2_norm_of_gf_noalloc: null argument or negative gf_index");
+      return NRPYELL_EINVAL;
+    }
+    const GridDims g = dims_from_params(params);
+    // scratch layout (doubles): xx0 [G0] | xx1 [G1] | weights [G0*G1] | partials | result
+    const long nw = (long)g.G0 * g.G1, npart = l2_partial_blocks(g);
+    const long need = g.G0 + g.G1 + nw + npart + 1;
+    if (need * (long)sizeof(double) > (long)NRPYELL_NOALLOC_SCRATCH_BYTES) {
+      set_last_error("nrpyell_L2_norm_of_gf_noalloc: grid needs " + std::to_string(need * sizeof(double)) +
+                     " bytes of scratch, more than the " + std::to_string(NRPYELL_NOALLOC_SCRATCH_BYTES) +
+                     " it has; use L2_norm_of_gf");
+      return NRPYELL_EINVAL;
+    }
+    double *sp = nullptr;
+    const int rc = scratch_on_current_device(&sp);
+    if (rc != NRPYELL_OK) return rc;
+    double *d_xx0 = sp, *d_xx1 = d_xx0 + g.G0, *d_w = d_xx1 + g.G1, *d_part = d_w + nw, *d_out = d_part + npart;
+    const hipStream_t st = default_stream();
+    HIPCHK(hipMemcpyAsync(d_xx0, xx[0], g.G0 * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d_xx1, xx[1], g.G1 * sizeof(double), hipMemcpyHostToDevice, st));
+    const L2WeightParams q = {params->AMAX, params->SINHWAA, params->bScale, params->dxx0,
+                              params->dxx1, params->dxx2, integration_radius};
+    launch_l2_weights(g, q, d_xx0, d_xx1, d_w, st);
+    const long n = launch_l2_partial(g, in_gf + (long)gf_index * g.ntot, d_w, d_part, st);
+    launch_l2_finalize_weights(d_part, n, d_w, nw, g.N2, d_out, st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(log10_l2, d_out, sizeof(double), hipMemcpyDeviceToHost));
+    return NRPYELL_OK;
+  });
+}
+
 // gridfunction_z_axis.c:6-52: z and gf along the +z axis (i1 = NG), -z axis (i1 = G1-NG-1)
 // and the i0 = NG line, at i2 = G2/2, "%.17e %.17e" per line.
 extern "C" void gridfunction_z_axis(const paramstruct *params, const int gf_index, REAL *xx[3], const REAL *in_gf,

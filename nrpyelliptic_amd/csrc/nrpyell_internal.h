@@ -156,6 +156,15 @@ void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, co
 long launch_residual_l2(const GridDims &g, const void *u, bool u_is_float, const double *aux, const double *coef,
                         const double *w2d, double *partial, const int *done, double *res_out, hipStream_t st);
 long launch_l2_partial(const GridDims &g, const double *gf, const double *w2d, double *partial, hipStream_t st);
+long l2_partial_blocks(const GridDims &g);  // partial sums launch_l2_partial writes
+// Grid constants of the L2 volume weights computed on the device.
+struct L2WeightParams {
+  double AMAX, SINHWAA, bScale, dxx0, dxx1, dxx2, radius;
+};
+void launch_l2_weights(const GridDims &g, const L2WeightParams &q, const double *xx0, const double *xx1, double *w2d,
+                       hipStream_t st);
+void launch_l2_finalize_weights(const double *partial, long npart, const double *w2d, long nw, int n2,
+                                double *log10_out, hipStream_t st);
 void launch_l2_finalize(const double *partial, long npart, double volume, SolverDeviceState *dst, int steps_per_check,
                         double target, double *log10_out, hipStream_t st);
 void launch_inner_gather(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs,

@@ -275,6 +275,67 @@ __global__ void __launch_bounds__(256) l2_finalize_kernel(const double *partial,
   }
 }
 
+// L2 volume weights on the device (the allocation-free L2 entry point): per (i0,i1) the dV of
+// host_l2_weights from the coordinate values xx0[i0], xx1[i1], zero outside the interior and
+// outside the integration radius. Same expression order as the host; exp/sin/cos are the
+// device's, so a weight can differ from the host's in the last bit.
+__global__ void __launch_bounds__(256) l2_weights_kernel(GridDims g, L2WeightParams q, const double *xx0,
+                                                         const double *xx1, double *w2d) {
+  const long k = blockIdx.x * 256L + threadIdx.x;
+  if (k >= (long)g.G0 * g.G1) return;
+  const int i0 = (int)(k % g.G0), i1 = (int)(k / g.G0);
+  double dV = 0.0;
+  if (i0 >= NG && i0 < NG + g.N0 && i1 >= NG && i1 < NG + g.N1) {
+    const double w = q.SINHWAA, A = q.AMAX, b = q.bScale;
+    const double D = exp(1.0 / w) - exp(-1.0 / w);
+    const double ep = exp(xx0[i0] / w), em = exp(-xx0[i0] / w);
+    const double f0 = A * (ep - em) / D, f0p = A * (ep + em) / (w * D);
+    const double s = sin(xx1[i1]), c = cos(xx1[i1]);
+    const double f3sq = f0 * f0 + b * b, f2sq = f0 * f0 + b * b * s * s;
+    const double r = sqrt(s * s * f0 * f0 + f3sq * c * c);
+    if (r < q.radius) dV = f0 * f0p * f2sq * fabs(s) / sqrt(f3sq) * q.dxx0 * q.dxx1 * q.dxx2;
+  }
+  w2d[k] = dV;
+}
+
+// Fixed-order sum of the partials and of the weight table by one block:
+// log10(1e-16 + sqrt(S / V)) with V = N2 * sum(w2d) (every i2 plane carries the same weights).
+__global__ void __launch_bounds__(256) l2_finalize_weights_kernel(const double *partial, long npart,
+                                                                  const double *w2d, long nw, int n2,
+                                                                  double *log10_out) {
+  double acc = 0.0, vol = 0.0;
+  for (long i = threadIdx.x; i < npart; i += 256) acc += partial[i];
+  for (long i = threadIdx.x; i < nw; i += 256) vol += w2d[i];
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_down(acc, o, 64);
+    vol += __shfl_down(vol, o, 64);
+  }
+  __shared__ double sw[4], sv[4];
+  if ((threadIdx.x & 63) == 0) {
+    sw[threadIdx.x >> 6] = acc;
+    sv[threadIdx.x >> 6] = vol;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double S = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+    const double V = ((sv[0] + sv[1]) + (sv[2] + sv[3])) * n2;
+    *log10_out = log10(1e-16 + sqrt(S / V));
+  }
+}
+
+void launch_l2_weights(const GridDims &g, const L2WeightParams &q, const double *xx0, const double *xx1, double *w2d,
+                       hipStream_t st) {
+  const long n = (long)g.G0 * g.G1;
+  hipLaunchKernelGGL(l2_weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, q, xx0, xx1, w2d);
+}
+
+void launch_l2_finalize_weights(const double *partial, long npart, const double *w2d, long nw, int n2,
+                                double *log10_out, hipStream_t st) {
+  hipLaunchKernelGGL(l2_finalize_weights_kernel, dim3(1), dim3(256), 0, st, partial, npart, w2d, nw, n2, log10_out);
+}
+
+long l2_partial_blocks(const GridDims &g) { return (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2; }
+
 long launch_residual_l2(const GridDims &g, const void *u, bool u_is_float, const double *aux, const double *coef,
                         const double *w2d, double *partial, const int *done, double *res_out, hipStream_t st) {
   dim3 grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2), blk(256);

@@ -101,6 +101,8 @@ def lib() -> C.CDLL:
     L.nrpyell_apply_bcs_sommerfeld.argtypes = [C.POINTER(abi.paramstruct), C.POINTER(abi.bc_struct), abi.XX3,
                                                C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                                abi.REALP, abi.REALP]
+    L.nrpyell_L2_norm_of_gf_noalloc.argtypes = [C.POINTER(abi.paramstruct), C.c_int, C.c_double, abi.XX3, abi.REALP,
+                                                C.POINTER(C.c_double)]
     L.compute_TRC.argtypes = [C.c_double] * 6
     L.compute_TRC.restype = C.c_double
     L.set_optimum_eta_damping.argtypes = [C.c_double, C.c_double, C.POINTER(abi.paramstruct)]
@@ -282,6 +284,15 @@ class Problem:
     def l2(self, gf: np.ndarray, radius: float = 100.0, gf_index: int = 0) -> float:
         d = DeviceArray(gf.size).upload(gf)
         return lib().L2_norm_of_gf(C.byref(self.gd.params), gf_index, radius, self.gd.xx, d.as_real_p())
+
+    def l2_noalloc(self, gf: np.ndarray, radius: float = 100.0, gf_index: int = 0) -> float:
+        """L2_norm_of_gf through nrpyell_L2_norm_of_gf_noalloc (no device allocation inside the
+        call; the upload of gf here is the caller's)."""
+        d = DeviceArray(gf.size).upload(gf)
+        out = C.c_double()
+        check(lib().nrpyell_L2_norm_of_gf_noalloc(C.byref(self.gd.params), gf_index, radius, self.gd.xx,
+                                                  d.as_real_p(), C.byref(out)), "L2_norm_of_gf_noalloc")
+        return out.value
 
     def log10_residual(self, radius: float = 100.0) -> float:
         """residual_all_points + L2_norm_of_gf on y_n, as main.c:146-151 does each step."""
