@@ -324,7 +324,8 @@ def batched_solve(n=64, seed=42):
     """The batched configuration: n random punctures (seed 42: q log-uniform in [1,8], D in
     [6,20], Newtonian tangential momenta, random spins; grids mixing Nxx0 in {96,128,160},
     Nxx1 in {96,128}, Nxx2 in {8,16}) relaxed together to log10 L2 <= -10 with tile shapes from
-    the autotuner; throughput counts the substeps of the configurations that converged."""
+    the autotuner; members that diverge at CFL 0.7 are relaxed again from the zero guess at the
+    fallback CFL (runtime.relax_batch). Throughput counts every substep run."""
     from nrpyelliptic_amd import config, runtime
     from nrpyelliptic_amd.autotune import TileTuner
     probs = [runtime.Problem(c) for c in config.random_batch(n, seed=seed)]
@@ -332,14 +333,17 @@ def batched_solve(n=64, seed=42):
     t0 = time.perf_counter()
     tiles = tuner.choose_all(probs)
     tune_s = time.perf_counter() - t0
-    B = runtime.BatchSolver(probs, tiles=tiles, log10_target=-10.0, max_steps=20000)
-    res, sec = B.solve()
+    res, sec, info = runtime.relax_batch(probs, tiles=tiles, log10_target=-10.0, max_steps=30000)
     conv = [(P, r) for P, r in zip(probs, res) if r.converged]
-    pts = sum(4.0 * P.cfg.Nxx[0] * P.cfg.Nxx[1] * P.cfg.Nxx[2] * r.steps for P, r in conv)
-    out = {"configurations": n, "converged": len(conv), "diverged_or_stopped": n - len(conv), "seconds": sec,
+    # every substep run counts, the diverged first attempts of the fallback members included
+    pts = sum(4.0 * P.cfg.Nxx[0] * P.cfg.Nxx[1] * P.cfg.Nxx[2] * (r.steps + i["first_steps"])
+              for P, r, i in zip(probs, res, info))
+    fell = [k for k in range(n) if info[k]["fallback"]]
+    out = {"configurations": n, "converged": len(conv), "not_converged": n - len(conv), "seconds": sec,
            "gp_substeps_per_s": pts / sec, "mean_steps": sum(r.steps for _, r in conv) / max(len(conv), 1),
+           "fallback_cfl": runtime.FALLBACK_CFL, "fallback_members": fell,
+           "fallback_first_attempt_steps": [info[k]["first_steps"] for k in fell],
            "autotune_seconds": tune_s, "tiles": sorted({f"{t[0]}x{t[1]}" for t in tiles})}
-    B.close()
     for P in probs:
         P.close()
     return out

@@ -274,7 +274,19 @@ typedef struct {
   double outer_bc_speed;  /* wave speed of the outer BC; 0 = the BC's default: the grid's
                              uu/vv_wavespeed_at_OB (radiation), NRPYELL_ET_SOMMERFELD_SPEED
                              (Sommerfeld)                                                   */
+  int stop_rule;          /* NRPYELL_STOP_EVERY_STEP: the residual of the state before every
+                             step is checked and the solve stops at log10 L2 <= log10_target
+                             (max_steps caps the steps);
+                             NRPYELL_STOP_ET_THORN: the Einstein Toolkit thorn's loop
+                             (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:
+                             181-223): with k = check_every, the checks after steps 1, 1+k,
+                             1+2k, ... and after step max_steps+1 (max_steps = the thorn's
+                             max_iterations) decide, the stop needs log10 L2 < log10_target,
+                             and the solve ends after step max_steps+1 if none passes (status
+                             NRPYELL_STATUS_MAX_STEPS). FP64 single-configuration solves only. */
 } nrpyell_solve_options;
+#define NRPYELL_STOP_EVERY_STEP 0
+#define NRPYELL_STOP_ET_THORN 1
 
 #define NRPYELL_OUTER_BC_RADIATION 0
 #define NRPYELL_OUTER_BC_SOMMERFELD6 1
@@ -302,7 +314,12 @@ typedef struct {
   double log10_l2_initial;/* log10 L2 of the residual of the initial guess                  */
   double seconds;         /* wall time of the solve (device-synchronised)                   */
   int converged;
+  int status;             /* NRPYELL_STATUS_*: why the solve ended                          */
 } nrpyell_solve_result;
+#define NRPYELL_STATUS_MAX_STEPS 0  /* still relaxing when max_steps was reached            */
+#define NRPYELL_STATUS_CONVERGED 1  /* log10 L2 <= log10_target                              */
+#define NRPYELL_STATUS_NAN 2        /* the residual became NaN                               */
+#define NRPYELL_STATUS_DIVERGED 3   /* the residual rose 4 decades above its lowest value    */
 
 void nrpyell_default_options(nrpyell_solve_options *opt);
 /* Build device tables for griddata (params, xx, bcstruct, rfmstruct, gridfuncs must be set

@@ -826,6 +826,10 @@ static void reset_state(nrpyell_solver *s) {
   s->hist32_log10.clear();
   s->fp32_offset = 0;
   memset(s->hstate, 0, sizeof(SolverDeviceState));
+  if (s->opt.stop_rule == NRPYELL_STOP_ET_THORN) {
+    s->hstate->rule_every = s->opt.check_every;
+    s->hstate->rule_last = s->opt.max_steps + 1;
+  }
   HIPCHK(hipMemcpyAsync(s->dstate, s->hstate, sizeof(SolverDeviceState), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
 }
@@ -848,6 +852,7 @@ extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
   o->planes = 0;
   o->outer_bc = NRPYELL_OUTER_BC_RADIATION;
   o->outer_bc_speed = 0.0;
+  o->stop_rule = NRPYELL_STOP_EVERY_STEP;
 }
 
 extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell_solve_options *opt,
@@ -859,6 +864,14 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     s->g = dims_from_params(&gd->params);
     s->dt = dt;
     if (opt) s->opt = *opt; else nrpyell_default_options(&s->opt);
+    if (s->opt.stop_rule != NRPYELL_STOP_EVERY_STEP && s->opt.stop_rule != NRPYELL_STOP_ET_THORN) {
+      set_last_error("stop_rule must be NRPYELL_STOP_EVERY_STEP or NRPYELL_STOP_ET_THORN");
+      return NRPYELL_EINVAL;
+    }
+    if (s->opt.stop_rule == NRPYELL_STOP_ET_THORN && (s->opt.precision != 64 || s->opt.max_steps < 0)) {
+      set_last_error("the thorn's stopping rule needs precision 64 and max_steps >= 0");
+      return NRPYELL_EINVAL;
+    }
     if (s->opt.check_every < 1) s->opt.check_every = 1;
     if (s->opt.steps_per_graph < s->opt.check_every) s->opt.steps_per_graph = s->opt.check_every;
     s->opt.steps_per_graph -= s->opt.steps_per_graph % s->opt.check_every;
@@ -1088,11 +1101,13 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       }
       reset_state(s);
       // every step's stage 1 checks the state it starts from: a converged solve took `steps`
-      // steps (the count when its check passed), an unconverged one all `nchecks` launched
+      // steps (the count when its check passed), an unconverged one all `nchecks` launched.
+      // Under the thorn's rule the device itself ends the solve after step max_steps + 1.
+      const bool thorn = s->opt.stop_rule == NRPYELL_STOP_ET_THORN;
       while (true) {
         HIPCHK(hipGraphLaunch(s->graph64, s->stream));
         read_state(s);
-        if (s->hstate->done || steps + s->hstate->nchecks >= s->opt.max_steps) break;
+        if (s->hstate->done || (!thorn && steps + s->hstate->nchecks >= s->opt.max_steps)) break;
       }
       steps += s->hstate->done ? s->hstate->steps : s->hstate->nchecks;
     }
@@ -1106,6 +1121,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
     res->log10_l2 = s->hstate->last_log10;
     res->seconds = std::chrono::duration<double>(t1 - t0).count();
     res->converged = s->hstate->done == 1;
+    res->status = s->hstate->done == 4 ? NRPYELL_STATUS_MAX_STEPS : s->hstate->done;
     if (s->hstate->done == 2) { set_last_error("residual became NaN"); return NRPYELL_ENAN; }
     if (s->hstate->done == 3) {
       set_last_error("residual diverged (4 decades above its lowest value)");
@@ -1435,6 +1451,10 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
     b->n = n;
     if (opt) b->opt = *opt; else nrpyell_default_options(&b->opt);
     b->opt.precision = 64;
+    if (b->opt.stop_rule != NRPYELL_STOP_EVERY_STEP) {
+      set_last_error("the batched solver stops every configuration by NRPYELL_STOP_EVERY_STEP");
+      return NRPYELL_EINVAL;
+    }
     for (int k = 0; k < n; k++) {
       nrpyell_solve_options o = b->opt;
       o.tile_i0 = tile_i0 ? tile_i0[k] : b->opt.tile_i0;
@@ -1512,6 +1532,7 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
       nrpyell_solve_result &r = res[k];
       memset(&r, 0, sizeof(r));
       r.converged = b->h_heads[4 * k] == 1.0;
+      r.status = (int)b->h_heads[4 * k];
       // converged, NaN or diverged: the steps when its check decided; running: all launched
       r.steps = b->h_heads[4 * k] != 0.0 ? (int)b->h_heads[4 * k + 1] : launched;
       r.log10_l2 = b->h_heads[4 * k + 2];

@@ -12,13 +12,19 @@
 
 // Device-resident convergence state of one solve.
 struct SolverDeviceState {
-  int done;          // 0 running, 1 converged, 2 NaN, 3 diverged
+  int done;          // 0 running, 1 converged, 2 NaN, 3 diverged, 4 step cap of the thorn's rule
   int steps;         // RK4 steps completed at the last residual check
   int nhist;
   int nchecks;       // residual checks made by fused stage-1 kernels
   double last_log10;
   int have_best;
   double best_log10;  // lowest residual seen
+  // stopping rule: rule_every == 0: every check decides, stop at log10 L2 <= target (the
+  // resident solver's rule); rule_every = k >= 1: the Einstein Toolkit thorn's rule -- only the
+  // checks after steps 1, 1+k, 1+2k, ... and after step rule_last decide, the stop needs
+  // log10 L2 < target, and the solve ends after step rule_last in any case
+  int rule_every;
+  int rule_last;
   int hist_step[NRPYELL_HIST];
   double hist_log10[NRPYELL_HIST];
 };
@@ -54,6 +60,17 @@ __device__ inline void record_check(SolverDeviceState *st, double l, double targ
   if (!st->have_best || l < st->best_log10) {
     st->best_log10 = l;
     st->have_best = 1;
+  }
+  if (st->rule_every > 0) {
+    // NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:185-223: for n = 0..N_final,
+    // step, then check when n % k == 0 or n == N_final (the state after n + 1 steps), stop on
+    // log10 L2 < target
+    const int m = st->steps;
+    const bool due = m >= 1 && ((m - 1) % st->rule_every == 0 || m == st->rule_last);
+    if (due && l < target) st->done = 1;
+    else if (m >= st->rule_last) st->done = 4;
+    else if (l > st->best_log10 + NRPYELL_DIVERGE_DECADES) st->done = 3;
+    return;
   }
   if (l <= target) st->done = 1;
   else if (l > st->best_log10 + NRPYELL_DIVERGE_DECADES) st->done = 3;

@@ -29,7 +29,12 @@ class solve_options(C.Structure):
     _fields_ = [("precision", C.c_int), ("check_every", C.c_int), ("steps_per_graph", C.c_int),
                 ("log10_target", C.c_double), ("max_steps", C.c_int), ("fp32_switch_log10", C.c_double),
                 ("fp32_plateau_steps", C.c_int), ("integration_radius", C.c_double),
-                ("tile_i0", C.c_int), ("planes", C.c_int), ("outer_bc", C.c_int), ("outer_bc_speed", C.c_double)]
+                ("tile_i0", C.c_int), ("planes", C.c_int), ("outer_bc", C.c_int), ("outer_bc_speed", C.c_double),
+                ("stop_rule", C.c_int)]
+
+
+# stopping rules of the resident solver (nrpyell_solve_options.stop_rule)
+STOP_RULE = {"every_step": 0, "et_thorn": 1}
 
 
 # outer boundary conditions of the solvers (nrpyell_solve_options.outer_bc)
@@ -50,7 +55,12 @@ ADMBASE_NAMES = ["uuGF", "alp", "betax", "betay", "betaz", "gxx", "gxy", "gxz", 
 
 class solve_result(C.Structure):
     _fields_ = [("steps", C.c_int), ("fp32_steps", C.c_int), ("log10_l2", C.c_double),
-                ("log10_l2_initial", C.c_double), ("seconds", C.c_double), ("converged", C.c_int)]
+                ("log10_l2_initial", C.c_double), ("seconds", C.c_double), ("converged", C.c_int),
+                ("status", C.c_int)]
+
+
+# nrpyell_solve_result.status
+STATUS = {0: "max_steps", 1: "converged", 2: "nan", 3: "diverged"}
 
 
 class NrpyellError(RuntimeError):
@@ -119,14 +129,16 @@ def check(rc: int, what: str = "") -> None:
 
 def default_options(**kw) -> solve_options:
     """nrpyell_solve_options with the library defaults, overridden by keyword; outer_bc takes
-    "radiation" or "sommerfeld6" as well as the integer."""
+    "radiation" or "sommerfeld6" and stop_rule "every_step" or "et_thorn" as well as the
+    integers."""
     o = solve_options()
     lib().nrpyell_default_options(C.byref(o))
+    names = {"outer_bc": OUTER_BC, "stop_rule": STOP_RULE}
     for k, v in kw.items():
-        if k == "outer_bc" and isinstance(v, str):
-            if v not in OUTER_BC:
-                raise NrpyellError(f"outer_bc {v!r}: expected one of {sorted(OUTER_BC)}")
-            v = OUTER_BC[v]
+        if k in names and isinstance(v, str):
+            if v not in names[k]:
+                raise NrpyellError(f"{k} {v!r}: expected one of {sorted(names[k])}")
+            v = names[k][v]
         setattr(o, k, v)
     return o
 
@@ -212,6 +224,7 @@ class Problem:
         L.wavespeed_gf_all_points(C.byref(p), cfg.CFL, self.dt, gd.xx, gd.gridfuncs.auxevol_gfs)
         c = L.compute_wavespeed_at_OB(C.byref(p), gd.gridfuncs.auxevol_gfs)
         p.uu_wavespeed_at_OB = p.vv_wavespeed_at_OB = c
+        self.cfl = cfg.CFL
         self._solver: Optional[Solver] = None
         self._closed = False
 
@@ -221,6 +234,25 @@ class Problem:
         for d in range(3):
             libc.free(C.cast(self.gd.xx[d], C.c_void_p))
             self.gd.xx[d] = abi.REALP()
+
+    def set_wave_speed_cfl(self, cfl: float) -> None:
+        """Recompute the wave speed c = cfl * (local minimum proper spacing) / dt
+        (wavespeed_gf_all_points.c:40-41) and the outer-boundary speed from it
+        (compute_wavespeed_at_OB.c:10). dt stays: find_timestep returns the smallest proper
+        spacing whatever the CFL factor (find_timestep.c:45). The relaxation's steady state,
+        c^2 (lap u + AA^2/(8 (psi+u)^7)) = 0 with the radiation BC, does not depend on c; the
+        factor sets how fast the relaxation moves and whether RK4 stays stable. Solvers built
+        before the call keep the old speed."""
+        L = lib()
+        p = self.gd.params
+        L.wavespeed_gf_all_points(C.byref(p), float(cfl), self.dt, self.gd.xx, self.gd.gridfuncs.auxevol_gfs)
+        c = L.compute_wavespeed_at_OB(C.byref(p), self.gd.gridfuncs.auxevol_gfs)
+        p.uu_wavespeed_at_OB = p.vv_wavespeed_at_OB = c
+        self.cfl = float(cfl)
+
+    def zero_guess(self) -> None:
+        """u = v = 0 at every point (initial_guess_all_points.c:7), the reference's start."""
+        lib().initial_guess_all_points(C.byref(self.gd.params), self.gd.xx, self.gd.gridfuncs.y_n_gfs)
 
     # --- device arrays of the griddata ----------------------------------------------------
     def get(self, name: str) -> np.ndarray:
@@ -510,3 +542,45 @@ class BatchSolver:
             self.close()
         except Exception:
             pass
+
+
+# A configuration whose relaxation diverges at the reference's CFL factor 0.7 is solved again
+# from the zero guess at this factor. Frozen-coefficient bound: the 10th-order second
+# difference has symbol at most 6.8267 (at the Nyquist mode), and with the local wave speed
+# c = CFL ds_min/dt the RK4 step sees |lambda dt| <= CFL sqrt(6.8267 (1 + r1^2 + r2^2)), r the
+# ratios of the smallest proper spacing to the other two; RK4 is stable on the imaginary axis up
+# to 2 sqrt(2). At 0.7 every grid of the batched workloads exceeds that somewhere near the
+# origin (2.6-3.2; most configurations relax anyway); at 0.5 every one stays below it (<= 2.26).
+FALLBACK_CFL = 0.5
+
+
+def relax_batch(problems, tiles=None, fallback_cfl: Optional[float] = FALLBACK_CFL, **opts):
+    """Relax `problems` together (BatchSolver); then every configuration whose relaxation
+    diverged (status "diverged": its residual rose 4 decades above its lowest) is relaxed again,
+    from the zero guess, with the wave speed set at `fallback_cfl` (set_wave_speed_cfl; the
+    converged u does not depend on it). Returns (results, seconds, info): results[k] is the
+    configuration's final solve, info[k] {"fallback": bool, "first_steps": steps of the diverged
+    attempt}. Problems that fell back keep the fallback wave speed."""
+    tiles = list(tiles) if tiles is not None else None
+    B = BatchSolver(problems, tiles=tiles, **opts)
+    try:
+        res, sec = B.solve()
+    finally:
+        B.close()
+    info = [{"fallback": False, "first_steps": 0} for _ in problems]
+    redo = [k for k, r in enumerate(res) if r.status == 3]
+    if redo and fallback_cfl:
+        sub = [problems[k] for k in redo]
+        for P in sub:
+            P.set_wave_speed_cfl(fallback_cfl)
+            P.zero_guess()
+        B2 = BatchSolver(sub, tiles=[tiles[k] for k in redo] if tiles is not None else None, **opts)
+        try:
+            res2, sec2 = B2.solve()
+        finally:
+            B2.close()
+        sec += sec2
+        for k, r in zip(redo, res2):
+            info[k] = {"fallback": True, "first_steps": int(res[k].steps)}
+            res[k] = r
+    return res, sec, info

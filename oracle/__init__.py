@@ -264,6 +264,40 @@ class OracleProblem:
     def log10_residual(self, y_n, radius=100.0):
         return self.l2(self.residual(y_n), radius)
 
+    def relax_every_step(self, st, target, max_steps):
+        """The residual of y_n before every step (main.c:136-151: residual_all_points +
+        L2_norm_of_gf at the top of each iteration, then MoL_step_forward_in_time at :185);
+        stop at the first log10 L2 <= target or after max_steps steps. Returns (steps,
+        history of (steps, log10 L2))."""
+        hist = []
+        n = 0
+        while True:
+            l2 = self.log10_residual(st["y_n"])
+            hist.append((n, l2))
+            if l2 <= target or n >= max_steps:
+                return n, hist
+            self.mol_step(st)
+            n += 1
+
+    def relax_et_thorn(self, st, target, max_iterations, info_output_freq):
+        """The Einstein Toolkit thorn's relaxation loop
+        (NRPyEllipticET_conformally_flat_BBH_Hyperbolic_Relaxation.c:181-229): k =
+        info_output_freq (1 when 0); for n = 0..max_iterations: one MoL step, then, when
+        n % k == 0 or n == max_iterations, the residual of the state after n + 1 steps; stop when
+        log10 L2 < target. If no check passes the loop ends after max_iterations + 1 steps (the
+        error branch after the loop tests n == N_final, which the loop never leaves n at).
+        Returns (steps, converged, history of (steps, log10 L2))."""
+        k = info_output_freq if info_output_freq != 0 else 1
+        hist = []
+        for n in range(max_iterations + 1):
+            self.mol_step(st)
+            if n % k == 0 or n == max_iterations:
+                l2 = self.log10_residual(st["y_n"])
+                hist.append((n + 1, l2))
+                if l2 < target:
+                    return n + 1, True, hist
+        return max_iterations + 1, False, hist
+
 
 # ---------------------------------------------------------------------------------------
 # The compiled reference (oracle/_ref), present where it could be built.

@@ -10,9 +10,18 @@ Fixtures are data only (inputs and the reference's outputs):
                                  twenty MoL steps from u=v=0 (shipped free_parameters.h punctures)
   solve_<name>.npz              per-step log10 L2 of a full relaxation (main.c loop) and the
                                  converged u on a strided sub-grid and along the z axis
+  deep_<name>.npz               the reference relaxed far below the stopping target: u* at
+                                 every interior point (float64) when log10 L2 reaches the deep
+                                 target, the step at which the every-step check first met -10
+                                 and u there as float32 differences from u*
+  cfl_fallback_seed42.npz       for the seed-42 batch members whose relaxation diverges at the
+                                 reference's CFL factor 0.7: their residual histories at 0.7
+                                 (to the divergence) and the full relaxation at 0.5 (steps to
+                                 -10, every-step check)
 """
 from __future__ import annotations
 
+import dataclasses
 import hashlib
 import json
 import os
@@ -115,6 +124,86 @@ def solve_fixture(name, cfg, target, max_steps):
     print(name, "steps", n, "final", hist[-1], "%.1fs" % (time.time() - t0), flush=True)
 
 
+def interior_u(R):
+    G0, G1, G2 = R.Ng
+    return R.arr("y_n_gfs")[: R.ntot].reshape(G2, G1, G0)[5:-5, 5:-5, 5:-5].copy()
+
+
+def deep_solve_fixture(name, cfg, stop_log10=-10.0, deep_log10=-14.0, max_steps=30000, stall=2000):
+    """The every-step check until log10 L2 <= stop_log10 (the reference's stopping step and u
+    there), then on every 10th step until deep_log10, or until the lowest residual has not
+    improved for `stall` steps (the FP64 floor)."""
+    R = oracle.RefProblem(cfg)
+    t0 = time.time()
+    n, n_stop, u_stop, hist = 0, None, None, []
+    best, best_n = 1e9, 0
+    while True:
+        if n_stop is None or n % 10 == 0:
+            l2 = R.l2(R.residual(R.arr("y_n_gfs")))
+            hist.append((n, l2))
+            if n_stop is None and l2 <= stop_log10:
+                n_stop, u_stop = n, interior_u(R)
+            if l2 < best - 1e-3:
+                best, best_n = l2, n
+            if l2 <= deep_log10 or n >= max_steps or (n_stop is not None and n - best_n > stall):
+                break
+        R.mol_step()
+        n += 1
+    u_deep = interior_u(R)
+    h = np.array(hist)
+    np.savez_compressed(os.path.join(HERE, f"deep_{name}.npz"), u_deep=u_deep, steps_deep=n, log10_deep=h[-1, 1],
+                        steps_stop=n_stop, stop_log10=stop_log10, du_stop=(u_stop - u_deep).astype(np.float32),
+                        hist_steps=h[:, 0].astype(np.int32), hist_log10=h[:, 1], dt=R.dt,
+                        seconds=time.time() - t0)
+    print(name, "stop", n_stop, "deep", n, h[-1, 1], "max|u_stop-u*| %.3e" % np.max(np.abs(u_stop - u_deep)),
+          "%.1fs" % (time.time() - t0), flush=True)
+
+
+# Members of config.random_batch(64, seed=42) whose relaxation diverges at CFL 0.7 (found by the
+# batched solver's divergence rule on the GPU; each is confirmed here on the reference C).
+SEED42_DIVERGING = [1, 4, 11, 15, 19, 28, 38, 46, 56]
+
+
+def cfl_fallback_fixture(indices, fallback_cfl=0.5, max_steps=30000):
+    cfgs = config.random_batch(64, seed=42)
+    rows = {}
+    for k in indices:
+        t0 = time.time()
+        R = oracle.RefProblem(cfgs[k])
+        h07 = []
+        for n in range(4001):
+            if n % 25 == 0:
+                l2 = R.l2(R.residual(R.arr("y_n_gfs")))
+                h07.append(l2)
+                if not np.isfinite(l2) or l2 > min(h07) + 4.0:
+                    break
+            R.mol_step()
+        R = oracle.RefProblem(dataclasses.replace(cfgs[k], CFL=fallback_cfl))
+        h05 = []
+        n = 0
+        while True:
+            l2 = R.l2(R.residual(R.arr("y_n_gfs")))
+            if n % 25 == 0:
+                h05.append(l2)
+            if l2 <= -10.0 or n >= max_steps:
+                break
+            R.mol_step()
+            n += 1
+        rows[k] = (np.array(h07), np.array(h05), n, l2)
+        print(f"seed42_{k}: CFL 0.7 {h07[0]:.2f} -> {h07[-1]:.2f} in {25 * (len(h07) - 1)} steps; "
+              f"CFL {fallback_cfl} -10 in {n} steps ({time.time() - t0:.0f}s)", flush=True)
+    L07 = max(len(r[0]) for r in rows.values())
+    L05 = max(len(r[1]) for r in rows.values())
+    pad = lambda a, L: np.concatenate([a, np.full(L - len(a), np.nan)])
+    np.savez_compressed(os.path.join(HERE, "cfl_fallback_seed42.npz"), indices=np.array(indices),
+                        fallback_cfl=fallback_cfl, every=25,
+                        hist07=np.stack([pad(rows[k][0], L07) for k in indices]),
+                        len07=np.array([len(rows[k][0]) for k in indices]),
+                        hist_fallback=np.stack([pad(rows[k][1], L05) for k in indices]),
+                        steps_fallback=np.array([rows[k][2] for k in indices]),
+                        log10_fallback=np.array([rows[k][3] for k in indices]))
+
+
 if __name__ == "__main__":
     if oracle.ref_lib() is None:
         oracle.build()
@@ -122,3 +211,6 @@ if __name__ == "__main__":
     ops_fixtures()
     solve_fixture("axisym_spin_64x64x4", config.axisymmetric_spinning(), -8.0, 5000)
     solve_fixture("equal_mass_boosted_128x128x16", config.equal_mass_boosted(), -10.0, 9000)
+    deep_solve_fixture("equal_mass_boosted_128x128x16", config.equal_mass_boosted())
+    deep_solve_fixture("q4_D10_seed7_128x128x16", config.unequal_mass_q4())
+    cfl_fallback_fixture(SEED42_DIVERGING)
