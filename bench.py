@@ -198,12 +198,16 @@ def pmc_probe():
     S = P.solver(steps_per_graph=10)
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
     S.timed_steps(20)
+    S1 = P.solver(steps_per_graph=10, substeps_per_launch=1)
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    S1.timed_steps(20)
     a = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
-    print("pmc probe: 20 steps (80 substep launches) + 5 copies of 64 MiB", flush=True)
+    print("pmc probe: 20 steps of the production path (40 two-substep launches), 20 steps of one substep per "
+          "launch (80 launches), 5 copies of 64 MiB", flush=True)
 
 
 def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
@@ -379,10 +383,11 @@ def batched_nn_tuned(net, f_nn, f_zero, n=32, seed=900):
     return out
 
 
-def load_traffic():
-    """HBM bytes per substep launch from the committed rocprofv3 PMC summary, scaled by the
-    copy-kernel calibration recorded with it (null when absent)."""
-    path = os.path.join(ROOT, "profiles", "substep_pmc_128x128x16.json")
+def load_traffic(spl=2):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary,
+    scaled by the copy-kernel calibration recorded with it (null when absent)."""
+    name = "substep2_pmc_128x128x16.json" if spl == 2 else "substep_pmc_128x128x16.json"
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             d = json.load(f)
@@ -443,8 +448,11 @@ def main():
     runtime.lib().nrpyell_device_synchronize()
     sec = ms / 1e3
     value = 4 * npts * args.steps / sec
-    us_per_launch = 1e3 * ms / (4 * args.steps)
-    achieved = BYTES_PER_GP_SUBSTEP * npts / (us_per_launch * 1e-6) / 1e9
+    # the dominant kernel does `spl` substeps per launch: 4 / spl launches per RK4 step
+    spl = S.substeps_per_launch
+    us_per_launch = 1e3 * ms / ((4 // spl) * args.steps)
+    bytes_per_launch = spl * BYTES_PER_GP_SUBSTEP * npts
+    achieved = bytes_per_launch / (us_per_launch * 1e-6) / 1e9
 
     if args.timed_only:
         print(json.dumps({"metric": METRIC, "value": value, "unit": "gp-substeps/s", "steps": args.steps,
@@ -463,8 +471,15 @@ def main():
     S_et.close()
     et_thorn = {"outer_bc": "sommerfeld6", "wave_speed": runtime.ET_SOMMERFELD_SPEED, "steps": r_et.steps,
                 "seconds": r_et.seconds, "log10_l2": r_et.log10_l2, "converged": bool(r_et.converged)}
-    # per-stage kernel durations (back-to-back launches, HIP events on the solver stream)
-    stage_us = {f"stage{k}": round(S.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)}
+    # per-launch kernel durations (back-to-back launches, HIP events on the solver stream): the
+    # two-substep kernels of the production path, and the one-substep kernels with the RK4 step
+    # they make (the drop-in MoL_step_forward_in_time, batched and FP32 paths use those)
+    stage_us = {"stages12": round(S.time_substep(5, 200), 3), "stages34": round(S.time_substep(6, 200), 3)}
+    S1 = P.solver(steps_per_graph=100, substeps_per_launch=1)
+    stage_us.update({f"one_substep_stage{k}": round(S1.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)})
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    stage_us["one_substep_per_launch_us_per_step"] = round(1e3 * S1.timed_steps(2000)[0] / 2000, 3)
+    S1.close()
     dropin_us = dropin_call_us(P)
     admbase = admbase_throughput(P)
     batched = batched_solve()
@@ -489,10 +504,12 @@ def main():
                                "residual+L2 every step", "Nxx": list(cfg.Nxx), "global_batch": 1,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(),
-                     "kernel": "substep_kernel (fused RHS + radiation BC + RK4 stage update + ghost images)",
-                     "us_per_launch": us_per_launch, "bytes_per_launch": BYTES_PER_GP_SUBSTEP * npts,
-                     "stage_us": stage_us},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(spl),
+                     "kernel": ("substep2_kernel (two fused RK4 substeps per launch: RHS + radiation BC + stage "
+                                "updates + ghost images, residual of y_n in the first)" if spl == 2 else
+                                "substep_kernel (fused RHS + radiation BC + RK4 stage update + ghost images)"),
+                     "substeps_per_launch": spl, "us_per_launch": us_per_launch,
+                     "bytes_per_launch": bytes_per_launch, "stage_us": stage_us},
         "dropin_call_us": dropin_us,
         "et_consumer_admbase": admbase,
         "batched_64_seed42": batched,

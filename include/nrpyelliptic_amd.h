@@ -284,6 +284,10 @@ typedef struct {
                              max_iterations) decide, the stop needs log10 L2 < log10_target,
                              and the solve ends after step max_steps+1 if none passes (status
                              NRPYELL_STATUS_MAX_STEPS). FP64 single-configuration solves only. */
+  int substeps_per_launch;  /* FP64 steps of the resident solver: 2 = two RK4 substeps per
+                             launch (two launches per step; radiation outer BC, Nxx2 >= 6); 1 =
+                             one per launch; 0 = 2 where the grid and BC allow it, else 1. Both
+                             give the same states bit for bit.                               */
 } nrpyell_solve_options;
 #define NRPYELL_STOP_EVERY_STEP 0
 #define NRPYELL_STOP_ET_THORN 1
@@ -334,11 +338,14 @@ int nrpyell_solver_steps(nrpyell_solver *s, int n);
 /* Fused residual + L2 of the current y_n; returns log10 L2 in *log10_l2. */
 int nrpyell_solver_residual_l2(nrpyell_solver *s, double *log10_l2);
 /* Time n launches of the fused substep kernel for stage `stage` (1..4; 0 = stage 1 with the
- * fused residual reduction) with HIP events on the solver's stream; mean microseconds. */
+ * fused residual reduction; 5 and 6: the two-substep kernels of stages 1+2, with the residual
+ * reduction, and 3+4) with HIP events on the solver's stream; mean microseconds. */
 int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_per_launch);
 /* Exactly n RK4 steps of the production FP64 path (graph replay, residual reduced every
  * step, no early stop), timed with HIP events on the solver's stream. */
 int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms, double *log10_l2);
+/* RK4 substeps per launch of the solver's FP64 steps (1 or 2, see substeps_per_launch). */
+int nrpyell_solver_substeps_per_launch(nrpyell_solver *s);
 /* Residual history of the last solve: (steps completed, log10 L2) pairs; returns the count. */
 int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2);
 

@@ -12,8 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnrpyell_hip.so")
-SOURCES = ["nrpyell_host.cpp", "nrpyell_kernels.hip", "nrpyell_substep.hip", "nrpyell_capi.hip", "nrpyell_mlp.hip",
-           "nrpyell_adm.hip"]
+SOURCES = ["nrpyell_host.cpp", "nrpyell_kernels.hip", "nrpyell_substep.hip", "nrpyell_substep2.hip",
+           "nrpyell_capi.hip", "nrpyell_mlp.hip", "nrpyell_adm.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

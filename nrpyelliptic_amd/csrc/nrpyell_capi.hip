@@ -685,6 +685,8 @@ struct nrpyell_solver {
   nrpyell_solve_options opt{};
   hipStream_t stream = nullptr;
   int tx = 32, pl = 1, ntile0 = 0, ntile1 = 0;
+  bool pair_path = false;  // FP64 steps as two launches of two substeps each
+  int ybuf = 0;            // pair path: y_n currently in y_n_gfs (0) or k_even_gfs (1)
   double *coef64 = nullptr;   // metric coefficients (residual kernel)
   double *kc64 = nullptr;     // per (i0,i1): c^2 x 5 coefficients, L2 weight / c^4 (substep kernel)
   float *kc32 = nullptr;
@@ -758,6 +760,7 @@ struct nrpyell_solver {
     a.nbc_rows = (g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS;
     a.ntile0 = ntile0;
     a.ntile1 = ntile1;
+    a.i0_first = NRPYELL_NG + g.N0 - ntile0 * tx;
     a.ng_mul = div_mul((unsigned)(g.N2 / pl));
     a.nt0_mul = div_mul((unsigned)ntile0);
     a.nbc_mul = div_mul((unsigned)a.nbc_rows);
@@ -768,6 +771,45 @@ struct nrpyell_solver {
     return a;
   }
 
+  // The pair path keeps y_n in one of two arrays: y_n_gfs or k_even_gfs (which the pair
+  // kernels do not otherwise use); each step writes the other one.
+  double *ycur(int j) const { return j == 0 ? gd->gridfuncs.y_n_gfs : gd->gridfuncs.k_even_gfs; }
+  // One FP64 RK4 step as two launches of two substeps, from y in ycur(j) to ycur(1 - j); with
+  // fuse_check the first launch reduces the residual of y_n into per-block partials and the
+  // second one's block 0 updates the convergence state.
+  void step64_pair(hipStream_t st, int j, bool fuse_check) {
+    MoL_gridfunctions_struct &f = gd->gridfuncs;
+    SubstepArgs<double> a = args<double>(1, ycur(j), f.k_odd_gfs, nullptr, f.y_nplus1_running_total_gfs, src64,
+                                         kc64, geo64);
+    a.in = ycur(j);
+    a.yn = ycur(j);
+    a.out = f.k_odd_gfs;
+    pair_layout(a);
+    if (fuse_check) {
+      a.l2_partial = partial;
+      a.state = dstate;
+      a.l2_volume = volume;
+      a.l2_target = opt.log10_target;
+    }
+    launch_substep_pair(a, 1, st);
+    a.in = f.k_odd_gfs;
+    a.yn = ycur(j);
+    a.out = ycur(1 - j);
+    launch_substep_pair(a, 3, st);
+  }
+  // the pair kernel's BC blocks cover NRPYELL_PAIR_BC_ROWS rows each
+  void pair_layout(SubstepArgs<double> &a) const {
+    a.nbc_rows = (g.N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS;
+    a.nbc_mul = div_mul((unsigned)a.nbc_rows);
+  }
+  // After a pair-path run that left y_n in ycur(j): bring it back to y_n_gfs.
+  void settle(hipStream_t st, int j) {
+    if (j == 1)
+      HIPCHK(hipMemcpyAsync(gd->gridfuncs.y_n_gfs, gd->gridfuncs.k_even_gfs, 2 * g.ntot * sizeof(double),
+                            hipMemcpyDeviceToDevice, st));
+  }
+  // RK4 steps in a graph: the pair path needs an even count (y_n back in y_n_gfs)
+  int graph_steps64() const { return pair_path ? opt.steps_per_graph + (opt.steps_per_graph & 1) : opt.steps_per_graph; }
   // One FP64 RK4 step; with fuse_check the stage-1 kernel also reduces the residual L2 of
   // y_n (before the step) into the convergence state.
   void step64(hipStream_t st, bool fuse_check = false) {
@@ -853,6 +895,7 @@ extern "C" void nrpyell_default_options(nrpyell_solve_options *o) {
   o->outer_bc = NRPYELL_OUTER_BC_RADIATION;
   o->outer_bc_speed = 0.0;
   o->stop_rule = NRPYELL_STOP_EVERY_STEP;
+  o->substeps_per_launch = 0;
 }
 
 extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell_solve_options *opt,
@@ -906,12 +949,28 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     }
     s->tx = tx;
     s->pl = pl;
+    {
+      // two substeps per launch: FP64 steps with the radiation BC; the BC blocks' reflected rows
+      // and the phi windows need Nxx1 >= 10 (checked above) and Nxx2 >= 6
+      const bool can = s->opt.outer_bc == NRPYELL_OUTER_BC_RADIATION && g.N2 >= 6;
+      const int want = s->opt.substeps_per_launch;
+      if (want != 0 && want != 1 && want != 2) {
+        set_last_error("substeps_per_launch must be 0, 1 or 2");
+        return NRPYELL_EINVAL;
+      }
+      if (want == 2 && !can) {
+        set_last_error("two substeps per launch need the radiation outer BC and Nxx2 >= 6");
+        return NRPYELL_EINVAL;
+      }
+      s->pair_path = want == 2 || (want == 0 && can);
+    }
     const int ty = NRPYELL_BLOCK / tx;
     s->ntile0 = (g.N0 + tx - 1) / tx;
     s->ntile1 = (g.N1 + ty - 1) / ty;
     {
       // the kernels split block indices with div_by: exact while index x divisor <= 2^32
-      const double lim = 4294967296.0, nbc = (double)((g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS);
+      // (the two-substep kernel's BC blocks cover fewer rows: NRPYELL_PAIR_BC_ROWS >= 8)
+      const double lim = 4294967296.0, nbc = (double)((g.N1 + 7) / 8);
       const double nsub = (double)s->ntile0 * s->ntile1 * (g.N2 / pl);
       if (nsub * (g.N2 / pl) > lim || (double)s->ntile0 * s->ntile1 * s->ntile0 > lim || nbc * nbc * g.N2 > lim) {
         set_last_error("grid too large for the fused solver's block indexing");
@@ -985,6 +1044,9 @@ static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double t
       // FP32 bulk phase: the residual is evaluated in FP64 from the FP32 state every check_every steps
       s->step32(s->stream);
       if (k % s->opt.check_every == 0) s->check(s->stream, true, s->opt.check_every, target);
+    } else if (s->pair_path) {
+      // FP64, two substeps per launch: y alternates between the two arrays (nsteps is even)
+      s->step64_pair(s->stream, (k - 1) & 1, true);
     } else {
       // FP64: every step's stage-1 kernel checks the residual of the state it starts from
       s->step64(s->stream, true);
@@ -1000,7 +1062,12 @@ static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double t
 extern "C" int nrpyell_solver_steps(nrpyell_solver *s, int n) {
   return api_call([&]() -> int {
     reset_state(s);
-    for (int k = 0; k < n; k++) s->step64(s->stream);
+    if (s->pair_path) {
+      for (int k = 0; k < n; k++) s->step64_pair(s->stream, k & 1, false);
+      s->settle(s->stream, n & 1);
+    } else {
+      for (int k = 0; k < n; k++) s->step64(s->stream);
+    }
     s->fill_ghosts(s->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1096,7 +1163,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       if (!s->graph64 || !s->key64.same(k64)) {
         if (s->graph64) (void)hipGraphExecDestroy(s->graph64);
         s->graph64 = nullptr;
-        s->graph64 = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
+        s->graph64 = capture(s, false, s->graph_steps64(), s->opt.log10_target);
         s->key64 = k64;
       }
       reset_state(s);
@@ -1110,6 +1177,9 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
         if (s->hstate->done || (!thorn && steps + s->hstate->nchecks >= s->opt.max_steps)) break;
       }
       steps += s->hstate->done ? s->hstate->steps : s->hstate->nchecks;
+      // pair path: a check that ended the solve left its state in the array of its parity;
+      // otherwise every graph ran an even number of steps
+      if (s->pair_path) s->settle(s->stream, s->hstate->done ? (s->hstate->steps & 1) : 0);
     }
     s->fill_ghosts(s->stream);
     HIPCHK(hipStreamSynchronize(s->stream));
@@ -1139,9 +1209,9 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
   return api_call([&]() -> int {
     const double keep = s->opt.log10_target;
     s->opt.log10_target = -1e300;
-    hipGraphExec_t g = capture(s, false, s->opt.steps_per_graph, s->opt.log10_target);
+    const int per = s->graph_steps64();
+    hipGraphExec_t g = capture(s, false, per, s->opt.log10_target);
     s->opt.log10_target = keep;
-    const int per = s->opt.steps_per_graph;
     hipGraphExec_t tail = nullptr;
     if (n % per) {
       s->opt.log10_target = -1e300;
@@ -1159,6 +1229,7 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    if (s->pair_path) s->settle(s->stream, (n % per) & 1);
     s->fill_ghosts(s->stream);
     read_state(s);
     *ms_out = ms;
@@ -1170,6 +1241,8 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
     return NRPYELL_OK;
   });
 }
+
+extern "C" int nrpyell_solver_substeps_per_launch(nrpyell_solver *s) { return s ? (s->pair_path ? 2 : 1) : 0; }
 
 extern "C" int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2) {
   return api_call([&]() -> int {
@@ -1191,6 +1264,43 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
   return api_call([&]() -> int {
     MoL_gridfunctions_struct &f = s->gd->gridfuncs;
     reset_state(s);
+    if (stage == 5 || stage == 6) {
+      // the two-substep kernels: 5 = stages 1+2 with the fused residual check, 6 = stages 3+4;
+      // their outputs (k_odd, the running total, k_even) are scratch, y_n is only read
+      if (!s->pair_path) {
+        set_last_error("this solver runs one substep per launch");
+        return NRPYELL_EINVAL;
+      }
+      SubstepArgs<double> a = s->args<double>(1, f.y_n_gfs, f.k_odd_gfs, nullptr, f.y_nplus1_running_total_gfs,
+                                              s->src64, s->kc64, s->geo64);
+      a.in = a.yn = f.y_n_gfs;
+      a.out = f.k_odd_gfs;
+      s->pair_layout(a);
+      a.l2_partial = s->partial;
+      a.state = s->dstate;
+      a.l2_volume = s->volume;
+      a.l2_target = -1e300;
+      if (stage == 6) {
+        a.in = f.k_odd_gfs;
+        a.out = f.k_even_gfs;
+        a.l2_partial = nullptr;  // time the steady-state launch, not the block-0 reduction
+      }
+      const int first = stage == 5 ? 1 : 3;
+      for (int k = 0; k < 3; k++) launch_substep_pair(a, first, s->stream);
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      HIPCHK(hipEventRecord(e0, s->stream));
+      for (int k = 0; k < n; k++) launch_substep_pair(a, first, s->stream);
+      HIPCHK(hipEventRecord(e1, s->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      *us_per_launch = 1000.0 * ms / n;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return NRPYELL_OK;
+    }
     // stage 0 = stage 1 with the fused residual-L2 check (the form the solver runs)
     const bool fuse = stage == 0;
     if (fuse) stage = 1;

@@ -43,6 +43,11 @@ struct SolverDeviceState {
 #ifndef NRPYELL_BC_ROWS
 #define NRPYELL_BC_ROWS (NRPYELL_BLOCK / 16)
 #endif
+// i1 rows per BC block of the two-substep kernel (each block also works on 10 halo rows; 8 and
+// 16 measured alike at 128x128x16 with 32x2 tiles, 16 faster with one-plane tiles)
+#ifndef NRPYELL_PAIR_BC_ROWS
+#define NRPYELL_PAIR_BC_ROWS 16
+#endif
 
 // Bookkeeping of one residual check (log10 L2 = l) taken after st->steps steps: history,
 // done = 1 at the target, 2 on NaN, 3 on divergence.
@@ -143,6 +148,7 @@ struct SubstepArgs {
   int pl;           // i2 planes per thread (1 or 2)
   int nbc_rows;     // BC blocks per i2 plane: ceil(N1 / NRPYELL_BC_ROWS)
   int ntile0, ntile1;
+  int i0_first;     // two-substep kernel: i0 of the first tile's column 0 (tiles end at ilast)
   // div_mul of N2/pl, ntile0 and nbc_rows: the kernels split block indices by them
   unsigned long long ng_mul, nt0_mul, nbc_mul;
   // Stage 1 (FP64) can also reduce the residual of its input y_n: the stencil it evaluates
@@ -167,6 +173,8 @@ void launch_fold(const GridDims &g, const double *aux, const double *coef, const
                  float *kc32, double *src, int *bad, hipStream_t st);
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
+// two RK4 substeps (first_stage 1: stages 1+2; 3: stages 3+4) in one launch (nrpyell_substep2.hip)
+void launch_substep_pair(const SubstepArgs<double> &a, int first_stage, hipStream_t st);
 template <typename T>
 void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
                             int tx, int pl, int stage, hipStream_t st);

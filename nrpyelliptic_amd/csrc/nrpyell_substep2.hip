@@ -1,0 +1,592 @@
+// nrpyell_substep2.hip -- two RK4 substeps per launch (stages 1+2, then stages 3+4): the FP64
+// path of the resident solver with the radiation outer BC.
+//
+// RK4 on  u_t = v - eta u,  v_t = c^2 (lap u + AA^2 / (8 (psi+u)^7))  couples neighbouring points
+// only through the stencil of u: the next stage's u is pointwise,
+//   u_{s+1} = y_u + a_s dt (v_s - eta u_s),
+// so a block that reads stage s's (u, v) on its tile plus the 5-point halo can form u_{s+1} on the
+// whole halo itself and evaluate stage s+1's stencil without waiting for its neighbours. One
+// launch then does two substeps with no redundant stencil work (only the pointwise u_{s+1} on the
+// halo is recomputed), and an RK4 step is two dependent launches instead of four. Per point the
+// arithmetic is that of the one-substep kernel (same stage updates, same stencil sums, same
+// radiation terms): the two paths give bit-identical states.
+//
+// Interior blocks: tiles of TX (i0) x TY (i1) points on PL i2 planes, aligned so that the last
+// tile ends at the last interior i0 (ilast). Only that tile's stencils reach the outer ghost
+// layers; it forms u_{s+1} there from stage s's radiation condition (the same closed form as the
+// BC blocks, from its LDS box).
+//
+// BC blocks (rows r0..r0+TYB-1 of one plane) produce the outer-layer values of stage s+1. That
+// needs stage s+1 on the outer columns with i1 neighbours +-5 rows, hence stage s on +-5 more rows
+// and stage s's inputs on +-10 rows: rows outside [NG, N1+NG) are taken through the SinhSymTP
+// reflection (row -> mirrored row, phi -> phi + pi) as "virtual rows", every quantity on them
+// evaluated at the source point in the source's own orientation (the i1 arm read with the
+// reflection's direction), so it equals the source's own value bit for bit.
+//
+// Buffers of one step n -> n+1 (the solver alternates y between two arrays A_j, A_(1-j)):
+//   launch 1 (stages 1+2): in = y = A_j,  writes k_odd = y3 = y + dt/2 k2 and rt = dt/6 k1 + dt/3 k2
+//   launch 2 (stages 3+4): in = k_odd, y = A_j, rt; writes A_(1-j) = y + (rt + dt/3 k3) + dt/6 k4
+// (Storing stage 4's u from launch 1, so that launch 2 reads two arrays instead of three on its
+// boxes and i2 arms, measured no faster: launch 2 gained what launch 1 lost.)
+// y_n is never overwritten within its step, so when stage 1's fused residual check (reduced by
+// the second launch's block 0) ends the solve, A_j still holds the state it checked.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "nrpyell_common.h"
+#include "nrpyell_internal.h"
+#include "nrpyell_stencil.h"
+
+#define NG NRPYELL_NG
+
+
+// Stage s's contribution to the next stage's input, y + a_s dt k_s, with the FMA placement of
+// stage_update (MoL_step_forward_in_time.c:44-45, :88-89, :129-130).
+template <int STAGE>
+__device__ __forceinline__ double next_input(double k, double dt, double yn) {
+  if (STAGE == 3) return fma(dt, k, yn);
+  return fma(0.5, dt * k, yn);
+}
+
+// Radiation term A = -c (d_r f + f / r) at outer layer pt (pt = 0: the last interior point) of
+// apply_bcs_outerradiation_and_inner.c:290-408, as bc_body evaluates it: f(di0, di1) is f at
+// offsets (di0, di1) from the point, d_0 the 11-point stencil shifted by -max(1, pt) toward the
+// interior, d_1 the centred one; geo = (1/r, dxx0/dr, dxx1/dr, (r_0/r)^3).
+template <class F>
+__device__ __forceinline__ double radiation_A(F &&f, int pt, const double geo[4], double c, double invdx0,
+                                              double invdx1) {
+  const int sh = pt < 1 ? 1 : pt;
+  double d0 = 0.0, d1 = 0.0;
+#pragma unroll
+  for (int j = 0; j < 11; j++) {
+    d0 = fma(c_fd1_shift[sh][j], f(j - 5 - sh, 0), d0);
+    d1 = fma(c_fd1_shift[0][j], f(0, j - 5), d1);
+  }
+  d0 *= invdx0;
+  d1 *= invdx1;
+  const double dr = geo[1] * d0 + geo[2] * d1;
+  return -c * (dr + f(0, 0) * geo[0]);
+}
+
+// Source row, source plane and i1 orientation of a row index that may lie beyond the grid's
+// stored ghost rows: the SinhSymTP reflection xx1 -> -xx1 (or 2 pi - xx1) with a half turn in phi.
+// Rows more than N1 - 10 beyond the last interior row occur only in the unused tail of a BC
+// block on a grid with N1 < NRPYELL_BC_ROWS + 10; they are clamped into the grid.
+__device__ __forceinline__ void vrow_source(const GridDims &g, int vr, int i2, int &row, int &pl, int &dir) {
+  if (vr < NG) {
+    row = min(2 * NG - 1 - vr, g.N1 + NG - 1);
+    pl = wrap2_near(g, i2 + g.N2 / 2);
+    dir = -1;
+  } else if (vr >= g.N1 + NG) {
+    row = max(2 * (g.N1 + NG) - 1 - vr, NG);
+    pl = wrap2_near(g, i2 + g.N2 / 2);
+    dir = -1;
+  } else {
+    row = vr;
+    pl = i2;
+    dir = 1;
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Interior blocks.
+// ---------------------------------------------------------------------------------------
+// LDS (doubles) of an interior block and of a BC block; the kernel holds one buffer of the
+// larger size and each body lays its arrays out in it.
+template <int TX, int PL>
+constexpr int interior2_lds() {
+  constexpr int NT = NRPYELL_BLOCK, TY = NT / TX, BOX = (TX + 2 * NG) * (TY + 2 * NG), NR = (BOX + NT - 1) / NT;
+  return 2 * PL * NR * NT + 24 * TY + NT / 64;
+}
+constexpr int bc2_lds() {
+  constexpr int NT = NRPYELL_BLOCK, TYB = NRPYELL_PAIR_BC_ROWS, NR1 = TYB + 2 * NG;
+  constexpr int NRB = ((TYB + 4 * NG) * 17 + NT - 1) / NT;
+  return 2 * NRB * NT + NR1 * (24 + 10 + 24 + 2 + 12) + TYB * (10 + 30 + 8 + 10 + 2 + 12);
+}
+
+template <int PAIR, int TX, int PL>
+__device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, const int tile, const int pgroup,
+                                               const unsigned bid, const unsigned nblk, double *lds) {
+  constexpr int S0 = PAIR;
+  constexpr int NT = NRPYELL_BLOCK, TY = NT / TX;
+  constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG, BOX = BX * BY, NR = (BOX + NT - 1) / NT;
+  constexpr int NW = PL + 2 * NG;
+  constexpr int NBI = 5 * TY * PL;  // outer points of the last tile's rows
+  constexpr int NGEO = (24 * TY + NT - 1) / NT;
+  static_assert(NBI <= NT, "one outer point per thread");
+  // stage-s u and stage-(s+1) u on the box of each plane, the last tile's geometry, block sums
+  double(*sx)[NR * NT] = reinterpret_cast<double(*)[NR * NT]>(lds);
+  double(*s1)[NR * NT] = reinterpret_cast<double(*)[NR * NT]>(lds + PL * NR * NT);
+  double(*s_geo)[TY][4] = reinterpret_cast<double(*)[TY][4]>(lds + 2 * PL * NR * NT);
+  double *s_red = lds + 2 * PL * NR * NT + 24 * TY;
+
+  const GridDims &g = a.g;
+  const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
+  const int t1 = (int)div_by((unsigned)tile, a.nt0_mul), t0 = tile - t1 * a.ntile0;
+  const int i2a = NG + PL * pgroup;
+  const int i0s = a.i0_first + t0 * TX, i1s = NG + t1 * TY;
+  const int i0 = i0s + tx, i1 = i1s + ty;
+  const int ilast = g.N0 + NG - 1;
+  const bool valid = i0 >= NG && i0 <= ilast && i1 < g.N1 + NG;
+  const bool last_tile = t0 == a.ntile0 - 1;  // block-uniform
+  const unsigned nt = (unsigned)g.ntot, s1s = g.G0, s2 = (unsigned)(g.G0 * g.G1);
+  const bool fuse_l2 = (PAIR == 1) && (a.l2_partial != nullptr);
+  const double eta = a.eta, dt = a.dt;
+  const int done = *a.done;
+
+  // ---- loads: unconditional, from clamped indices (values a stencil never reaches are loaded
+  // and not read). For stages 1+2 the stage input is y itself.
+  const int i0c = min(max(i0, NG), ilast), i1c = min(i1, g.N1 + NG - 1);
+  const unsigned k2 = (unsigned)i1c * s1s + i0c;
+  double bxu[PL][NR], bxv[PL][NR], byu[PL][NR];
+  double wxu[NW], wxv[NW], wyu[NW];
+  double yv[PL], psi[PL], AAc[PL], rtu[PL], rtv[PL], K[5];
+  double w2 = 0.0;
+#pragma unroll
+  for (int q = 0; q < PL; q++) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int k = tid + NT * r;
+      const int bx = k % BX, by = k / BX;
+      const int j0 = min(max(i0s - NG + bx, 0), g.G0 - 1), j1 = min(i1s - NG + by, g.G1 - 1);
+      const unsigned p = (unsigned)(i2a + q) * s2 + (unsigned)j1 * s1s + j0;
+      bxu[q][r] = ld32(a.in, p);
+      bxv[q][r] = ld32(a.in, nt + p);
+      byu[q][r] = PAIR == 1 ? bxu[q][r] : ld32(a.yn, p);
+    }
+    if (q == 0) {
+#pragma unroll
+      for (int jj = 0; jj < NW; jj++) {
+        const unsigned pw = (unsigned)wrap2_near(g, i2a - NG + jj) * s2 + k2;
+        wxu[jj] = ld32(a.in, pw);
+        wxv[jj] = ld32(a.in, nt + pw);
+        wyu[jj] = PAIR == 1 ? wxu[jj] : ld32(a.yn, pw);
+      }
+      double kw;
+      ld2x32(a.coef, 6 * k2, K[0], K[1]);
+      ld2x32(a.coef, 6 * k2 + 2, K[2], K[3]);
+      ld2x32(a.coef, 6 * k2 + 4, K[4], kw);
+      if (fuse_l2) w2 = kw;
+    }
+    const unsigned p = (unsigned)(i2a + q) * s2 + k2;
+    yv[q] = PAIR == 1 ? wxv[NG + q] : ld32(a.yn, nt + p);
+    ld2x32(a.src, 2 * p, psi[q], AAc[q]);
+    rtu[q] = 0.0;
+    rtv[q] = 0.0;
+    if (PAIR == 3) {
+      rtu[q] = ld32((const double *)a.rt, p);
+      rtv[q] = ld32((const double *)a.rt, nt + p);
+    }
+  }
+  // the last tile's outer points: plane bq, row brow, layer bpt; y_u there and v at ilast
+  const int bq = min(tid / (5 * TY), PL - 1), brow = (tid / 5) % TY, bpt = tid % 5 + 1;
+  const bool b_on = tid < NBI && i1s + brow < g.N1 + NG;
+  double b_yu = 0.0, b_xv0 = 0.0;
+  double geo_r[NGEO];
+  if (last_tile) {
+    const unsigned bp = (unsigned)(i2a + bq) * s2 + (unsigned)min(i1s + brow, g.N1 + NG - 1) * s1s;
+    b_yu = ld32(a.yn, bp + ilast + bpt);
+    b_xv0 = ld32(a.in, nt + bp + ilast);
+#pragma unroll
+    for (int r = 0; r < NGEO; r++) {
+      const int k = min(tid + NT * r, 24 * TY - 1);
+      const int pt = k / (4 * TY), row = (k / 4) % TY, comp = k % 4;
+      geo_r[r] = ld32(a.geo, ((unsigned)pt * g.G1 + min(i1s + row, g.G1 - 1)) * 4 + comp);
+    }
+  }
+  if (done) return;  // converged: uniform across the grid
+
+  // ---- stage s's u and the pointwise stage-(s+1) u on the whole box
+#pragma unroll
+  for (int q = 0; q < PL; q++) {
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      sx[q][tid + NT * r] = bxu[q][r];
+      s1[q][tid + NT * r] = next_input<S0>(fma(-eta, bxu[q][r], bxv[q][r]), dt, byu[q][r]);
+    }
+  }
+  if (last_tile) {
+#pragma unroll
+    for (int r = 0; r < NGEO; r++)
+      if (tid + NT * r < 24 * TY) (&s_geo[0][0][0])[tid + NT * r] = geo_r[r];
+  }
+  __syncthreads();
+
+  // ---- the last tile: stage-(s+1) u on the outer layers of its rows, from stage s's radiation
+  // condition rhs_pt = A_pt + (r_0/r_pt)^3 (rhs_0 - A_0), rhs_0 = v - eta u at ilast
+  if (last_tile) {
+    if (b_on) {
+      const int cl = ilast - i0s + NG, by = brow + NG;  // box column of ilast, box row
+      const double *box = sx[bq];
+      const double Ap = radiation_A([&](int d0, int d1) { return box[(by + d1) * BX + cl + bpt + d0]; }, bpt,
+                                    s_geo[bpt][brow], a.cOB_u, g.invdx0, g.invdx1);
+      const double A0 = radiation_A([&](int d0, int d1) { return box[(by + d1) * BX + cl + d0]; }, 0,
+                                    s_geo[0][brow], a.cOB_u, g.invdx0, g.invdx1);
+      const double rhs0 = fma(-eta, box[by * BX + cl], b_xv0);
+      const double rhs = fma(s_geo[bpt][brow][3], rhs0 - A0, Ap);
+      s1[bq][by * BX + cl + bpt] = next_input<S0>(rhs, dt, b_yu);
+    }
+    __syncthreads();
+  }
+
+  // ---- both stages at the tile points
+  double l2acc = 0.0;
+  const int cb = (ty + NG) * BX + tx + NG;
+#pragma unroll
+  for (int q = 0; q < PL; q++) {
+    if (valid) {
+      const int i2 = i2a + q;
+      const unsigned p = (unsigned)i2 * s2 + k2;
+      double xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+      // stage s
+      const double uu = wxu[NG + q], vv = wxv[NG + q];
+#pragma unroll
+      for (int k = 1; k <= 5; k++) {
+        xm[k] = sx[q][cb - k]; xp[k] = sx[q][cb + k];
+        ym[k] = sx[q][cb - k * BX]; yp[k] = sx[q][cb + k * BX];
+        zm[k] = wxu[NG + q - k]; zp[k] = wxu[NG + q + k];
+      }
+      double ku0, kv0;
+      point_rhs<double>(uu, vv, psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, eta, ku0, kv0);
+      if (fuse_l2) l2acc += w2 * kv0 * kv0;  // w / c^4 x (c^2 R)^2
+      // stage s+1
+      const double u1 = s1[q][cb];
+      const double v1 = next_input<S0>(kv0, dt, yv[q]);
+#pragma unroll
+      for (int k = 1; k <= 5; k++) {
+        xm[k] = s1[q][cb - k]; xp[k] = s1[q][cb + k];
+        ym[k] = s1[q][cb - k * BX]; yp[k] = s1[q][cb + k * BX];
+        zm[k] = next_input<S0>(fma(-eta, wxu[NG + q - k], wxv[NG + q - k]), dt, wyu[NG + q - k]);
+        zp[k] = next_input<S0>(fma(-eta, wxu[NG + q + k], wxv[NG + q + k]), dt, wyu[NG + q + k]);
+      }
+      double ku1, kv1;
+      point_rhs<double>(u1, v1, psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, eta, ku1, kv1);
+      double ou, ov;
+      if (PAIR == 1) {
+        double ru = (1.0 / 6.0) * (dt * ku0), rv = (1.0 / 6.0) * (dt * kv0);
+        ru = fma(1.0 / 3.0, dt * ku1, ru);
+        rv = fma(1.0 / 3.0, dt * kv1, rv);
+        ou = fma(0.5, dt * ku1, wyu[NG + q]);
+        ov = fma(0.5, dt * kv1, yv[q]);
+        st32(a.rt, p, ru);
+        st32(a.rt, nt + p, rv);
+      } else {
+        const double ru = fma(1.0 / 3.0, dt * ku0, rtu[q]), rv = fma(1.0 / 3.0, dt * kv0, rtv[q]);
+        ou = wyu[NG + q] + fma(1.0 / 6.0, dt * ku1, ru);
+        ov = yv[q] + fma(1.0 / 6.0, dt * kv1, rv);
+      }
+      st32(a.out, p, ou);
+      st32(a.out, nt + p, ov);
+      for_each_image_in_planes(g, i0, i1, i2, [&](long d) {
+        st32(a.out, (unsigned)d, ou);
+        st32(a.out, nt + (unsigned)d, ov);
+      });
+    }
+  }
+
+  // ---- residual L2 of y_n (stage 1's input): one partial per block; block 0 of the stages-3+4
+  // launch that follows reduces them in block order and updates the convergence state
+  if (fuse_l2) {
+    for (int o = 32; o > 0; o >>= 1) l2acc += __shfl_down(l2acc, o, 64);
+    if ((tid & 63) == 0) s_red[tid >> 6] = l2acc;
+    __syncthreads();
+    if (tid == 0) a.l2_partial[bid] = wave_sum<NT / 64>(s_red);
+  }
+  if (PAIR == 3 && a.l2_partial != nullptr && bid == 0) {
+    double acc = 0.0;
+    for (unsigned i = tid; i < nblk; i += NT) acc += a.l2_partial[i];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    __syncthreads();
+    if ((tid & 63) == 0) s_red[tid >> 6] = acc;
+    __syncthreads();
+    if (tid == 0) {
+      const double S = wave_sum<NT / 64>(s_red);
+      const double l = log10(1e-16 + sqrt(S / a.l2_volume));
+      SolverDeviceState *st = a.state;
+      st->steps = st->nchecks;  // completed steps when stage 1 read y_n
+      st->nchecks++;
+      record_check(st, l, a.l2_target);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// BC blocks: the outer layers of rows r0 .. r0+TYB-1 of one plane after stage s+1.
+//   box:  stage-s (u, v) on rows r0-10 .. r0+TYB+9 (virtual rows) x columns ilast-11 .. ilast+5
+//   R1:   rows r0-5 .. r0+TYB+4 x columns ilast-6 .. ilast+5: stage-(s+1) (u, v), from stage s's
+//         stencil (interior columns) and radiation condition (outer columns)
+//   then stage s+1's RHS at ilast and radiation terms on the tile rows, and the outputs.
+// Every global load is issued before the first barrier (one memory round trip per block); the
+// phases after it work from registers and LDS.
+// ---------------------------------------------------------------------------------------
+template <int PAIR>
+__device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int rb, const int plane, double *lds) {
+  constexpr int S0 = PAIR;
+  constexpr int NT = NRPYELL_BLOCK, TYB = NRPYELL_PAIR_BC_ROWS;
+  constexpr int NVR = TYB + 4 * NG, NC = 17, NBOX = NVR * NC, NRB = (NBOX + NT - 1) / NT;
+  constexpr int NR1 = TYB + 2 * NG, NI = 7 * NR1, NIR = (NI + NT - 1) / NT;
+  constexpr int NA0 = NR1 * 12, NAR = (NA0 + NT - 1) / NT;  // stage-s radiation terms
+  constexpr int NO0 = NR1 * 10, NOR = (NO0 + NT - 1) / NT;  // stage-s outer values
+  constexpr int NGE = NR1 * 24, NGER = (NGE + NT - 1) / NT;  // geometry of the R1 rows
+  constexpr int NARM = TYB * 30, NARMR = (NARM + NT - 1) / NT;  // i2 arms at (ilast, tile rows)
+  static_assert(TYB * 12 <= NT && TYB * 10 <= NT && TYB * 8 <= NT, "one tile-row item per thread");
+  double *bxu = lds, *bxv = lds + NRB * NT;  // [vr - (r0-10)][c - (ilast-11)]
+  double *q = lds + 2 * NRB * NT;
+  auto take = [&](int n) { double *r = q; q += n; return r; };
+  double(*sgeo)[6][4] = reinterpret_cast<double(*)[6][4]>(take(NR1 * 24));  // at each R1 row's source row
+  double(*syo)[2][5] = reinterpret_cast<double(*)[2][5]>(take(NR1 * 10));   // y at the outer points of R1
+  double(*srti)[2][5] = reinterpret_cast<double(*)[2][5]>(take(TYB * 10));  // (stages 3+4) running total there
+  double(*sarm)[10][3] = reinterpret_cast<double(*)[10][3]>(take(TYB * 30));  // stage-s u, v, y_u on i2 arms at ilast
+  double(*skl)[8] = reinterpret_cast<double(*)[8]>(take(TYB * 8));          // coefficients, source at (ilast, row)
+  double(*ru1)[12] = reinterpret_cast<double(*)[12]>(take(NR1 * 12));       // stage-(s+1) u on R1
+  double(*rv1)[12] = reinterpret_cast<double(*)[12]>(take(NR1 * 12));       // stage-(s+1) v on R1
+  double(*sk0)[2] = reinterpret_cast<double(*)[2]>(take(NR1 * 2));          // stage-s k at (ilast, R1 row)
+  double(*sA0)[2][6] = reinterpret_cast<double(*)[2][6]>(take(NR1 * 12));   // stage-s radiation terms, R1 rows
+  double(*srto)[2][5] = reinterpret_cast<double(*)[2][5]>(take(TYB * 10));  // running total at the outputs
+  double(*sk1)[2] = reinterpret_cast<double(*)[2]>(take(TYB * 2));
+  double(*sA1)[2][6] = reinterpret_cast<double(*)[2][6]>(take(TYB * 12));
+
+  const GridDims &g = a.g;
+  const int tid = threadIdx.x;
+  const unsigned nt = (unsigned)g.ntot, s1s = g.G0, s2 = (unsigned)(g.G0 * g.G1);
+  const int ilast = g.N0 + NG - 1, c_lo = ilast - 11;
+  const int r0 = NG + rb * TYB, i2 = NG + plane;
+  const double eta = a.eta, dt = a.dt;
+  const int done = *a.done;
+
+  // ---- loads
+  double lxu[NRB], lxv[NRB];
+#pragma unroll
+  for (int r = 0; r < NRB; r++) {
+    const int k = min(tid + NT * r, NBOX - 1), e = k / NC, c = k % NC;
+    int row, pl, dir;
+    vrow_source(g, r0 - 2 * NG + e, i2, row, pl, dir);
+    const unsigned p = (unsigned)pl * s2 + (unsigned)row * s1s + (c_lo + c);
+    lxu[r] = ld32(a.in, p);
+    lxv[r] = ld32(a.in, nt + p);
+  }
+  // this thread's stage-s stencil points on R1 (columns ilast-6 .. ilast)
+  double iK[NIR][5], ipsi[NIR], iAAc[NIR], izm[NIR][6], izp[NIR][6], iyu[NIR], iyv[NIR];
+#pragma unroll
+  for (int it = 0; it < NIR; it++) {
+    const int t = min(tid + NT * it, NI - 1);
+    const int e = t / 7, col = ilast - 6 + t % 7;
+    int row, pl, dir;
+    vrow_source(g, r0 - NG + e, i2, row, pl, dir);
+    const unsigned k2 = (unsigned)row * s1s + col, p = (unsigned)pl * s2 + k2;
+    double kw;
+    ld2x32(a.coef, 6 * k2, iK[it][0], iK[it][1]);
+    ld2x32(a.coef, 6 * k2 + 2, iK[it][2], iK[it][3]);
+    ld2x32(a.coef, 6 * k2 + 4, iK[it][4], kw);
+    ld2x32(a.src, 2 * p, ipsi[it], iAAc[it]);
+#pragma unroll
+    for (int k = 1; k <= 5; k++) {
+      izm[it][k] = ld32(a.in, (unsigned)wrap2_near(g, pl - k) * s2 + k2);
+      izp[it][k] = ld32(a.in, (unsigned)wrap2_near(g, pl + k) * s2 + k2);
+    }
+    iyu[it] = PAIR == 1 ? 0.0 : ld32(a.yn, p);
+    iyv[it] = PAIR == 1 ? 0.0 : ld32(a.yn, nt + p);
+  }
+  double lgeo[NGER];
+#pragma unroll
+  for (int r = 0; r < NGER; r++) {
+    const int k = min(tid + NT * r, NGE - 1), e = k / 24, pt = (k / 4) % 6, c = k % 4;
+    int row, pl, dir;
+    vrow_source(g, r0 - NG + e, i2, row, pl, dir);
+    lgeo[r] = ld32(a.geo, ((unsigned)pt * g.G1 + row) * 4 + c);
+  }
+  double lyo[NOR], lrt = 0.0;
+#pragma unroll
+  for (int r = 0; r < NOR; r++) {
+    const int k = min(tid + NT * r, NO0 - 1), e = k / 10, gf = (k % 10) / 5, pt = k % 5 + 1;
+    int row, pl, dir;
+    vrow_source(g, r0 - NG + e, i2, row, pl, dir);
+    lyo[r] = ld32(a.yn, (unsigned)pl * s2 + (unsigned)row * s1s + (ilast + pt) + gf * nt);
+  }
+  if (PAIR == 3) {
+    const int k = min(tid, TYB * 10 - 1), tr = k / 10, gf = (k % 10) / 5, pt = k % 5 + 1;
+    lrt = ld32((const double *)a.rt, i2 * s2 + (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + (ilast + pt) + gf * nt);
+  }
+  double larm[NARMR];
+#pragma unroll
+  for (int r = 0; r < NARMR; r++) {
+    const int k = min(tid + NT * r, NARM - 1), tr = k / 30, j = (k / 3) % 10, f = k % 3;
+    const int kk = j < 5 ? -(j + 1) : j - 4;  // plane offsets -1..-5, +1..+5
+    const unsigned pa = (unsigned)wrap2_near(g, i2 + kk) * s2 + (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + ilast;
+    larm[r] = f == 0 ? ld32(a.in, pa) : f == 1 ? ld32(a.in, nt + pa) : ld32(a.yn, pa);
+  }
+  double lkl = 0.0;
+  if (tid < TYB * 8) {
+    const int tr = tid / 8, c = tid % 8;
+    const unsigned k2 = (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + ilast;
+    lkl = c < 6 ? ld32(a.coef, 6 * k2 + c) : ld32(a.src, 2 * (i2 * s2 + k2) + (c - 6));
+  }
+  if (done) return;
+
+  // ---- everything to LDS
+#pragma unroll
+  for (int r = 0; r < NRB; r++) {
+    bxu[tid + NT * r] = lxu[r];
+    bxv[tid + NT * r] = lxv[r];
+  }
+#pragma unroll
+  for (int r = 0; r < NGER; r++)
+    if (tid + NT * r < NGE) (&sgeo[0][0][0])[tid + NT * r] = lgeo[r];
+#pragma unroll
+  for (int r = 0; r < NOR; r++)
+    if (tid + NT * r < NO0) (&syo[0][0][0])[tid + NT * r] = lyo[r];
+  if (PAIR == 3 && tid < TYB * 10) (&srti[0][0][0])[tid] = lrt;
+#pragma unroll
+  for (int r = 0; r < NARMR; r++)
+    if (tid + NT * r < NARM) (&sarm[0][0][0])[tid + NT * r] = larm[r];
+  if (tid < TYB * 8) (&skl[0][0])[tid] = lkl;
+  __syncthreads();
+  // box value at virtual row vr (absolute) and column c (absolute)
+  auto BX_U = [&](int vr, int c) { return bxu[(vr - (r0 - 2 * NG)) * NC + (c - c_lo)]; };
+  auto BX_V = [&](int vr, int c) { return bxv[(vr - (r0 - 2 * NG)) * NC + (c - c_lo)]; };
+
+  // ---- stage s at the R1 points of columns ilast-6 .. ilast
+#pragma unroll
+  for (int it = 0; it < NIR; it++) {
+    const int t = tid + NT * it;
+    if (t < NI) {
+      const int e = t / 7, col = ilast - 6 + t % 7, vr = r0 - NG + e;
+      int row, pl, dir;
+      vrow_source(g, vr, i2, row, pl, dir);
+      double xm[6], xp[6], ym[6], yp[6];
+      const double uu = BX_U(vr, col), vv = BX_V(vr, col);
+#pragma unroll
+      for (int k = 1; k <= 5; k++) {
+        xm[k] = BX_U(vr, col - k); xp[k] = BX_U(vr, col + k);
+        ym[k] = BX_U(vr - dir * k, col); yp[k] = BX_U(vr + dir * k, col);  // the source's orientation
+      }
+      double ku, kv;
+      point_rhs<double>(uu, vv, ipsi[it], iAAc[it], iK[it], xm, xp, ym, yp, izm[it], izp[it], eta, ku, kv);
+      ru1[e][col - (ilast - 6)] = next_input<S0>(ku, dt, PAIR == 1 ? uu : iyu[it]);
+      rv1[e][col - (ilast - 6)] = next_input<S0>(kv, dt, PAIR == 1 ? vv : iyv[it]);
+      if (col == ilast) {
+        sk0[e][0] = ku;
+        sk0[e][1] = kv;
+      }
+    }
+  }
+  // ---- stage s's radiation terms A_pt (pt 0..5, both gridfunctions) on the R1 rows
+#pragma unroll
+  for (int it = 0; it < NAR; it++) {
+    const int t = tid + NT * it;
+    if (t < NA0) {
+      const int e = t / 12, gf = (t % 12) / 6, pt = t % 6, vr = r0 - NG + e;
+      int row, pl, dir;
+      vrow_source(g, vr, i2, row, pl, dir);
+      const int col = ilast + pt;
+      sA0[e][gf][pt] = gf == 0
+          ? radiation_A([&](int d0, int d1) { return BX_U(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_u,
+                        g.invdx0, g.invdx1)
+          : radiation_A([&](int d0, int d1) { return BX_V(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_v,
+                        g.invdx0, g.invdx1);
+    }
+  }
+  __syncthreads();
+  // ---- stage-(s+1) (u, v) on the outer columns of R1; the stage-s running total at the outputs
+#pragma unroll
+  for (int it = 0; it < NOR; it++) {
+    const int t = tid + NT * it;
+    if (t < NO0) {
+      const int e = t / 10, gf = (t % 10) / 5, pt = t % 5 + 1;
+      const double rhs = fma(sgeo[e][pt][3], sk0[e][gf] - sA0[e][gf][0], sA0[e][gf][pt]);
+      const double y = syo[e][gf][pt - 1];
+      (gf == 0 ? ru1 : rv1)[e][6 + pt] = next_input<S0>(rhs, dt, y);
+      const int tr = e - NG;
+      if (tr >= 0 && tr < TYB)
+        srto[tr][gf][pt - 1] =
+            PAIR == 1 ? (1.0 / 6.0) * (dt * rhs) : fma(1.0 / 3.0, dt * rhs, srti[tr][gf][pt - 1]);
+    }
+  }
+  __syncthreads();
+
+  // ---- stage s+1 at (ilast, tile rows): the RHS from the stage-(s+1) u on R1 and on the i2 arm
+  if (tid < TYB && r0 + tid < g.N1 + NG) {
+    const int e = tid + NG;
+    double xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+#pragma unroll
+    for (int k = 1; k <= 5; k++) {
+      const double *m = sarm[tid][k - 1], *pp = sarm[tid][4 + k];
+      zm[k] = next_input<S0>(fma(-eta, m[0], m[1]), dt, PAIR == 1 ? m[0] : m[2]);
+      zp[k] = next_input<S0>(fma(-eta, pp[0], pp[1]), dt, PAIR == 1 ? pp[0] : pp[2]);
+      xm[k] = ru1[e][6 - k]; xp[k] = ru1[e][6 + k];
+      ym[k] = ru1[e - k][6]; yp[k] = ru1[e + k][6];
+    }
+    const double K[5] = {skl[tid][0], skl[tid][1], skl[tid][2], skl[tid][3], skl[tid][4]};
+    double ku, kv;
+    point_rhs<double>(ru1[e][6], rv1[e][6], skl[tid][6], skl[tid][7], K, xm, xp, ym, yp, zm, zp, eta, ku, kv);
+    sk1[tid][0] = ku;
+    sk1[tid][1] = kv;
+  }
+  // ---- stage s+1's radiation terms on the tile rows
+  if (tid < TYB * 12) {
+    const int tr = tid / 12, gf = (tid % 12) / 6, pt = tid % 6, e = tr + NG;
+    if (r0 + tr < g.N1 + NG) {
+      const double(*R)[12] = gf == 0 ? ru1 : rv1;
+      sA1[tr][gf][pt] = radiation_A([&](int d0, int d1) { return R[e + d1][6 + pt + d0]; }, pt, sgeo[e][pt],
+                                    gf == 0 ? a.cOB_u : a.cOB_v, g.invdx0, g.invdx1);
+    }
+  }
+  __syncthreads();
+  // ---- outputs on the outer layers of the tile rows, and their parity images
+  if (tid < TYB * 10) {
+    const int tr = tid / 10, gf = (tid % 10) / 5, pt = tid % 5 + 1, row = r0 + tr, e = tr + NG;
+    if (row < g.N1 + NG) {
+      const unsigned p = (unsigned)i2 * s2 + (unsigned)row * s1s + (ilast + pt);
+      const double rhs = fma(sgeo[e][pt][3], sk1[tr][gf] - sA1[tr][gf][0], sA1[tr][gf][pt]);
+      const double rt = srto[tr][gf][pt - 1], y = syo[e][gf][pt - 1];
+      double o;
+      if (PAIR == 1) {
+        st32(a.rt, gf * nt + p, fma(1.0 / 3.0, dt * rhs, rt));
+        o = fma(0.5, dt * rhs, y);
+      } else {
+        o = y + fma(1.0 / 6.0, dt * rhs, rt);
+      }
+      st32(a.out, gf * nt + p, o);
+      for_each_image_in_planes(g, ilast + pt, row, i2, [&](long d) { st32(a.out, gf * nt + (unsigned)d, o); });
+    }
+  }
+}
+
+// One configuration: the nbc_rows*N2 BC blocks first, then the interior blocks in XCD-aware
+// order.
+template <int PAIR, int TX, int PL>
+__global__ void __launch_bounds__(NRPYELL_BLOCK) substep2_kernel(SubstepArgs<double> a) {
+  constexpr int NL = interior2_lds<TX, PL>() > bc2_lds() ? interior2_lds<TX, PL>() : bc2_lds();
+  __shared__ double lds[NL];
+  const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
+  if (L < nb) {
+    const unsigned pln = div_by(L, a.nbc_mul);
+    bc2_body<PAIR>(a, (int)(L - pln * (unsigned)a.nbc_rows), (int)pln, lds);
+    return;
+  }
+  const unsigned ni = gridDim.x - nb, q = xcd_logical(L - nb, ni), ng = (unsigned)(a.g.N2 / PL);
+  const unsigned tile = div_by(q, a.ng_mul);
+  interior2_body<PAIR, TX, PL>(a, (int)tile, (int)(q - tile * ng), q, ni, lds);
+}
+
+template <int PAIR>
+static void launch_pair(const SubstepArgs<double> &a, hipStream_t st) {
+  dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(NRPYELL_BLOCK);
+  if (a.pl == 2) {
+    switch (a.tx) {
+      case 64: hipLaunchKernelGGL((substep2_kernel<PAIR, 64, 2>), grd, blk, 0, st, a); break;
+      default: hipLaunchKernelGGL((substep2_kernel<PAIR, 32, 2>), grd, blk, 0, st, a); break;
+    }
+    return;
+  }
+  switch (a.tx) {
+    case 16: hipLaunchKernelGGL((substep2_kernel<PAIR, 16, 1>), grd, blk, 0, st, a); break;
+    case 64: hipLaunchKernelGGL((substep2_kernel<PAIR, 64, 1>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((substep2_kernel<PAIR, 32, 1>), grd, blk, 0, st, a); break;
+  }
+}
+
+void launch_substep_pair(const SubstepArgs<double> &a, int first_stage, hipStream_t st) {
+  if (first_stage == 1) launch_pair<1>(a, st);
+  else launch_pair<3>(a, st);
+}

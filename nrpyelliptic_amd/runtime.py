@@ -30,7 +30,7 @@ class solve_options(C.Structure):
                 ("log10_target", C.c_double), ("max_steps", C.c_int), ("fp32_switch_log10", C.c_double),
                 ("fp32_plateau_steps", C.c_int), ("integration_radius", C.c_double),
                 ("tile_i0", C.c_int), ("planes", C.c_int), ("outer_bc", C.c_int), ("outer_bc_speed", C.c_double),
-                ("stop_rule", C.c_int)]
+                ("stop_rule", C.c_int), ("substeps_per_launch", C.c_int)]
 
 
 # stopping rules of the resident solver (nrpyell_solve_options.stop_rule)
@@ -101,6 +101,7 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_time_substep.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     L.nrpyell_solver_timed_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.nrpyell_solver_substeps_per_launch.argtypes = [vp]
     L.nrpyell_batch_create.argtypes = [C.POINTER(C.POINTER(abi.griddata_struct)), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options),
                                        C.POINTER(vp)]
@@ -486,6 +487,11 @@ class Solver:
         vals = (C.c_double * max_entries)()
         n = lib().nrpyell_solver_history(self.h, max_entries, steps, vals)
         return np.array(steps[:n]), np.array(vals[:n])
+
+    @property
+    def substeps_per_launch(self) -> int:
+        """1 or 2: RK4 substeps per launch of the FP64 steps."""
+        return lib().nrpyell_solver_substeps_per_launch(self.h)
 
     def time_substep(self, stage: int, n: int) -> float:
         v = C.c_double()

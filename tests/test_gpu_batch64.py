@@ -4,8 +4,11 @@ mixing Nxx0 in {96,128,160}, Nxx1 in {96,128}, Nxx2 in {8,16}) relaxed together 
 log10 L2 <= -10 with tile shapes from the autotuner. Configurations whose relaxation diverges at
 the reference's CFL factor 0.7 (the reference's C diverges on them too; tests/test_oracle_cfl.py)
 are relaxed again from the zero guess with the wave speed at CFL 0.5 (runtime.relax_batch).
-Every configuration converges; members are spot-checked against their own single-configuration
-solves (bit-identical states)."""
+Every configuration converges; the members that fall back are the ones the reference's C diverges
+on, and each stops at the reference's own step at CFL 0.5; members are spot-checked against their
+own single-configuration solves (bit-identical states)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -27,6 +30,12 @@ def test_all_64_seed42_configurations_converge():
         assert all(r.log10_l2 <= -10.0 for r in res)
         assert all(probs[k].cfl == runtime.FALLBACK_CFL for k in fell)
         assert 0 < len(fell) < 16
+        # the reference's C diverges on exactly these members and, at the fallback CFL, stops at
+        # the same step as the batched solver (tests/golden/cfl_fallback_seed42.npz)
+        fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "cfl_fallback_seed42.npz"))
+        assert fell == [int(k) for k in fx["indices"]]
+        for k, n in zip(fx["indices"], fx["steps_fallback"]):
+            assert res[int(k)].steps == int(n), (int(k), res[int(k)].steps, int(n))
         # spot checks: two members that fell back, two that did not
         plain = [k for k in range(64) if not info[k]["fallback"]]
         for k in fell[:2] + plain[:2]:

@@ -96,7 +96,8 @@ def test_reference_diverges_at_07_and_relaxes_at_fallback():
         assert 25 * (m - 1) <= 4000, k
         assert l2 <= -10.0 and n < 30000, k
         hf = hf[~np.isnan(hf)]
-        assert np.nanmax(hf[1:]) < hf[0] + 1.0, k  # no growth at the fallback factor
+        # transients of a decade early on, no growth: the residual never climbs 2 decades
+        assert np.max(hf) < hf[0] + 2.0 and hf[-1] < hf[0] - 5.0, k
 
 
 def test_oracle_diverges_at_07_and_not_at_05_on_a_member():
