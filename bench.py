@@ -210,118 +210,90 @@ def pmc_probe():
           "launch (80 launches), 5 copies of 64 MiB", flush=True)
 
 
-def nn_seeded_time_to_residual(P, S_plain, r_plain, n_held_out=8):
-    """Time to log10 L2 <= -10: plain FP64 from the zero guess vs the network guess (with the
-    zero-guess fallback) and the FP32->FP64 solver, on the bench problem and on held-out
-    configurations (seed 900, 128x128x16) the network never saw."""
+def nn_seeded_paths(P, n_held_out=256):
+    """The learned initial guess against the plain method.
+
+    Bench problem (equal-mass boosted punctures, 128x128x16): the reference's path (zero guess,
+    configured eta, FP64) against the network's guess with the FP32 -> FP64 solver, at the
+    configured eta and at the damping tuned to that start (autotune.EtaTuner, calibrated in FP32
+    on a seed-901 configuration of the grid: neither the bench problem, the held-out set nor the
+    training set). Per path: steps and seconds to log10 L2 <= -10 (seeding included), max |u - u*|
+    against the reference's own u* (tests/golden/deep_equal_mass_boosted_128x128x16.npz: the
+    reference's C relaxed to log10 L2 = -14), and the accuracy-matched time: the seconds of the
+    first residual target in -10, -10.5, ... whose u is within 1e-10 of u*.
+
+    Held-out (seed 900, n configurations at 128x128x16): learned.evaluate_batched, both starts
+    on the batched FP64 solver, the network's at the damping EtaTuner gives it in FP64."""
     import numpy as np
     from nrpyelliptic_amd import config, learned, runtime
+    from nrpyelliptic_amd.autotune import EtaTuner
     from nrpyelliptic_amd.mlp import InitialGuessMLP
     w = os.path.join(ROOT, "nrpyelliptic_amd", "data", "mlp_initial_guess.npz")
     if not os.path.exists(w):
         return None
     net = InitialGuessMLP(seed=7)
     net.load(w)
+    u_star = np.load(os.path.join(ROOT, "tests", "golden", "deep_equal_mass_boosted_128x128x16.npz"))["u_deep"]
+    G0, G1, G2 = P.Ng
 
-    def nn_solve(prob, plain):
-        used, r0, r1 = learned.seed_with_fallback(net, prob)
-        S = prob.solver(precision=32, log10_target=-10.0, max_steps=10000)
-        r = S.solve()
-        S.close()
-        return {"plain_steps": plain.steps, "plain_s": plain.seconds, "nn_steps": r.steps, "nn_fp32_steps": r.fp32_steps,
-                "nn_s": r.seconds, "nn_used": bool(used), "log10_res0_zero": r0, "log10_res0_nn": r1,
-                "converged": bool(plain.converged and r.converged)}
-
-    # damping strength tuned per grid (EtaTuner) on a calibration configuration of the grid
-    # (seed 901: neither the bench problem, the held-out set nor the training set)
-    from nrpyelliptic_amd.autotune import EtaTuner
-    tuner = EtaTuner(cache_path=None)
     t0 = time.perf_counter()
     calib = runtime.Problem(config.random_batch(1, seed=901, grids=False, Nxx=(128, 128, 16))[0])
-    f_nn = tuner.choose(calib, lambda Q: learned.seed_with_fallback(net, Q), "nn")
-    f_zero = tuner.choose(calib, lambda Q: Q.set("y_n_gfs", np.zeros(2 * Q.ntot)), "zero")
+    nn_start = lambda Q: learned.seed_with_fallback(net, Q)  # noqa: E731
+    f_nn32 = EtaTuner(cache_path=None, precision=32).choose(calib, nn_start, "nn")
+    f_nn64 = EtaTuner(cache_path=None, precision=64).choose(calib, nn_start, "nn")
+    f_zero = EtaTuner(cache_path=None, precision=64).choose(calib, lambda Q: Q.zero_guess(), "zero")
     calib.close()
     tune_s = time.perf_counter() - t0
 
-    def tuned_solves(prob, plain):
-        """NN-seeded FP32->FP64 solve at f_nn x eta and plain FP64 at f_zero x eta, against
-        the plain FP64 solve at the configured eta; eta restored afterwards."""
-        eta0 = prob.gd.params.eta_damping
+    eta0 = P.gd.params.eta_damping
+
+    def run(nn, factor, target):
         try:
-            used, _ = learned.seed_with_tuned_damping(net, prob, f_nn, f_zero)
-            S = prob.solver(precision=32, log10_target=-10.0, max_steps=20000)
-            rn = S.solve()
-            S.close()
-            prob.gd.params.eta_damping = f_zero * eta0
-            prob.set("y_n_gfs", np.zeros(2 * prob.ntot))
-            S = prob.solver(log10_target=-10.0, max_steps=20000)
-            rz = S.solve()
+            seed_s = 0.0
+            if nn:
+                t = time.perf_counter()
+                used, _ = learned.seed_with_tuned_damping(net, P, factor, 1.0)
+                seed_s = time.perf_counter() - t
+            else:
+                used = False
+                P.gd.params.eta_damping = factor * eta0
+                P.zero_guess()
+            S = P.solver(precision=32 if nn else 64, log10_target=target, max_steps=40000)
+            r = S.solve()
             S.close()
         finally:
-            prob.gd.params.eta_damping = eta0
-        return {"plain_steps": plain.steps, "plain_s": plain.seconds, "nn_steps": rn.steps, "nn_fp32_steps": rn.fp32_steps,
-                "nn_s": rn.seconds, "nn_used": bool(used), "plain_tuned_steps": rz.steps, "plain_tuned_s": rz.seconds,
-                "converged": bool(plain.converged and rn.converged and rz.converged)}
+            P.gd.params.eta_damping = eta0
+        u = P.get("y_n_gfs")[: P.ntot].reshape(G2, G1, G0)[5:-5, 5:-5, 5:-5]
+        return {"steps": r.steps, "fp32_steps": r.fp32_steps, "seconds": r.seconds + seed_s, "seed_seconds": seed_s,
+                "log10_l2": r.log10_l2, "converged": bool(r.converged), "nn_used": bool(used),
+                "max_abs_u_minus_ustar": float(np.max(np.abs(u - u_star)))}
 
-    bench_cfg = nn_solve(P, r_plain)
-    bench_tuned = tuned_solves(P, r_plain)
-    # accuracy of the two paths on the bench problem: max |u - u*| with u* converged to 1e-14
-    interior = np.zeros(P.Ng[::-1], dtype=bool)
-    interior[5:-5, 5:-5, 5:-5] = True
-    interior = interior.ravel()
-    eta0 = P.gd.params.eta_damping
-    errs = {}
-    try:
-        for name, eta, nn, tgt in (("star", eta0, False, -14.0), ("plain", eta0, False, -10.0),
-                                   ("nn_tuned", f_nn * eta0, True, -10.0)):
-            if nn:
-                P.gd.params.eta_damping = eta0
-                learned.seed_with_tuned_damping(net, P, f_nn, f_zero)
-            else:
-                P.gd.params.eta_damping = eta
-                P.set("y_n_gfs", np.zeros(2 * P.ntot))
-            S = P.solver(precision=32 if nn else 64, log10_target=tgt, max_steps=40000)
-            S.solve()
-            S.close()
-            errs[name] = P.get("y_n_gfs")[: P.ntot][interior].copy()
-    finally:
-        P.gd.params.eta_damping = eta0
-    bench_tuned["max_abs_u_error_plain"] = float(np.max(np.abs(errs["plain"] - errs["star"])))
-    bench_tuned["max_abs_u_error_nn_tuned"] = float(np.max(np.abs(errs["nn_tuned"] - errs["star"])))
-    rows, rows_t = [], []
-    for cfg in config.random_batch(n_held_out, seed=900, grids=False, Nxx=(128, 128, 16)):
-        Q = runtime.Problem(cfg)
-        S = Q.solver(log10_target=-10.0, max_steps=10000)
-        Q.set("y_n_gfs", np.zeros(2 * Q.ntot))
-        try:
-            rz = S.solve()
-        except runtime.NrpyellError:
-            rz = None
-        S.close()
-        if rz is not None and rz.converged:
-            rows.append(nn_solve(Q, rz))
-            rows_t.append(tuned_solves(Q, rz))
-        Q.close()
-    batch = batched_nn_tuned(net, f_nn, f_zero)
-    ok = [r for r in rows if r["converged"]]
-    okt = [r for r in rows_t if r["converged"]]
-    return {"bench_problem": bench_cfg, "speedup_bench_problem": bench_cfg["plain_s"] / bench_cfg["nn_s"],
-            "held_out": {"n": len(ok), "speedup_time": sum(r["plain_s"] for r in ok) / max(sum(r["nn_s"] for r in ok), 1e-12),
-                         "speedup_steps": sum(r["plain_steps"] for r in ok) / max(sum(r["nn_steps"] for r in ok), 1),
-                         "nn_used_frac": sum(r["nn_used"] for r in ok) / max(len(ok), 1)},
-            "eta_tuned": {
-                "factor_nn": f_nn, "factor_zero": f_zero, "calibration_seconds": tune_s,
-                "calibration": "seed 901 configuration, 128x128x16: fewest steps to -10 among the factors whose "
-                               "u is within half the reference path's error of a solve to -14",
-                "bench_problem": bench_tuned,
-                "speedup_bench_problem": bench_tuned["plain_s"] / bench_tuned["nn_s"],
-                "speedup_bench_problem_vs_plain_tuned": bench_tuned["plain_tuned_s"] / bench_tuned["nn_s"],
-                "held_out": {"n": len(okt),
-                             "speedup_time": sum(r["plain_s"] for r in okt) / max(sum(r["nn_s"] for r in okt), 1e-12),
-                             "speedup_steps": sum(r["plain_steps"] for r in okt) / max(sum(r["nn_steps"] for r in okt), 1),
-                             "speedup_time_vs_plain_tuned": sum(r["plain_tuned_s"] for r in okt)
-                             / max(sum(r["nn_s"] for r in okt), 1e-12)},
-                "batched_32_seed900": batch}}
+    paths = {"plain_fp64": (False, 1.0), "nn_mixed_configured_eta": (True, 1.0),
+             "nn_mixed_tuned_eta": (True, f_nn32)}
+    bench = {}
+    for name, (nn, f) in paths.items():
+        row = run(nn, f, -10.0)
+        for tgt in (-10.0, -10.5, -11.0, -11.5, -12.0, -12.5, -13.0, -13.5):
+            acc = row if tgt == -10.0 else run(nn, f, tgt)
+            if acc["max_abs_u_minus_ustar"] <= 1e-10:
+                row["accuracy_matched"] = {"target_log10": tgt, "steps": acc["steps"], "seconds": acc["seconds"],
+                                           "max_abs_u_minus_ustar": acc["max_abs_u_minus_ustar"]}
+                break
+        bench[name] = row
+    plain = bench["plain_fp64"]
+    for name in ("nn_mixed_configured_eta", "nn_mixed_tuned_eta"):
+        r = bench[name]
+        r["speedup_time_to_residual"] = plain["seconds"] / r["seconds"]
+        if "accuracy_matched" in r and "accuracy_matched" in plain:
+            r["speedup_accuracy_matched"] = plain["accuracy_matched"]["seconds"] / r["accuracy_matched"]["seconds"]
+    held = learned.evaluate_batched(
+        net, config.random_batch(n_held_out, seed=900, grids=False, Nxx=(128, 128, 16)), f_nn64, f_zero,
+        log=lambda s: None)
+    return {"eta_factors": {"nn_fp32_fp64": f_nn32, "nn_fp64": f_nn64, "zero": f_zero},
+            "calibration": "EtaTuner on a seed-901 configuration of the 128x128x16 grid: fewest steps to -10 among "
+                           "the eta factors whose u is within half the reference path's error of a solve to -14, "
+                           "in the precision the path runs",
+            "calibration_seconds": tune_s, "bench_problem": bench, "held_out_batched": held}
 
 
 def batched_solve(n=64, seed=42):
@@ -353,34 +325,14 @@ def batched_solve(n=64, seed=42):
     return out
 
 
-def batched_nn_tuned(net, f_nn, f_zero, n=32, seed=900):
-    """n held-out configurations (seed 900, 128x128x16) relaxed together to log10 L2 <= -10 by
-    the batched FP64 solver: from the zero guess at the configured eta, then from the network's
-    guess at the damping tuned to that start (each configuration's own eta)."""
-    from nrpyelliptic_amd import config, learned, runtime
-    probs = [runtime.Problem(c) for c in config.random_batch(n, seed=seed, grids=False, Nxx=(128, 128, 16))]
-    eta0 = [P.gd.params.eta_damping for P in probs]
+def load_dropin_kernels():
+    """Kernel durations of the reference-surface entry points from the committed rocprofv3
+    summary (null when absent)."""
     try:
-        B = runtime.BatchSolver(probs, log10_target=-10.0, max_steps=20000)
-        rz, sz = B.solve()
-        B.close()
-        used = [learned.seed_with_tuned_damping(net, P, f_nn, f_zero)[0] for P in probs]
-        B = runtime.BatchSolver(probs, log10_target=-10.0, max_steps=20000)
-        rn, sn = B.solve()
-        B.close()
-    finally:
-        for P, e in zip(probs, eta0):
-            P.gd.params.eta_damping = e
-    both = [k for k in range(n) if rz[k].converged and rn[k].converged]
-    out = {"configurations": n, "converged_zero": sum(bool(r.converged) for r in rz),
-           "converged_nn_tuned": sum(bool(r.converged) for r in rn), "converged_both": len(both),
-           "nn_used": int(sum(used)),
-           "zero_guess_seconds": sz, "nn_tuned_seconds": sn, "speedup_time": sz / max(sn, 1e-12),
-           "mean_steps_zero": sum(rz[k].steps for k in both) / max(len(both), 1),
-           "mean_steps_nn_tuned": sum(rn[k].steps for k in both) / max(len(both), 1)}
-    for P in probs:
-        P.close()
-    return out
+        with open(os.path.join(ROOT, "profiles", "dropin_kernels_128x128x16.json")) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def load_traffic(spl=2):
@@ -410,6 +362,9 @@ def main():
                     help="only the warmup and the timed steps (the region the profiles cover)")
     ap.add_argument("--admbase-only", action="store_true",
                     help="only the solve and the Einstein Toolkit consumer path (ADMBase on 128^3 points)")
+    ap.add_argument("--dropin-only", action="store_true",
+                    help="only the reference-surface entry points, 200 calls each (the region the drop-in "
+                         "kernel profile covers)")
     args = ap.parse_args()
     if args.pmc_probe:
         pmc_probe()
@@ -432,6 +387,11 @@ def main():
     npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
     S = P.solver(steps_per_graph=100)
 
+    if args.dropin_only:
+        P.set("y_n_gfs", np.zeros(2 * P.ntot))
+        S.timed_steps(1000)  # a relaxed, nonzero state
+        print(json.dumps({"dropin_call_us": dropin_call_us(P, n=200)}), flush=True)
+        return
     if args.admbase_only:
         # 3000 graph-replayed steps relax u to ~1e-10 (the solve stops at 3144); the kernel's
         # cost does not depend on the values
@@ -462,7 +422,7 @@ def main():
     # time to residual 1e-10 from the zero guess (the reference's every-step criterion)
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
     r = S.solve()
-    nn = nn_seeded_time_to_residual(P, S, r)
+    nn = nn_seeded_paths(P)
     # the Einstein Toolkit thorn's solver variant on its default grid (N0, N1, N2 = 128, 128, 16):
     # the same relaxation with the thorn's 6th-order Sommerfeld outer BC and wave speed
     S_et = P.solver(steps_per_graph=100, outer_bc="sommerfeld6")
@@ -511,13 +471,14 @@ def main():
                      "substeps_per_launch": spl, "us_per_launch": us_per_launch,
                      "bytes_per_launch": bytes_per_launch, "stage_us": stage_us},
         "dropin_call_us": dropin_us,
+        "dropin_kernel_us": load_dropin_kernels(),
         "et_consumer_admbase": admbase,
         "batched_64_seed42": batched,
         "cpu_baseline": cpu,
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
                              "log10_l2": r.log10_l2, "converged": bool(r.converged),
                              "reference_steps": 3144, "check_interval_steps": 1,
-                             "nn_seeded_mixed_vs_plain_fp64": nn, "et_thorn_sommerfeld_bc": et_thorn},
+                             "learned_initial_guess": nn, "et_thorn_sommerfeld_bc": et_thorn},
     }
     print(json.dumps(line), flush=True)
 

@@ -98,9 +98,9 @@ class TileTuner:
 ETA_FACTORS = (1.0, 1.4, 2.0, 2.8, 4.0, 5.6)  # multiples of the configured eta
 
 
-def _eta_key(prob: Problem, start: str, target: float) -> str:
+def _eta_key(prob: Problem, start: str, target: float, precision: int) -> str:
     p = prob.gd.params
-    return f"{p.Nxx0}x{p.Nxx1}x{p.Nxx2}_A{p.AMAX:g}_w{p.SINHWAA:g}_{start}_t{target:g}"
+    return f"{p.Nxx0}x{p.Nxx1}x{p.Nxx2}_A{p.AMAX:g}_w{p.SINHWAA:g}_{start}_t{target:g}_p{precision}"
 
 
 class EtaTuner:
@@ -118,14 +118,16 @@ class EtaTuner:
     to be measured): u* = a solve from the zero guess at the configured eta to target - 4; the
     reference error e_ref = max |u - u*| of the zero-guess solve at the configured eta to the
     target (the reference driver's path); then, for every factor of the configured eta, a
-    solve from the initial state `start(prob)` writes to the target. The factor with the
-    fewest steps among those whose error is at most `margin` x e_ref is cached per (grid,
-    start, target), 1.0 if none qualifies; configurations of the grid with other punctures
+    solve from the initial state `start(prob)` writes to the target, in `precision` (32 for the
+    FP32 -> FP64 path the network's start runs on). The factor with the fewest steps among those
+    whose error is at most `margin` x e_ref is cached per (grid, start, target, precision), 1.0 if
+    none qualifies; configurations of the grid with other punctures
     and foci reuse it. The problem's eta and state are restored afterwards. `cache_path=None`
     keeps the table in memory only."""
 
     def __init__(self, cache_path: Optional[str] = None, factors: Sequence[float] = ETA_FACTORS,
-                 target: float = -10.0, max_steps: int = 40000, margin: float = 0.5):
+                 target: float = -10.0, max_steps: int = 40000, margin: float = 0.5, precision: int = 64):
+        self.precision = precision  # of the solves from `start` (the path the factor will serve)
         self.cache_path = cache_path
         self.factors = tuple(factors)
         self.target = target
@@ -137,11 +139,12 @@ class EtaTuner:
             with open(cache_path) as f:
                 self.table = json.load(f)
 
-    def _solve(self, prob: Problem, eta: float, start: Callable[[Problem], object], target: float):
-        """(steps, interior u) of one FP64 solve, or None if it does not converge."""
+    def _solve(self, prob: Problem, eta: float, start: Callable[[Problem], object], target: float,
+               precision: int = 64):
+        """(steps, interior u) of one solve, or None if it does not converge."""
         prob.gd.params.eta_damping = eta
         start(prob)  # after eta: a seed writes v = eta u
-        s = Solver(prob, log10_target=target, max_steps=self.max_steps)
+        s = Solver(prob, log10_target=target, max_steps=self.max_steps, precision=precision)
         try:
             r = s.solve()
         except NrpyellError:
@@ -166,7 +169,7 @@ class EtaTuner:
             out = {"reference_steps": ref[0], "reference_error": float(np.max(np.abs(ref[1] - u_star))),
                    "factors": {}}
             for f in self.factors:
-                got = self._solve(prob, f * eta0, start, self.target)
+                got = self._solve(prob, f * eta0, start, self.target, self.precision)
                 if got is not None:
                     out["factors"][f] = (got[0], float(np.max(np.abs(got[1] - u_star))))
         finally:
@@ -177,7 +180,7 @@ class EtaTuner:
 
     def choose(self, prob: Problem, start: Callable[[Problem], object], start_name: str) -> float:
         """Factor of the configured eta: fewest steps at no loss of accuracy (1.0 by default)."""
-        k = _eta_key(prob, start_name, self.target)
+        k = _eta_key(prob, start_name, self.target, self.precision)
         if k not in self.table:
             m = self.measure(prob, start)
             best = 1.0

@@ -1,8 +1,10 @@
 """Puncture configurations and grid parameters, mapped onto the reference paramstruct.
 
 The shipped defaults are NRPyElliptic_Ccodes/free_parameters.h:1-41 (CFL 0.7, AMAX 1e6,
-SINHWAA 0.07, bScale 5, eta 20).  `named_workload()` builds the seeded synthetic
-workloads the benchmarks and tests use (all draws from numpy's default_rng).
+SINHWAA 0.07, bScale 5, eta 20). The workloads the benchmarks and tests use:
+shipped_free_parameters, axisymmetric_spinning (64x64x4), equal_mass_boosted (128x128x16),
+unequal_mass_q4 (seed-7 spins) and random_batch (seeded draws of q, D, momenta, spins and grids;
+all draws from numpy's default_rng).
 """
 from __future__ import annotations
 

@@ -319,12 +319,27 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
 // Every global load is issued before the first barrier (one memory round trip per block); the
 // phases after it work from registers and LDS.
 // ---------------------------------------------------------------------------------------
+// Stage-s stencil point t of a BC block: (R1 row e, column col); t < 6 TYB: a tile row (e = NG
+// + t / 6) and a column ilast-6 .. ilast-1; then column ilast on R1 rows 0 .. TYB+2NG-1.
+__device__ __forceinline__ void r1_stencil_point(int t, int ilast, int &e, int &col) {
+  constexpr int TYB = NRPYELL_PAIR_BC_ROWS;
+  if (t < 6 * TYB) {
+    e = NG + t / 6;
+    col = ilast - 6 + t % 6;
+  } else {
+    e = t - 6 * TYB;
+    col = ilast;
+  }
+}
+
 template <int PAIR>
 __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int rb, const int plane, double *lds) {
   constexpr int S0 = PAIR;
   constexpr int NT = NRPYELL_BLOCK, TYB = NRPYELL_PAIR_BC_ROWS;
   constexpr int NVR = TYB + 4 * NG, NC = 17, NBOX = NVR * NC, NRB = (NBOX + NT - 1) / NT;
-  constexpr int NR1 = TYB + 2 * NG, NI = 7 * NR1, NIR = (NI + NT - 1) / NT;
+  // stage-s stencils: columns ilast-6 .. ilast-1 on the tile rows (the one-sided d_0 of stage
+  // s+1's radiation terms reads them there only) and column ilast on every R1 row (d_1 there)
+  constexpr int NR1 = TYB + 2 * NG, NI = 6 * TYB + NR1, NIR = (NI + NT - 1) / NT;
   constexpr int NA0 = NR1 * 12, NAR = (NA0 + NT - 1) / NT;  // stage-s radiation terms
   constexpr int NO0 = NR1 * 10, NOR = (NO0 + NT - 1) / NT;  // stage-s outer values
   constexpr int NGE = NR1 * 24, NGER = (NGE + NT - 1) / NT;  // geometry of the R1 rows
@@ -369,8 +384,8 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   double iK[NIR][5], ipsi[NIR], iAAc[NIR], izm[NIR][6], izp[NIR][6], iyu[NIR], iyv[NIR];
 #pragma unroll
   for (int it = 0; it < NIR; it++) {
-    const int t = min(tid + NT * it, NI - 1);
-    const int e = t / 7, col = ilast - 6 + t % 7;
+    int e, col;
+    r1_stencil_point(min(tid + NT * it, NI - 1), ilast, e, col);
     int row, pl, dir;
     vrow_source(g, r0 - NG + e, i2, row, pl, dir);
     const unsigned k2 = (unsigned)row * s1s + col, p = (unsigned)pl * s2 + k2;
@@ -450,7 +465,9 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   for (int it = 0; it < NIR; it++) {
     const int t = tid + NT * it;
     if (t < NI) {
-      const int e = t / 7, col = ilast - 6 + t % 7, vr = r0 - NG + e;
+      int e, col;
+      r1_stencil_point(t, ilast, e, col);
+      const int vr = r0 - NG + e;
       int row, pl, dir;
       vrow_source(g, vr, i2, row, pl, dir);
       double xm[6], xp[6], ym[6], yp[6];

@@ -1,8 +1,8 @@
 """The trained initial-guess network (nrpyelliptic_amd/data/mlp_initial_guess.npz: 6 x 256
-SiLU coordinate MLP on the BF16 MFMA kernels, trained by nrpyelliptic_amd.learned on 2753
-converged FP64 solutions of configurations drawn with seed 1000) against the plain method on
-held-out configurations (seed 900), and the fallback rule (zero guess whenever the network's
-guess has the larger initial residual)."""
+SiLU coordinate MLP on the BF16 MFMA kernels, trained by nrpyelliptic_amd.learned on the FP64
+solutions of the 4096 configurations drawn with seed 1000) against the plain method on held-out
+configurations (seed 900), and the fallback rule (zero guess whenever the network's guess has the
+larger initial residual)."""
 import os
 
 import numpy as np
@@ -77,3 +77,19 @@ def test_evaluate_with_tuned_damping():
     print("tuned evaluate:", r, flush=True)
     assert r["converged"] and r["nn_used"] and r["nn_steps"] < 0.5 * r["zero_steps"]
     assert r["max_du"] <= 1e-7
+
+
+def test_network_start_on_32_held_out_configurations_batched():
+    """32 held-out configurations (seed 900, 128x128x16) relaxed together from the zero guess and
+    from the network's guess (configured eta, learned.evaluate_batched): the same configurations
+    converge, the network's guess is kept on nearly all of them, it needs fewer steps in total, and
+    the two converged states agree to the level the residual target bounds them."""
+    net = InitialGuessMLP(seed=7)
+    net.load(WEIGHTS)
+    out = learned.evaluate_batched(net, config.random_batch(32, seed=900, grids=False, Nxx=(128, 128, 16)),
+                                   1.0, 1.0, log=lambda s: None)
+    print("batched held-out:", out, flush=True)
+    assert out["converged_zero"] == out["converged_nn"] == 32
+    assert out["nn_guess_used"] >= 28
+    assert out["speedup_steps"] > 1.05
+    assert out["max_du_nn_vs_zero"] <= 1e-7
