@@ -344,6 +344,11 @@ int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, double *us_
 /* Exactly n RK4 steps of the production FP64 path (graph replay, residual reduced every
  * step, no early stop), timed with HIP events on the solver's stream. */
 int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms, double *log10_l2);
+/* A new damping strength mid-relaxation: params.eta_damping = eta and v += (eta - old eta) u at
+ * every point, so that u and u_t = v - eta u carry over unchanged (the steady state, v = eta u,
+ * moves with eta). Solvers built on griddata pick the new eta up at their next solve. Synchronous;
+ * NRPYELL_OK or NRPYELL_EINVAL. */
+int nrpyell_set_eta_damping(griddata_struct *gd, double eta);
 /* RK4 substeps per launch of the solver's FP64 steps (1 or 2, see substeps_per_launch). */
 int nrpyell_solver_substeps_per_launch(nrpyell_solver *s);
 /* Residual history of the last solve: (steps completed, log10 L2) pairs; returns the count. */

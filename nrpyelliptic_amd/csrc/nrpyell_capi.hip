@@ -1242,6 +1242,21 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
   });
 }
 
+extern "C" int nrpyell_set_eta_damping(griddata_struct *gd, double eta) {
+  return api_call([&]() -> int {
+    if (!gd || !gd->gridfuncs.y_n_gfs || !(eta > 0.0)) {
+      set_last_error("nrpyell_set_eta_damping: null griddata or eta <= 0");
+      return NRPYELL_EINVAL;
+    }
+    const long n = dims_from_params(&gd->params).ntot;
+    launch_shift_v(n, eta - gd->params.eta_damping, gd->gridfuncs.y_n_gfs, default_stream());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    gd->params.eta_damping = eta;
+    return NRPYELL_OK;
+  });
+}
+
 extern "C" int nrpyell_solver_substeps_per_launch(nrpyell_solver *s) { return s ? (s->pair_path ? 2 : 1) : 0; }
 
 extern "C" int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2) {

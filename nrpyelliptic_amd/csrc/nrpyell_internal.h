@@ -208,6 +208,7 @@ void launch_wavespeed(const GridDims &g, const double *ds2d, double CFL, double 
 void launch_convert_d2f(long n, const double *in, float *out, hipStream_t st);
 void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st);
 void launch_fill(long n, double v, double *out, hipStream_t st);
+void launch_shift_v(long n, double d, double *y, hipStream_t st);
 
 // host setup (nrpyell_host.cpp)
 GridDims dims_from_params(const paramstruct *p);

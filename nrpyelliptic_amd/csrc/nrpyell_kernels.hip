@@ -680,3 +680,13 @@ __global__ void fill_kernel(long n, double v, double *out) {
 void launch_fill(long n, double v, double *out, hipStream_t st) {
   hipLaunchKernelGGL(fill_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, v, out);
 }
+
+// v += d u at every point (y = u | v, n points per gridfunction): a new damping eta + d with u and
+// u_t = v - eta u unchanged
+__global__ void shift_v_kernel(long n, double d, double *y) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) y[n + k] = fma(d, y[k], y[n + k]);
+}
+void launch_shift_v(long n, double d, double *y, hipStream_t st) {
+  hipLaunchKernelGGL(shift_v_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n, d, y);
+}

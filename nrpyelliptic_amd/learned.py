@@ -130,6 +130,33 @@ def seed_with_tuned_damping(net: InitialGuessMLP, P: runtime.Problem, f_nn: floa
     return used, eta0
 
 
+def solve_damping_schedule(net: InitialGuessMLP, P: runtime.Problem, f_hi: float, switch_log10: float = -9.0,
+                           target: float = -12.0, precision: int = 32, smooth_passes: int = 2,
+                           max_steps: int = 40000):
+    """The network's start relaxed in two phases: at f_hi x the configured eta (the damping tuned
+    to that start: fast on the grid-scale error the guess leaves) down to switch_log10, then at the
+    configured eta (which relaxes the long-wavelength error a strong damping leaves behind, the
+    error the masked residual hardly sees) down to target, with u and u_t = v - eta u carried
+    across the switch (Problem.set_eta_damping). Returns (steps, fp32 steps, solver seconds, seed
+    seconds, converged, network guess used); P's eta is the configured one afterwards."""
+    eta0 = P.gd.params.eta_damping
+    t0 = time.perf_counter()
+    used, _ = seed_with_tuned_damping(net, P, f_hi, 1.0, smooth_passes)
+    seed_s = time.perf_counter() - t0
+    try:
+        S = P.solver(precision=precision, log10_target=switch_log10, max_steps=max_steps)
+        r1 = S.solve()
+        S.close()
+        P.set_eta_damping(eta0)
+        S = P.solver(log10_target=target, max_steps=max_steps)
+        r2 = S.solve()
+        S.close()
+    finally:
+        P.gd.params.eta_damping = eta0
+    return (r1.steps + r2.steps, r1.fp32_steps, r1.seconds + r2.seconds, seed_s,
+            bool(r1.converged and r2.converged), bool(used))
+
+
 def _solve(S):
     try:
         return S.solve()

@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_timed_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.nrpyell_solver_substeps_per_launch.argtypes = [vp]
+    L.nrpyell_set_eta_damping.argtypes = [C.POINTER(abi.griddata_struct), C.c_double]
     L.nrpyell_batch_create.argtypes = [C.POINTER(C.POINTER(abi.griddata_struct)), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options),
                                        C.POINTER(vp)]
@@ -250,6 +251,10 @@ class Problem:
         c = L.compute_wavespeed_at_OB(C.byref(p), self.gd.gridfuncs.auxevol_gfs)
         p.uu_wavespeed_at_OB = p.vv_wavespeed_at_OB = c
         self.cfl = float(cfl)
+
+    def set_eta_damping(self, eta: float) -> None:
+        """Change the damping mid-relaxation keeping u and u_t = v - eta u (nrpyell_set_eta_damping)."""
+        check(lib().nrpyell_set_eta_damping(C.byref(self.gd), float(eta)), "set_eta_damping")
 
     def zero_guess(self) -> None:
         """u = v = 0 at every point (initial_guess_all_points.c:7), the reference's start."""
