@@ -308,6 +308,40 @@ def nn_seeded_paths(P, n_held_out=256):
             "calibration_seconds": tune_s, "bench_problem": bench, "held_out_batched": held}
 
 
+def mixed_precision_q4():
+    """The FP32 -> FP64 solver on the unequal-mass workload (q = 4, D = 10, seed-7 spins,
+    128x128x16) beside the FP64 solver: steps, FP32 steps and seconds to log10 L2 <= -10 from the
+    zero guess, and each answer's max |du| and rms du against the reference's FP64 u at its own
+    stopping step and against its u* (tests/golden/deep_q4_D10_seed7_128x128x16.npz, written
+    from the reference's C)."""
+    import numpy as np
+    from nrpyelliptic_amd import config, runtime
+    d = np.load(os.path.join(ROOT, "tests", "golden", "deep_q4_D10_seed7_128x128x16.npz"))
+    u_star = d["u_deep"]
+    u_ref = u_star + d["du_stop"].astype(np.float64)
+    P = runtime.Problem(config.unequal_mass_q4())
+    G0, G1, G2 = P.Ng
+    out = {"workload": "q=4, D=10, seed-7 spins, 128x128x16", "reference_fp64_steps": int(d["steps_stop"])}
+    try:
+        for name, prec in (("fp64", 64), ("fp32_then_fp64", 32)):
+            P.zero_guess()
+            S = P.solver(precision=prec)
+            r = S.solve()
+            S.close()
+            u = P.get("y_n_gfs")[: P.ntot].reshape(G2, G1, G0)[5:-5, 5:-5, 5:-5]
+            du, ds = u - u_ref, u - u_star
+            out[name] = {"steps": r.steps, "fp32_steps": r.fp32_steps, "seconds": r.seconds,
+                         "log10_l2": r.log10_l2, "converged": bool(r.converged),
+                         "max_abs_du_vs_reference_fp64": float(np.max(np.abs(du))),
+                         "rms_du_vs_reference_fp64": float(np.sqrt(np.mean(du * du))),
+                         "max_abs_u_minus_ustar": float(np.max(np.abs(ds)))}
+        out["reference_max_abs_u_minus_ustar"] = float(np.max(np.abs(u_ref - u_star)))
+        out["speedup_mixed"] = out["fp64"]["seconds"] / out["fp32_then_fp64"]["seconds"]
+    finally:
+        P.close()
+    return out
+
+
 def batched_solve(n=64, seed=42):
     """The batched configuration: n random punctures (seed 42: q log-uniform in [1,8], D in
     [6,20], Newtonian tangential momenta, random spins; grids mixing Nxx0 in {96,128,160},
@@ -455,6 +489,7 @@ def main():
     dropin_us = dropin_call_us(P)
     admbase = admbase_throughput(P)
     batched = batched_solve()
+    mixed_q4 = mixed_precision_q4()
 
     cpu = None if args.no_cpu_baseline else run_cpu_baseline_subprocess()
     if cpu is not None and isinstance(cpu.get("admbase"), dict):
@@ -486,6 +521,7 @@ def main():
         "dropin_kernel_us": load_dropin_kernels(),
         "et_consumer_admbase": admbase,
         "batched_64_seed42": batched,
+        "mixed_precision_q4_seed7": mixed_q4,
         "cpu_baseline": cpu,
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
                              "log10_l2": r.log10_l2, "converged": bool(r.converged),

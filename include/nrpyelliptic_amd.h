@@ -378,10 +378,13 @@ int nrpyell_mlp_get_params(nrpyell_mlp *m, float *host);
 int nrpyell_mlp_set_params(nrpyell_mlp *m, const float *host);
 /* x: device BF16 [M x 64]; y: device FP32 [M] */
 int nrpyell_mlp_forward(nrpyell_mlp *m, const void *x, int M, float *y);
-/* mean squared error against device targets t [M], gradients (host, num_params) */
+/* mean squared error against device targets t [M], gradients (host, num_params); the weight
+ * gradients are reduced over fixed row ranges in a fixed order: repeated calls agree bit for bit */
 int nrpyell_mlp_loss_grad(nrpyell_mlp *m, const void *x, const float *t, int M, double *loss, float *grads_host);
+/* one Adam step with the gradient of the last nrpyell_mlp_loss_grad call */
 int nrpyell_mlp_adam_step(nrpyell_mlp *m, float lr);
-/* nsteps Adam steps on uniform minibatches from a device dataset (x [N x 64], t [N]) */
+/* nsteps Adam steps on uniform minibatches from a device dataset (x [N x 64], t [N]); three
+ * launches per step (forward + loss + data-gradient chain, weight gradients, Adam); deterministic */
 int nrpyell_mlp_train(nrpyell_mlp *m, const void *x, const float *t, long N, int batch, int nsteps, float lr,
                       unsigned long long seed, double *losses_host);
 /* rows (interior + pure outer points, rounded up to 64) of a grid's feature matrix */

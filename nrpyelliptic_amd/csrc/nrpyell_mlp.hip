@@ -2,7 +2,7 @@
 // solutions this solver produced, run on hand-written BF16 MFMA kernels (CDNA4
 // v_mfma_f32_32x32x16_bf16) with FP32 accumulation. No reference counterpart.
 //
-// Network: K0 = 32 (padded) input features -> 5 x [Linear(256) + SiLU] -> Linear(1).
+// Network: 23 input features (zero-padded to K0 = 64) -> 5 x [Linear(256) + SiLU] -> Linear(1).
 // Activations between layers are stored in BF16 (MFMA operands), pre-activations in FP32;
 // master weights, gradients and Adam moments are FP32, weights are re-rounded to BF16 after
 // every update.
@@ -35,72 +35,381 @@ struct MlpOffsets {
 __device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
 __device__ __forceinline__ bf16_t to_bf16(float x) { return (bf16_t)x; }
 
-// C(m,c) = sum_i A(m,i) B(c,i) with A(m,i) = A[m*a_m + i*a_i], B(c,i) = B[c*b_c + i*b_i].
-// Block tile 64 x 64 (4 waves, each 32 x 32 = one MFMA accumulator), inner dimension in
-// chunks of 32 staged through LDS. Requirements (checked by the host): M, N multiples of 64,
-// the inner range [i_begin, i_end) a multiple of 32.
-//   EPI_FWD : z = acc + bias[c]; Z[m*N+c] = z; H[m*N+c] = bf16(silu(z))
-//   EPI_BWD : out[m*N+c] = bf16(acc * silu'(Zprev[m*N+c]))   (gradient w.r.t. the previous pre-activation)
-//   EPI_WGT : atomicAdd(G[m*N+c], acc)                       (weight gradient, inner dim split over blockIdx.z)
-enum { EPI_FWD = 0, EPI_BWD = 1, EPI_WGT = 2 };
+// splitmix64 of (seed, m): the dataset row of minibatch row m (uniform, with replacement)
+__device__ __forceinline__ long draw_row(unsigned long long seed, int m, long N) {
+  unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (unsigned long long)(m + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (long)(z % (unsigned long long)N);
+}
 
-template <int EPI>
-__global__ void __launch_bounds__(256) mfma_gemm_kernel(const bf16_t *__restrict__ A, long a_m, long a_i,
-                                                        const bf16_t *__restrict__ B, long b_c, long b_i, int M,
-                                                        int N, int inner, int chunk, const float *__restrict__ bias,
-                                                        float *__restrict__ Z, bf16_t *__restrict__ H,
-                                                        const float *__restrict__ Zprev, float *__restrict__ G) {
-  __shared__ bf16_t As[64][32 + 8];
-  __shared__ bf16_t Bs[64][32 + 8];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int m0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
-  const int i_begin = blockIdx.z * chunk;
-  int i_end = i_begin + chunk;
-  if (i_end > inner) i_end = inner;
-  floatx16 acc;
+// ---------------------------------------------------------------------------------------
+// Training: one minibatch step is three launches.
+//
+// 1. mlp_train_rows_kernel: a block takes TR_R rows through the forward pass (activations in
+//    LDS, ping-pong), the output layer and the loss, then back through the data-gradient chain
+//    dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}). Every product is row-local, so nothing crosses
+//    blocks. silu'(z) of each layer stays in registers: the lane that produces z in the forward
+//    epilogue holds the same (row, column) entries of dZ in the backward one. H_l (l = 0..5)
+//    and dZ_l (l = 1..5) leave the block transposed ([width x M], rows contiguous), which is the
+//    layout the weight gradients consume as MFMA operands.
+// 2. mlp_wgrad_kernel: dW_l = dZ_l^T H_{l-1}, db_l = column sums of dZ_l, and the output layer's
+//    gradients, over TR_SPLITS row ranges; each range writes its own partial (no atomics).
+// 3. mlp_adam_kernel: g = the partials summed in range order (deterministic), Adam, and the BF16
+//    weight copies (W_l and, for l >= 2, W_l transposed, the data-gradient chain's operand).
+//
+// Arithmetic per entry matches the network definition: FP32 accumulation of BF16 products in
+// ascending inner index, z = acc + b, h = bf16(silu(z)), y = sum_k h_k w_k + b (FP32),
+// dy = 2 (y - t) / M, dZ_5 = bf16(dy w_out silu'(z_5)), dZ_{l-1} = bf16(acc silu'(z_{l-1})).
+// ---------------------------------------------------------------------------------------
+constexpr int TR_R = 32;        // rows per block
+constexpr int TR_WAVES = 8;     // wave w owns columns 32w .. 32w+31 of every 256-wide layer
+constexpr int TR_LD = MLP_W + 8;
+constexpr int TR_SPLITS = 32;   // row ranges of the weight-gradient reduction
+constexpr int WG_GROUPS = MLP_HIDDEN * TR_SPLITS;  // (layer, row range) groups of the weight gradients
+constexpr int TR_LDS_DSL = 3;         // layers whose silu'(z) the row kernel keeps in LDS
+constexpr int WG_G = 2;               // inner steps per load group of the weight-gradient kernel
+
+// Operand layout of a matrix X[rows][inner] (rows a multiple of 32, inner a multiple of 16):
+// lane (li, lj) of a wave feeds the 32x32x16 MFMA for rows 32 t .. 32 t + 31 and inner step q
+// with X[32 t + li][16 q + 8 lj .. 16 q + 8 lj + 7]; the layout keeps those 8 elements at
+// ((t * (inner / 16) + q) * 64 + 32 lj + li) * 8, so a wave's operand load is 1 KB of
+// consecutive bytes (a row-major load would touch 32 rows, 32 bytes each).
+__device__ __forceinline__ long swz(int row, int col, int inner16) {
+  return ((long)((row >> 5) * inner16 + (col >> 4)) * 64 + ((col >> 3) & 1) * 32 + (row & 31)) * 8 + (col & 7);
+}
+
+// Where the BF16 weight copies live: pw[k] = bf16(p[k]) (row-major, for inference) and, in
+// pws, W_l (rows n, inner k; l = 1..5) and W_l^T (rows k, inner n; l = 2..5) in the operand layout.
+struct WeightCopies {
+  long w_off[MLP_HIDDEN + 1];  // W_l in the FP32 parameters
+  long fs[MLP_HIDDEN + 1];     // W_l in pws
+  long ts[MLP_HIDDEN + 1];     // W_l^T in pws
+};
+
+struct TrainArgs {
+  const bf16_t *X;  // dataset features [N x 64]
+  const float *t;   // dataset targets [N]
+  long N;
+  unsigned long long seed;
+  int gather;  // 1: minibatch row m is dataset row draw_row(seed, m, N); 0: row m
+  int M;       // minibatch rows (multiple of 64)
+  const bf16_t *pws;  // weights in the operand layout (WeightCopies)
+  const float *p;
+  MlpOffsets off;
+  WeightCopies wc;
+  bf16_t *Ht[MLP_HIDDEN + 1];      // H_l^T [width_l x M], l = 0..5
+  bf16_t *dZt[MLP_HIDDEN + 1];     // dZ_l^T [256 x M], l = 1..5
+  float *dy;                       // [M]
+  double *loss;                    // += sum over the block's rows of (y - t)^2 / M
+};
+
+// rows r0 .. r0+31 of an LDS tile (width WD) -> the transposed global [WD x M] in the operand
+// layout (rows = the WD columns, inner = minibatch rows)
+template <int WD>
+__device__ __forceinline__ void store_transposed(const bf16_t *buf, bf16_t *__restrict__ out, int r0, int M, int tid) {
+  constexpr int PER = TR_R * WD / (64 * TR_WAVES);  // rows per thread: 16 (WD 256) or 4 (WD 64)
+  const int c = tid % WD, q = tid / WD;
+  const int m = r0 + PER * q, M16 = M / 16;
+  bf16_t v[PER];
 #pragma unroll
-  for (int r = 0; r < 16; r++) acc[r] = 0.0f;
-  const int wm = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int li = lane & 31, lj = lane >> 5;
-  for (int ib = i_begin; ib < i_end; ib += 32) {
-    // stage 64 x 32 of A and of B: 2048 elements each, 8 per thread
+  for (int i = 0; i < PER; i++) v[i] = buf[(PER * q + i) * TR_LD + c];
+  if (PER == 16) {
+    bf16x8 lo, hi;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int e = tid + 256 * q;
-      const int row = e >> 5, col = e & 31;
-      As[row][col] = A[(long)(m0 + row) * a_m + (long)(ib + col) * a_i];
-      Bs[row][col] = B[(long)(c0 + row) * b_c + (long)(ib + col) * b_i];
+    for (int i = 0; i < 8; i++) {
+      lo[i] = v[i];
+      hi[i] = v[8 + i];
     }
-    __syncthreads();
+    *(bf16x8 *)(out + swz(c, m, M16)) = lo;
+    *(bf16x8 *)(out + swz(c, m + 8, M16)) = hi;
+  } else {
+    bf16_t *o = out + swz(c, m, M16);
 #pragma unroll
-    for (int kk = 0; kk < 32; kk += 16) {
-      bf16x8 a, b;
-#pragma unroll
-      for (int t = 0; t < 8; t++) {
-        a[t] = As[wm + li][kk + 8 * lj + t];
-        b[t] = Bs[wc + li][kk + 8 * lj + t];
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-    __syncthreads();
+    for (int i = 0; i < PER; i++) o[i] = v[i];
   }
-  // accumulator r of lane (li, lj): row 8*(r/4) + 4*lj + r%4, column li
+}
+
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = sigmoidf_(z);
+  return s * (1.0f + z * (1.0f - s));
+}
+
+__global__ void __launch_bounds__(512) mlp_train_rows_kernel(TrainArgs a) {
+  __shared__ __attribute__((aligned(16))) bf16_t Hb[2][TR_R * TR_LD];
+  __shared__ float s_part[TR_WAVES][TR_R];
+  __shared__ float s_t[TR_R], s_dy[TR_R];
+  __shared__ long s_row[TR_R];
+  // silu'(z) of layers 1..TR_LDS_DSL, each thread's 16 entries at [layer][r][thread] (read back by
+  // the same thread); layers above stay in registers
+  __shared__ float s_dsl[TR_LDS_DSL][16][64 * TR_WAVES];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 31, lj = lane >> 5;
+  const int r0 = blockIdx.x * TR_R;
+  // this lane's column of every 256-wide layer; the eight 32-column slices are rotated by the
+  // block index so that concurrent blocks read different weight rows (no change to any sum)
+  const int ct = (wave + blockIdx.x) & (TR_WAVES - 1);
+  const int c = ct * 32 + li;
+  if (tid < TR_R) {
+    const long row = a.gather ? draw_row(a.seed, r0 + tid, a.N) : (long)(r0 + tid);
+    s_row[tid] = row;
+    s_t[tid] = a.t[row];
+  }
+  __syncthreads();
+  if (tid < TR_R * 8) {  // 32 rows x 64 BF16 features = 256 chunks of 16 B
+    const int row = tid >> 3, ch = tid & 7;
+    *(bf16x8 *)(&Hb[0][row * TR_LD + 8 * ch]) = *(const bf16x8 *)(a.X + s_row[row] * MLP_K0 + 8 * ch);
+  }
+  __syncthreads();
+  store_transposed<MLP_K0>(Hb[0], a.Ht[0], r0, a.M, tid);
+  // accumulator r of lane (li, lj): row 8*(r/4) + 4*lj + r%4 of the block, column c
+  float dsl[MLP_HIDDEN][16];
+  // the weight operands of the next product are loaded (all of them, 16 B per lane per
+  // 16-wide inner step) as soon as the current product's MFMAs are issued, so their latency
+  // overlaps the epilogue, the barrier and the transposed store
+  bf16x8 bv[MLP_W / 16];
 #pragma unroll
-  for (int r = 0; r < 16; r++) {
-    const int m = m0 + wm + 8 * (r >> 2) + 4 * lj + (r & 3);
-    const int c = c0 + wc + li;
-    const long o = (long)m * N + c;
-    if (EPI == EPI_FWD) {
-      const float z = acc[r] + bias[c];
-      Z[o] = z;
-      H[o] = to_bf16(z * sigmoidf_(z));
-    } else if (EPI == EPI_BWD) {
-      const float zp = Zprev[o];
-      const float s = sigmoidf_(zp);
-      H[o] = to_bf16(acc[r] * (s * (1.0f + zp * (1.0f - s))));
-    } else {
-      atomicAdd(&G[o], acc[r]);
+  for (int q = 0; q < MLP_K0 / 16; q++)
+    bv[q] = *(const bf16x8 *)(a.pws + a.wc.fs[1] + ((long)ct * (MLP_K0 / 16) + q) * 512 + 8 * lane);
+  int cur = 0;
+#pragma unroll
+  for (int l = 1; l <= MLP_HIDDEN; l++) {
+    const int Kl = l == 1 ? MLP_K0 : MLP_W;
+    const bf16_t *Hin = Hb[cur] + li * TR_LD + 8 * lj;
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < Kl / 16; q++)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*(const bf16x8 *)(Hin + 16 * q), bv[q], acc, 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    // next: W_{l+1}, or W_5^T for the first step of the data-gradient chain
+    const bf16_t *Wn = a.pws + (l < MLP_HIDDEN ? a.wc.fs[l + 1] : a.wc.ts[MLP_HIDDEN]) + (long)ct * 16 * 512 + 8 * lane;
+#pragma unroll
+    for (int q = 0; q < MLP_W / 16; q++) bv[q] = *(const bf16x8 *)(Wn + 512 * q);
+    bf16_t *Hout = Hb[cur ^ 1];
+    const float bc = a.p[a.off.b[l] + c];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int row = 8 * (r >> 2) + 4 * lj + (r & 3);
+      const float z = acc[r] + bc;
+      if (l <= TR_LDS_DSL)
+        s_dsl[l - 1][r][tid] = dsilu(z);
+      else
+        dsl[l - 1][r] = dsilu(z);
+      Hout[row * TR_LD + c] = to_bf16(z * sigmoidf_(z));
     }
+    __syncthreads();
+    cur ^= 1;
+    store_transposed<MLP_W>(Hb[cur], a.Ht[l], r0, a.M, tid);
+  }
+  // output layer: wave w sums columns 32w .. 32w+31 of every row, then the 8 partials in a fixed order
+  const float *wout = a.p + a.off.w[MLP_HIDDEN + 1];
+  if (lane < TR_R) {
+    const bf16_t *h = Hb[cur] + lane * TR_LD + wave * 32;
+    float acc = 0.0f;
+#pragma unroll 8
+    for (int k = 0; k < 32; k++) acc += (float)h[k] * wout[wave * 32 + k];
+    s_part[wave][lane] = acc;
+  }
+  __syncthreads();
+  if (tid < 64) {
+    double l2 = 0.0;
+    if (tid < TR_R) {
+      const float y = (((s_part[0][tid] + s_part[1][tid]) + (s_part[2][tid] + s_part[3][tid])) +
+                       ((s_part[4][tid] + s_part[5][tid]) + (s_part[6][tid] + s_part[7][tid]))) +
+                      a.p[a.off.b[MLP_HIDDEN + 1]];
+      const float diff = y - s_t[tid];
+      const float dy = 2.0f * diff / (float)a.M;
+      s_dy[tid] = dy;
+      a.dy[r0 + tid] = dy;
+      l2 = (double)diff * (double)diff / (double)a.M;
+    }
+    for (int o = 32; o > 0; o >>= 1) l2 += __shfl_down(l2, o, 64);
+    if (tid == 0) atomicAdd(a.loss, l2);
+  }
+  __syncthreads();
+  // dZ_5 into the other buffer (H_4 there is no longer read)
+  {
+    bf16_t *D = Hb[cur ^ 1];
+    const float wc_ = wout[c];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int row = 8 * (r >> 2) + 4 * lj + (r & 3);
+      D[row * TR_LD + c] = to_bf16(s_dy[row] * wc_ * dsl[MLP_HIDDEN - 1][r]);
+    }
+  }
+  __syncthreads();
+  cur ^= 1;
+  store_transposed<MLP_W>(Hb[cur], a.dZt[MLP_HIDDEN], r0, a.M, tid);
+  // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}), l = 5..2: A = dZ_l (LDS), B(k, n) = W_l^T[k][n]
+#pragma unroll
+  for (int l = MLP_HIDDEN; l >= 2; l--) {
+    const bf16_t *Din = Hb[cur] + li * TR_LD + 8 * lj;
+    floatx16 acc;
+#pragma unroll
+    for (int r = 0; r < 16; r++) acc[r] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < MLP_W / 16; q++)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*(const bf16x8 *)(Din + 16 * q), bv[q], acc, 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    if (l > 2) {
+      const bf16_t *WT = a.pws + a.wc.ts[l - 1] + (long)ct * 16 * 512 + 8 * lane;
+#pragma unroll
+      for (int q = 0; q < MLP_W / 16; q++) bv[q] = *(const bf16x8 *)(WT + 512 * q);
+    }
+    bf16_t *Dout = Hb[cur ^ 1];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      const int row = 8 * (r >> 2) + 4 * lj + (r & 3);
+      Dout[row * TR_LD + c] = to_bf16(acc[r] * (l - 1 <= TR_LDS_DSL ? s_dsl[l - 2][r][tid] : dsl[l - 2][r]));
+    }
+    __syncthreads();
+    cur ^= 1;
+    store_transposed<MLP_W>(Hb[cur], a.dZt[l - 1], r0, a.M, tid);
+  }
+}
+
+struct WgradArgs {
+  const bf16_t *Ht[MLP_HIDDEN + 1];
+  const bf16_t *dZt[MLP_HIDDEN + 1];
+  const float *dy;
+  int M, chunk;  // rows per range (multiple of 16)
+  MlpOffsets off;
+  long n_params;
+  float *part;   // [TR_SPLITS x n_params]
+};
+
+// One wave's (32 NT) x (32 KT) block of C[n][k] = sum_{m in [m0, m1)} A[n][m] B[k][m] for the
+// rows n = 32 ta .., k = 32 tb .. of two transposed minibatch arrays in the operand layout (inner
+// = the M minibatch rows), loaded straight from global as MFMA operands (1 KB per wave and load)
+// in groups of WG_G 16-row inner steps, double-buffered (the loads of group g+1 are in flight
+// while group g's MFMAs run). C goes to G (row stride ldg); with gb != null the row sums of A
+// (bias gradient) go to gb.
+template <int NT, int KT>
+__device__ __forceinline__ void wgrad_block(const bf16_t *__restrict__ A, int ta, const bf16_t *__restrict__ B, int tb,
+                                            int M, int m0, int m1, float *__restrict__ G, int ldg,
+                                            float *__restrict__ gb) {
+  const int lane = threadIdx.x & 63, li = lane & 31, lj = lane >> 5;
+  const long M16 = M / 16;
+  const bf16_t *Ap[NT], *Bp[KT];
+#pragma unroll
+  for (int x = 0; x < NT; x++) Ap[x] = A + ((ta + x) * M16 * 64 + lane) * 8;
+#pragma unroll
+  for (int y = 0; y < KT; y++) Bp[y] = B + ((tb + y) * M16 * 64 + lane) * 8;
+  floatx16 acc[NT][KT];
+#pragma unroll
+  for (int x = 0; x < NT; x++)
+#pragma unroll
+    for (int y = 0; y < KT; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[x][y][r] = 0.0f;
+  float bsum[NT];
+#pragma unroll
+  for (int x = 0; x < NT; x++) bsum[x] = 0.0f;
+  const bool bias = gb != nullptr;
+  const int nsteps = m1 > m0 ? (m1 - m0) / 16 : 0;
+  const int ngroups = (nsteps + WG_G - 1) / WG_G;
+  bf16x8 a0[WG_G][NT], b0[WG_G][KT], a1[WG_G][NT], b1[WG_G][KT];
+  auto load = [&](bf16x8 (*av)[NT], bf16x8 (*bw)[KT], int g) {
+#pragma unroll
+    for (int q = 0; q < WG_G; q++)
+      if (g * WG_G + q < nsteps) {
+        const long o = (long)(m0 / 16 + g * WG_G + q) * 512;  // inner step m / 16
+#pragma unroll
+        for (int x = 0; x < NT; x++) av[q][x] = *(const bf16x8 *)(Ap[x] + o);
+#pragma unroll
+        for (int y = 0; y < KT; y++) bw[q][y] = *(const bf16x8 *)(Bp[y] + o);
+      }
+  };
+  auto mma = [&](bf16x8 (*av)[NT], bf16x8 (*bw)[KT], int g) {
+#pragma unroll
+    for (int q = 0; q < WG_G; q++)
+      if (g * WG_G + q < nsteps) {
+#pragma unroll
+        for (int x = 0; x < NT; x++)
+#pragma unroll
+          for (int y = 0; y < KT; y++)
+            acc[x][y] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[q][x], bw[q][y], acc[x][y], 0, 0, 0);
+        if (bias) {
+#pragma unroll
+          for (int x = 0; x < NT; x++)
+#pragma unroll
+            for (int i = 0; i < 8; i++) bsum[x] += (float)av[q][x][i];
+        }
+      }
+  };
+  if (ngroups > 0) load(a0, b0, 0);
+  for (int g = 0; g < ngroups; g += 2) {
+    if (g + 1 < ngroups) load(a1, b1, g + 1);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(a0, b0, g);
+    __builtin_amdgcn_sched_barrier(0);
+    if (g + 2 < ngroups) load(a0, b0, g + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    if (g + 1 < ngroups) mma(a1, b1, g + 1);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  // accumulator r of lane (li, lj) is row 8*(r/4) + 4*lj + r%4, column li of its 32 x 32 tile
+#pragma unroll
+  for (int x = 0; x < NT; x++)
+#pragma unroll
+    for (int y = 0; y < KT; y++)
+#pragma unroll
+      for (int r = 0; r < 16; r++)
+        G[(long)(32 * x + 8 * (r >> 2) + 4 * lj + (r & 3)) * ldg + 32 * y + li] = acc[x][y][r];
+  if (bias) {
+#pragma unroll
+    for (int x = 0; x < NT; x++) {
+      const float t = bsum[x] + __shfl_xor(bsum[x], 32, 64);
+      if (lj == 0) gb[32 * x + li] = t;
+    }
+  }
+}
+
+// Weight gradients over row ranges. Blocks 0 .. 4*WG_GROUPS-1: group (layer l, range s) has four
+// blocks nt = 0..3, each the 64 rows 64 nt .. 64 nt + 63 of dW_l (all K_l columns): for l >= 2
+// wave w takes columns 64 w .. 64 w + 63 (2 x 2 MFMA tiles), for l = 1 wave w the 32 x 32
+// quarter (w / 2, w % 2) of the 64 x 64 block. Block b = 32 (g / 8) + 8 nt + g % 8 for group g,
+// so the four blocks of a group share b mod 8, i.e. one XCD (workgroups are dealt to the XCDs
+// round-robin) and one L2, where the group's dZ_l and H_{l-1} row slices are reused. The last
+// TR_SPLITS blocks: the output layer's gradients, one range each.
+__global__ void __launch_bounds__(256) mlp_wgrad_kernel(WgradArgs a) {
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int b = blockIdx.x;
+  if (b >= 4 * WG_GROUPS) {
+    const int s = b - 4 * WG_GROUPS;
+    const int m0 = s * a.chunk, m1 = min(a.M, m0 + a.chunk);
+    float *P = a.part + (long)s * a.n_params;
+    // gw[k] = sum_m dy[m] H_5[m][k], gb = sum_m dy[m]; thread k reads row k of H_5^T (operand layout)
+    const bf16_t *h = a.Ht[MLP_HIDDEN] + (((long)(tid >> 5) * (a.M / 16)) * 64 + (tid & 31)) * 8;
+    float acc = 0.0f, accb = 0.0f;
+    for (int m = m0; m < m1; m += 8) {
+      const bf16x8 hv = *(const bf16x8 *)(h + ((long)(m >> 4) * 64 + ((m >> 3) & 1) * 32) * 8);
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        acc += a.dy[m + i] * (float)hv[i];
+        accb += a.dy[m + i];
+      }
+    }
+    P[a.off.w[MLP_HIDDEN + 1] + tid] = acc;
+    if (tid == 0) P[a.off.b[MLP_HIDDEN + 1]] = accb;
+    return;
+  }
+  const int q = b >> 5, rem = b & 31, nt = rem >> 3, g = q * 8 + (rem & 7);
+  const int l = 1 + g / TR_SPLITS, s = g % TR_SPLITS;
+  const int m0 = s * a.chunk, m1 = min(a.M, m0 + a.chunk);
+  float *P = a.part + (long)s * a.n_params;
+  if (l == 1) {
+    const int n0 = nt * 64 + (wave >> 1) * 32, k0 = (wave & 1) * 32;
+    wgrad_block<1, 1>(a.dZt[1], n0 / 32, a.Ht[0], k0 / 32, a.M, m0, m1,
+                      P + a.off.w[1] + (long)n0 * MLP_K0 + k0, MLP_K0,
+                      (wave & 1) == 0 ? P + a.off.b[1] + n0 : nullptr);
+  } else {
+    const int n0 = nt * 64, k0 = wave * 64;
+    wgrad_block<2, 2>(a.dZt[l], n0 / 32, a.Ht[l - 1], k0 / 32, a.M, m0, m1,
+                      P + a.off.w[l] + (long)n0 * MLP_W + k0, MLP_W, wave == 0 ? P + a.off.b[l] + n0 : nullptr);
   }
 }
 
@@ -188,85 +497,49 @@ __global__ void __launch_bounds__(256) mlp_forward_fused_kernel(const bf16_t *__
     y[r0 + tid] = ((s_out[0][tid] + s_out[1][tid]) + (s_out[2][tid] + s_out[3][tid])) + p[off.b[MLP_HIDDEN + 1]];
 }
 
-// Output layer (256 -> 1): y[m] = sum_k H[m][k] w[k] + b (FP32 dot over BF16 activations).
-__global__ void out_layer_kernel(const bf16_t *__restrict__ H, const float *__restrict__ w, const float *__restrict__ b,
-                                 int M, float *__restrict__ y) {
-  const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (m >= M) return;
-  float acc = 0.0f;
+// the BF16 copies of parameter k (see WeightCopies)
+__device__ __forceinline__ void store_weight(const WeightCopies &wc, long k, float v, bf16_t *__restrict__ pw,
+                                             bf16_t *__restrict__ pws) {
+  const bf16_t b = to_bf16(v);
+  pw[k] = b;
 #pragma unroll
-  for (int k = lane; k < MLP_W; k += 64) acc += (float)H[(long)m * MLP_W + k] * w[k];
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
-  if (lane == 0) y[m] = acc + b[0];
-}
-
-// Loss and the output-layer backward: dy = 2 (y - t)/M per row, dZ5 = dy w silu'(Z5); the
-// loss is reduced per block (one atomic per block).
-__global__ void out_backward_kernel(const float *__restrict__ Z5, const float *__restrict__ w,
-                                    const float *__restrict__ y, const float *__restrict__ t, int M,
-                                    float *__restrict__ dy_out, bf16_t *__restrict__ dZ, double *__restrict__ loss) {
-  const int m = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  __shared__ double sl[4];
-  double l = 0.0;
-  if (m < M) {
-    const float diff = y[m] - t[m];
-    const float dy = 2.0f * diff / (float)M;
-    for (int k = lane; k < MLP_W; k += 64) {
-      const long o = (long)m * MLP_W + k;
-      const float z = Z5[o], s = sigmoidf_(z);
-      dZ[o] = to_bf16(dy * w[k] * (s * (1.0f + z * (1.0f - s))));
-    }
-    if (lane == 0) {
-      dy_out[m] = dy;
-      l = (double)diff * (double)diff / (double)M;
+  for (int l = 1; l <= MLP_HIDDEN; l++) {
+    const int Kl = l == 1 ? MLP_K0 : MLP_W;
+    const long d = k - wc.w_off[l];
+    if (d >= 0 && d < (long)MLP_W * Kl) {
+      const int n = (int)(d / Kl), kk = (int)(d % Kl);
+      pws[wc.fs[l] + swz(n, kk, Kl / 16)] = b;
+      if (l >= 2) pws[wc.ts[l] + swz(kk, n, MLP_W / 16)] = b;
     }
   }
-  if (lane == 0) sl[threadIdx.x >> 6] = l;
-  __syncthreads();
-  if (threadIdx.x == 0) atomicAdd(loss, (sl[0] + sl[1]) + (sl[2] + sl[3]));
 }
 
-// Output-layer weight gradients: gw[k] = sum_m dy[m] H[m][k], gb = sum_m dy[m]; rows split
-// over blockIdx.y, one atomic per column per block.
-__global__ void out_wgrad_kernel(const bf16_t *__restrict__ H, const float *__restrict__ dy, int M,
-                                 float *__restrict__ gw, float *__restrict__ gb) {
-  const int k = threadIdx.x;  // 256 columns
-  const int rows = (M + gridDim.y - 1) / gridDim.y, r0 = blockIdx.y * rows;
-  float acc = 0.0f, accb = 0.0f;
-  for (int m = r0; m < r0 + rows && m < M; m++) {
-    const float d = dy[m];
-    acc += d * (float)H[(long)m * MLP_W + k];
-    accb += d;
-  }
-  atomicAdd(&gw[k], acc);
-  if (k == 0) atomicAdd(gb, accb);
+__global__ void refresh_weights_kernel(long n, const float *__restrict__ p, WeightCopies wc, bf16_t *__restrict__ pw,
+                                       bf16_t *__restrict__ pws) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) store_weight(wc, k, p[k], pw, pws);
 }
 
-// Column sums of a BF16 [M x N] matrix into an FP32 gradient (bias gradients).
-__global__ void colsum_kernel(const bf16_t *__restrict__ X, int M, int N, float *__restrict__ g) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rows = (M + gridDim.y - 1) / gridDim.y, r0 = blockIdx.y * rows;
-  if (c >= N) return;
-  float acc = 0.0f;
-  for (int m = r0; m < r0 + rows && m < M; m++) acc += (float)X[(long)m * N + c];
-  atomicAdd(&g[c], acc);
-}
-
-__global__ void adam_kernel(long n, float *__restrict__ p, const float *__restrict__ g, float *__restrict__ mo,
-                            float *__restrict__ ve, float lr, float b1, float b2, float eps, float c1, float c2) {
+// g = sum of the TR_SPLITS partials in range order; with update, an Adam step on p and the
+// BF16 copies refreshed.
+__global__ void mlp_adam_kernel(long n, const float *__restrict__ part, float *__restrict__ g, int update,
+                                float *__restrict__ p, float *__restrict__ mo, float *__restrict__ ve, float lr,
+                                float b1, float b2, float eps, float c1, float c2, WeightCopies wc,
+                                bf16_t *__restrict__ pw, bf16_t *__restrict__ pws) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const float gk = g[k];
+  float gk = 0.0f;
+#pragma unroll
+  for (int s = 0; s < TR_SPLITS; s++) gk += part[(long)s * n + k];
+  g[k] = gk;
+  if (!update) return;
   const float m = b1 * mo[k] + (1.0f - b1) * gk;
   const float v = b2 * ve[k] + (1.0f - b2) * gk * gk;
   mo[k] = m;
   ve[k] = v;
-  p[k] -= lr * (m / c1) / (sqrtf(v / c2) + eps);
-}
-
-__global__ void to_bf16_kernel(long n, const float *__restrict__ x, bf16_t *__restrict__ y) {
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) y[k] = to_bf16(x[k]);
+  const float pk = p[k] - lr * (m / c1) / (sqrtf(v / c2) + eps);
+  p[k] = pk;
+  store_weight(wc, k, pk, pw, pws);
 }
 
 __global__ void gather_rows_kernel(const bf16_t *__restrict__ X, const float *__restrict__ t, long N, int M,
@@ -280,11 +553,6 @@ __global__ void gather_rows_kernel(const bf16_t *__restrict__ X, const float *__
   const long row = (long)(z % (unsigned long long)N);
   for (int k = threadIdx.x; k < MLP_K0; k += blockDim.x) Xb[(long)m * MLP_K0 + k] = X[row * MLP_K0 + k];
   if (threadIdx.x == 0) tb[m] = t[row];
-}
-
-__global__ void zero_kernel(long n, float *x) {
-  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) x[k] = 0.0f;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -421,92 +689,120 @@ struct nrpyell_mlp {
   std::vector<long> w_off, b_off, w_size;
   long n_params = 0;
   float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-  bf16_t *pw = nullptr;  // BF16 copy of all parameters (weights read by the MFMA kernels)
+  float *part = nullptr;  // per-range gradient partials [TR_SPLITS x n_params]
+  bf16_t *pw = nullptr;   // BF16 copy of all parameters (weights read by the MFMA kernels)
+  bf16_t *pws = nullptr;  // BF16 W_l and W_l^T in the MFMA operand layout (WeightCopies)
   long adam_t = 0;
-  int cap = 0;           // activations allocated for this many rows
-  bf16_t *Hs[MLP_HIDDEN + 1] = {};  // H_0 = input, H_1..H_5
-  float *Zs[MLP_HIDDEN + 1] = {};   // Z_1..Z_5
-  bf16_t *dZa = nullptr, *dZb = nullptr;
-  float *y = nullptr, *tb = nullptr;
+  int cap = 0;                       // minibatch arrays allocated for this many rows
+  bf16_t *Ht[MLP_HIDDEN + 1] = {};   // H_l^T [width_l x M], l = 0..5
+  bf16_t *dZt[MLP_HIDDEN + 1] = {};  // dZ_l^T [256 x M], l = 1..5
+  float *dy = nullptr;               // per-row output gradient
   double *loss = nullptr;
   hipStream_t st = nullptr;
   ~nrpyell_mlp() {
-    (void)hipFree(p); (void)hipFree(g); (void)hipFree(m); (void)hipFree(v); (void)hipFree(pw);
-    for (auto *h : Hs) (void)hipFree(h);
-    for (auto *z : Zs) (void)hipFree(z);
-    (void)hipFree(dZa); (void)hipFree(dZb); (void)hipFree(y); (void)hipFree(tb); (void)hipFree(loss); (void)hipFree(dy);
+    (void)hipFree(p); (void)hipFree(g); (void)hipFree(m); (void)hipFree(v); (void)hipFree(part);
+    (void)hipFree(pw); (void)hipFree(pws); (void)hipFree(loss);
+    release();
     if (st) (void)hipStreamDestroy(st);
   }
   int K(int l) const { return l == 1 ? MLP_K0 : MLP_W; }
+  void release() {
+    for (auto *&h : Ht) { (void)hipFree(h); h = nullptr; }
+    for (auto *&d : dZt) { (void)hipFree(d); d = nullptr; }
+    (void)hipFree(dy);
+    dy = nullptr;
+    cap = 0;
+  }
   void ensure(int M) {
     if (M <= cap) return;
     // release first; a failed allocation below (HipFailure) leaves cap 0 and null or valid
     // pointers only, so the next call allocates again
-    cap = 0;
-    for (auto *&h : Hs) { (void)hipFree(h); h = nullptr; }
-    for (auto *&z : Zs) { (void)hipFree(z); z = nullptr; }
-    (void)hipFree(dZa); (void)hipFree(dZb); (void)hipFree(y); (void)hipFree(tb); (void)hipFree(dy);
-    dZa = dZb = nullptr;
-    y = tb = dy = nullptr;
-    Hs[0] = mlp_alloc<bf16_t>((size_t)M * MLP_K0);
+    release();
+    Ht[0] = mlp_alloc<bf16_t>((size_t)M * MLP_K0);
     for (int l = 1; l <= MLP_HIDDEN; l++) {
-      Hs[l] = mlp_alloc<bf16_t>((size_t)M * MLP_W);
-      Zs[l] = mlp_alloc<float>((size_t)M * MLP_W);
+      Ht[l] = mlp_alloc<bf16_t>((size_t)M * MLP_W);
+      dZt[l] = mlp_alloc<bf16_t>((size_t)M * MLP_W);
     }
-    dZa = mlp_alloc<bf16_t>((size_t)M * MLP_W);
-    dZb = mlp_alloc<bf16_t>((size_t)M * MLP_W);
-    y = mlp_alloc<float>(M);
-    tb = mlp_alloc<float>(M);
     dy = mlp_alloc<float>(M);
     cap = M;
   }
-  void refresh_bf16() {
-    hipLaunchKernelGGL(to_bf16_kernel, dim3((n_params + 255) / 256), dim3(256), 0, st, n_params, p, pw);
+  MlpOffsets offsets() const {
+    MlpOffsets off;
+    for (int l = 1; l <= MLP_HIDDEN + 1; l++) {
+      off.w[l] = w_off[l];
+      off.b[l] = b_off[l];
+    }
+    return off;
   }
-  // forward from Hs[0] (M rows, M a multiple of 64) to y
-  void forward(int M) {
+  WeightCopies copies() const {
+    WeightCopies wc;
+    long o = 0;
     for (int l = 1; l <= MLP_HIDDEN; l++) {
-      const int Kl = K(l);
-      dim3 grd(M / 64, MLP_W / 64, 1);
-      hipLaunchKernelGGL(mfma_gemm_kernel<EPI_FWD>, grd, dim3(256), 0, st, Hs[l - 1], (long)Kl, 1L,
-                         pw + w_off[l], (long)Kl, 1L, M, MLP_W, Kl, Kl, p + b_off[l], Zs[l], Hs[l], nullptr,
-                         nullptr);
+      wc.w_off[l] = w_off[l];
+      wc.fs[l] = o;
+      o += (long)MLP_W * K(l);
     }
-    hipLaunchKernelGGL(out_layer_kernel, dim3((M + 3) / 4), dim3(256), 0, st, Hs[MLP_HIDDEN], p + w_off[6],
-                       p + b_off[6], M, y);
-  }
-  // loss and gradients into g (zeroed first); targets in tb
-  void backward(int M) {
-    hipLaunchKernelGGL(zero_kernel, dim3((n_params + 255) / 256), dim3(256), 0, st, n_params, g);
-    HIPCHK(hipMemsetAsync(loss, 0, sizeof(double), st));
-    hipLaunchKernelGGL(out_backward_kernel, dim3((M + 3) / 4), dim3(256), 0, st, Zs[MLP_HIDDEN], p + w_off[6], y, tb,
-                       M, dy, dZa, loss);
-    hipLaunchKernelGGL(out_wgrad_kernel, dim3(1, 64), dim3(MLP_W), 0, st, Hs[MLP_HIDDEN], dy, M, g + w_off[6],
-                       g + b_off[6]);
-    bf16_t *dZ = dZa, *dZn = dZb;
-    for (int l = MLP_HIDDEN; l >= 1; l--) {
-      const int Kl = K(l);
-      // dW_l[n][k] = sum_m dZ[m][n] H_{l-1}[m][k]: rows n, columns k, inner m split in chunks
-      const int chunk = 2048;
-      dim3 gw(MLP_W / 64, Kl / 64, (M + chunk - 1) / chunk);
-      hipLaunchKernelGGL(mfma_gemm_kernel<EPI_WGT>, gw, dim3(256), 0, st, dZ, 1L, (long)MLP_W, Hs[l - 1], 1L,
-                         (long)Kl, MLP_W, Kl, M, chunk, nullptr, nullptr, nullptr, nullptr, g + w_off[l]);
-      hipLaunchKernelGGL(colsum_kernel, dim3(MLP_W / 256, 64), dim3(256), 0, st, dZ, M, MLP_W, g + b_off[l]);
-      if (l > 1) {
-        // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1})
-        hipLaunchKernelGGL(mfma_gemm_kernel<EPI_BWD>, dim3(M / 64, MLP_W / 64, 1), dim3(256), 0, st, dZ, (long)MLP_W,
-                           1L, pw + w_off[l], 1L, (long)Kl, M, MLP_W, MLP_W, MLP_W, nullptr, nullptr, dZn,
-                           Zs[l - 1], nullptr);
-        bf16_t *tmp = dZ;
-        dZ = dZn;
-        dZn = tmp;
-      }
+    for (int l = 2; l <= MLP_HIDDEN; l++) {
+      wc.ts[l] = o;
+      o += (long)MLP_W * MLP_W;
     }
+    wc.w_off[0] = wc.fs[0] = wc.ts[0] = wc.ts[1] = 0;
+    return wc;
   }
-  float *dy = nullptr;  // per-row output gradient
+  static long pws_size() { return (long)MLP_W * MLP_K0 + (2L * MLP_HIDDEN - 2) * MLP_W * MLP_W; }
+  void refresh_bf16() {
+    hipLaunchKernelGGL(refresh_weights_kernel, dim3((n_params + 255) / 256), dim3(256), 0, st, n_params, p, copies(),
+                       pw, pws);
+  }
+  // forward, loss (added to *loss_out) and gradient partials of M minibatch rows (M a multiple
+  // of 64): rows drawn from the dataset (gather) or its first M rows
+  void loss_partials(const bf16_t *X, const float *t, long N, unsigned long long seed, int gather, int M,
+                     double *loss_out) {
+    ensure(M);
+    TrainArgs a;
+    a.X = X;
+    a.t = t;
+    a.N = N;
+    a.seed = seed;
+    a.gather = gather;
+    a.M = M;
+    a.pws = pws;
+    a.wc = copies();
+    a.p = p;
+    a.off = offsets();
+    for (int l = 0; l <= MLP_HIDDEN; l++) {
+      a.Ht[l] = Ht[l];
+      a.dZt[l] = dZt[l];
+    }
+    a.dy = dy;
+    a.loss = loss_out;
+    hipLaunchKernelGGL(mlp_train_rows_kernel, dim3(M / TR_R), dim3(64 * TR_WAVES), 0, st, a);
+    WgradArgs w;
+    for (int l = 0; l <= MLP_HIDDEN; l++) {
+      w.Ht[l] = Ht[l];
+      w.dZt[l] = dZt[l];
+    }
+    w.dy = dy;
+    w.M = M;
+    w.chunk = ((M + TR_SPLITS - 1) / TR_SPLITS + 15) / 16 * 16;
+    w.off = a.off;
+    w.n_params = n_params;
+    w.part = part;
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(4 * WG_GROUPS + TR_SPLITS), dim3(256), 0, st, w);
+  }
+  // g = the summed partials; with update, one Adam step at learning rate lr
+  void reduce(int update, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f;
+    float c1 = 1.0f, c2 = 1.0f;
+    if (update) {
+      adam_t++;
+      c1 = 1.0f - powf(b1, (float)adam_t);
+      c2 = 1.0f - powf(b2, (float)adam_t);
+    }
+    hipLaunchKernelGGL(mlp_adam_kernel, dim3((n_params + 255) / 256), dim3(256), 0, st, n_params, part, g, update, p,
+                       m, v, lr, b1, b2, 1e-8f, c1, c2, copies(), pw, pws);
+  }
 };
-
-static void backward_full(nrpyell_mlp *s, int M) { s->backward(M); }
 
 extern "C" int nrpyell_mlp_create(unsigned long long seed, nrpyell_mlp **out) {
   return api_call([&]() -> int {
@@ -549,6 +845,8 @@ extern "C" int nrpyell_mlp_create(unsigned long long seed, nrpyell_mlp **out) {
     s->m = mlp_alloc<float>(off);
     s->v = mlp_alloc<float>(off);
     s->pw = mlp_alloc<bf16_t>(off);
+    s->pws = mlp_alloc<bf16_t>((size_t)nrpyell_mlp::pws_size());
+    s->part = mlp_alloc<float>((size_t)TR_SPLITS * off);
     s->loss = mlp_alloc<double>(1);
     HIPCHK(hipMemcpy(s->p, h.data(), off * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(s->m, 0, off * sizeof(float)));
@@ -598,7 +896,7 @@ static int check_rows(int M) {
   return NRPYELL_OK;
 }
 
-// Forward pass on M rows of device BF16 features [M x 32]; y (device, FP32) gets M outputs.
+// Forward pass on M rows of device BF16 features [M x 64]; y (device, FP32) gets M outputs.
 extern "C" int nrpyell_mlp_forward(nrpyell_mlp *s, const void *x, int M, float *y_out) {
   return api_call([&]() -> int {
     if (check_rows(M)) return NRPYELL_EINVAL;
@@ -616,15 +914,15 @@ extern "C" int nrpyell_mlp_forward(nrpyell_mlp *s, const void *x, int M, float *
 }
 
 // Loss (mean squared error) and its gradient for M rows; grads copied to host when non-null.
+// Loss (mean squared error) and its gradient for M rows (device BF16 features [M x 64], FP32
+// targets); grads copied to host when non-null.
 extern "C" int nrpyell_mlp_loss_grad(nrpyell_mlp *s, const void *x, const float *t, int M, double *loss,
                                      float *grads_host) {
   return api_call([&]() -> int {
     if (check_rows(M)) return NRPYELL_EINVAL;
-    s->ensure(M);
-    HIPCHK(hipMemcpyAsync(s->Hs[0], x, (size_t)M * MLP_K0 * sizeof(bf16_t), hipMemcpyDeviceToDevice, s->st));
-    HIPCHK(hipMemcpyAsync(s->tb, t, (size_t)M * sizeof(float), hipMemcpyDeviceToDevice, s->st));
-    s->forward(M);
-    backward_full(s, M);
+    HIPCHK(hipMemsetAsync(s->loss, 0, sizeof(double), s->st));
+    s->loss_partials((const bf16_t *)x, t, M, 0, 0, M, s->loss);
+    s->reduce(0, 0.0f);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(loss, s->loss, sizeof(double), hipMemcpyDeviceToHost, s->st));
     if (grads_host) HIPCHK(hipMemcpyAsync(grads_host, s->g, s->n_params * sizeof(float), hipMemcpyDeviceToHost, s->st));
@@ -633,38 +931,28 @@ extern "C" int nrpyell_mlp_loss_grad(nrpyell_mlp *s, const void *x, const float 
   });
 }
 
-static void adam(nrpyell_mlp *s, float lr) {
-  s->adam_t++;
-  const float b1 = 0.9f, b2 = 0.999f;
-  const float c1 = 1.0f - powf(b1, (float)s->adam_t), c2 = 1.0f - powf(b2, (float)s->adam_t);
-  hipLaunchKernelGGL(adam_kernel, dim3((s->n_params + 255) / 256), dim3(256), 0, s->st, s->n_params, s->p, s->g, s->m,
-                     s->v, lr, b1, b2, 1e-8f, c1, c2);
-  s->refresh_bf16();
-}
-
 // nsteps of Adam on minibatches of `batch` rows drawn uniformly from a device dataset
-// (features BF16 [N x 32], targets FP32 [N]); cosine learning-rate decay from lr to lr/100.
-// losses (host, nsteps entries, may be null) receives each step's minibatch loss.
+// (features BF16 [N x 64], targets FP32 [N]); cosine learning-rate decay from lr to lr/100.
+// losses (host, nsteps entries, may be null) receives each step's minibatch loss. Three
+// launches per step (rows, weight gradients, Adam); the seed of step k's draw is
+// seed + 0x632BE59BD9B4E019 (k + 1).
 extern "C" int nrpyell_mlp_train(nrpyell_mlp *s, const void *X, const float *t, long N, int batch, int nsteps,
                                  float lr, unsigned long long seed, double *losses) {
   return api_call([&]() -> int {
-    if (check_rows(batch) || N <= 0) return NRPYELL_EINVAL;
-    s->ensure(batch);
-    std::vector<double> hl;
+    if (check_rows(batch) || N <= 0 || nsteps < 0) return NRPYELL_EINVAL;
     DevBuf<double> dlb(nsteps > 0 ? nsteps : 1);
     double *dl = dlb.p;
+    HIPCHK(hipMemsetAsync(dl, 0, (nsteps > 0 ? nsteps : 1) * sizeof(double), s->st));
     for (int k = 0; k < nsteps; k++) {
-      hipLaunchKernelGGL(gather_rows_kernel, dim3(batch), dim3(32), 0, s->st, (const bf16_t *)X, t, N, batch,
-                         seed + 0x632BE59BD9B4E019ull * (unsigned long long)(k + 1), s->Hs[0], s->tb);
-      s->forward(batch);
-      backward_full(s, batch);
-      HIPCHK(hipMemcpyAsync(dl + k, s->loss, sizeof(double), hipMemcpyDeviceToDevice, s->st));
+      s->loss_partials((const bf16_t *)X, t, N, seed + 0x632BE59BD9B4E019ull * (unsigned long long)(k + 1), 1, batch,
+                       dl + k);
       const float frac = nsteps > 1 ? (float)k / (float)(nsteps - 1) : 0.0f;
       const float lrk = lr * (0.01f + 0.99f * 0.5f * (1.0f + cosf(3.14159265f * frac)));
-      adam(s, lrk);
+      s->reduce(1, lrk);
     }
     HIPCHK(hipGetLastError());
-    if (losses) HIPCHK(hipMemcpyAsync(losses, dl, nsteps * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    if (losses && nsteps > 0)
+      HIPCHK(hipMemcpyAsync(losses, dl, nsteps * sizeof(double), hipMemcpyDeviceToHost, s->st));
     HIPCHK(hipStreamSynchronize(s->st));
     return NRPYELL_OK;
   });
@@ -687,9 +975,11 @@ extern "C" int nrpyell_mlp_sample_rows(const void *x_src, const float *t_src, lo
   });
 }
 
+// One Adam step with the gradient of the last nrpyell_mlp_loss_grad call (its partials).
 extern "C" int nrpyell_mlp_adam_step(nrpyell_mlp *s, float lr) {
   return api_call([&]() -> int {
-    adam(s, lr);
+    s->reduce(1, lr);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->st));
     return NRPYELL_OK;
   });
@@ -720,7 +1010,7 @@ extern "C" long nrpyell_mlp_feature_rows(const griddata_struct *gd) {
   return (n + 63) / 64 * 64;
 }
 
-// Features (device BF16 [rows x 32], padding rows zero), per-row u scale and grid index.
+// Features (device BF16 [rows x 64], padding rows zero), per-row u scale and grid index.
 extern "C" int nrpyell_mlp_features(const griddata_struct *gd, void *X, double *scale, long *idx) {
   return api_call([&]() -> int {
     const GridDims g = dims_from_params(&gd->params);

@@ -30,13 +30,16 @@ def test_forward_matches_fp64_reference():
     assert np.max(np.abs(y - y_ref)) <= 5e-3 * np.max(np.abs(y_ref)), np.max(np.abs(y - y_ref))
 
 
-def test_gradients_match_fp64_reference():
+# the weight-gradient reduction runs over 32 row ranges (multiples of 16 rows): 512 rows give 32
+# ranges of 16; 64 rows four ranges of 16 and 28 empty ones; 576 rows 18 ranges of 32 and 14 empty
+@pytest.mark.parametrize("M", [512, 64, 576])
+def test_gradients_match_fp64_reference(M):
     net = nmlp.InitialGuessMLP(seed=12)
     p = net.get_params().astype(np.float64)
-    bits, X, t = _inputs(512, 2)
+    bits, X, t = _inputs(M, 2)
     x = DeviceArray(bits.size, np.uint16).upload(bits.ravel())
     td = DeviceArray(t.size, np.float32).upload(t.astype(np.float32))
-    loss, g = net.loss_grad(x, td, 512)
+    loss, g = net.loss_grad(x, td, M)
     loss_ref, g_ref = mlp_ref.loss_grad(p, X, t.astype(np.float32).astype(np.float64))
     assert loss == pytest.approx(loss_ref, rel=1e-3)
     # per-tensor relative error of the gradient
@@ -49,6 +52,34 @@ def test_gradients_match_fp64_reference():
             off += size
     a, b = g[off:], g_ref[off:]
     assert np.linalg.norm(a - b) <= 1e-2 * np.linalg.norm(b)
+
+
+def test_gradients_are_deterministic():
+    """The weight-gradient partials are summed in a fixed order: repeated calls agree bit for bit."""
+    net = nmlp.InitialGuessMLP(seed=14)
+    bits, _, t = _inputs(1024, 5)
+    x = DeviceArray(bits.size, np.uint16).upload(bits.ravel())
+    td = DeviceArray(t.size, np.float32).upload(t.astype(np.float32))
+    _, g1 = net.loss_grad(x, td, 1024)
+    _, g2 = net.loss_grad(x, td, 1024)
+    assert np.array_equal(g1, g2)
+
+
+def test_adam_step_applies_the_last_gradient():
+    """nrpyell_mlp_adam_step after loss_grad: the first Adam step moves every parameter with a
+    nonzero gradient by lr * g / (|g| + eps) ~ lr sign(g)."""
+    net = nmlp.InitialGuessMLP(seed=15)
+    p0 = net.get_params()
+    bits, _, t = _inputs(256, 6)
+    x = DeviceArray(bits.size, np.uint16).upload(bits.ravel())
+    td = DeviceArray(t.size, np.float32).upload(t.astype(np.float32))
+    _, g = net.loss_grad(x, td, 256)
+    check_lr = 1e-3
+    from nrpyelliptic_amd.runtime import check
+    check(nmlp._bind().nrpyell_mlp_adam_step(net.h, check_lr), "adam_step")
+    dp = net.get_params() - p0
+    big = np.abs(g) > 1e-5
+    np.testing.assert_allclose(dp[big], -check_lr * np.sign(g[big]), rtol=2e-2, atol=0)
 
 
 def test_training_reduces_loss():
