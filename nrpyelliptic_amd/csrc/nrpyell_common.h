@@ -176,6 +176,15 @@ static __constant__ double c_fd1_shift[6][11] = {
     {-1.0 / 90, 1.0 / 8, -9.0 / 14, 2.0, -21.0 / 5, 63.0 / 10, -7.0, 6.0, -9.0 / 2, 4609.0 / 2520, 1.0 / 10},
     {1.0 / 10, -10.0 / 9, 45.0 / 8, -120.0 / 7, 35.0, -252.0 / 5, 105.0 / 2, -40.0, 45.0 / 2, -10.0, 7381.0 / 2520}};
 
+// Kernels whose lanes use different rows of c_fd1_shift work from a copy of it in LDS: a
+// lane-indexed read of the constant table is a vector memory load, i.e. a global-memory round
+// trip wherever the stencil is evaluated. Each of the first NRPYELL_FD_TABLE threads of a block
+// loads one entry with its other loads and stores it before the block's first barrier.
+#define NRPYELL_FD_TABLE 66
+__device__ __forceinline__ double load_fd_table(int tid) {
+  return tid < NRPYELL_FD_TABLE ? (&c_fd1_shift[0][0])[tid] : 0.0;
+}
+
 // 7-point one-sided first-derivative weights (6th order; taps 0..6 from the point outward):
 // -49/20, 6, -15/2, 20/3, -15/4, 6/5, -1/6.
 static __constant__ double c_fd1_onesided7[7] = {-49.0 / 20.0, 6.0, -15.0 / 2.0, 20.0 / 3.0,

@@ -196,7 +196,7 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   constexpr int NT = NRPYELL_BLOCK, NRB = (NBOX + NT - 1) / NT, NGEO = (24 * TYB + NT - 1) / NT;
   static_assert(12 * TYB <= NT && 10 * TYB <= NT, "one BC work item per thread");
   __shared__ T bu[NRB * NT], bv[NRB * NT];  // [by][c]: u, v at column ilast-6+c, row i1s-NG+by
-  __shared__ T s_rhs0[TYB][2], s_A[TYB][2][6], s_geo[6][TYB][4];
+  __shared__ T s_rhs0[TYB][2], s_A[TYB][2][6], s_geo[6][TYB][4], s_fd[NRPYELL_FD_TABLE];
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x;
@@ -247,6 +247,7 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
   const bool od_on = tid < 10 * TYB && i1s + od_row < g.N1 + NG;
   const T od_yn = ld32(a.yn, od_p);
   const T od_rt = STAGE != 1 ? ld32((const T *)a.rt, od_p) : (T)0;
+  const T fdv = (T)load_fd_table(tid);
   if (done) return;
 #pragma unroll
   for (int r = 0; r < NRB; r++) {
@@ -256,6 +257,7 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
 #pragma unroll
   for (int r = 0; r < NGEO; r++)
     if (tid + NT * r < 24 * TYB) (&s_geo[0][0][0])[tid + NT * r] = geo_r[r];
+  if (tid < NRPYELL_FD_TABLE) s_fd[tid] = fdv;
   __syncthreads();
 
   if (tid < TYB && i1s + tid < g.N1 + NG) {
@@ -281,7 +283,7 @@ __device__ __forceinline__ void bc_body(const SubstepArgs<T> &a, const int rb, c
       if (a.bc_kind == NRPYELL_OUTER_BC_RADIATION) {
 #pragma unroll
         for (int j = 0; j < 11; j++) {
-          d0 = fma((T)c_fd1_shift[sh][j], box[by * NBX + c0 + (j - 5 - sh)], d0);
+          d0 = fma(s_fd[sh * 11 + j], box[by * NBX + c0 + (j - 5 - sh)], d0);
           d1 = fma((T)c_fd1_shift[0][j], box[(by + j - 5) * NBX + c0], d1);
         }
       } else {

@@ -51,15 +51,17 @@ __device__ __forceinline__ double next_input(double k, double dt, double yn) {
 // Radiation term A = -c (d_r f + f / r) at outer layer pt (pt = 0: the last interior point) of
 // apply_bcs_outerradiation_and_inner.c:290-408, as bc_body evaluates it: f(di0, di1) is f at
 // offsets (di0, di1) from the point, d_0 the 11-point stencil shifted by -max(1, pt) toward the
-// interior, d_1 the centred one; geo = (1/r, dxx0/dr, dxx1/dr, (r_0/r)^3).
+// interior, d_1 the centred one; geo = (1/r, dxx0/dr, dxx1/dr, (r_0/r)^3). fd is the block's LDS
+// copy of c_fd1_shift (load_fd_table): pt differs between lanes, and a lane-indexed read of the
+// constant table is a global-memory round trip in the middle of the block's dependency chain.
 template <class F>
 __device__ __forceinline__ double radiation_A(F &&f, int pt, const double geo[4], double c, double invdx0,
-                                              double invdx1) {
+                                              double invdx1, const double *fd) {
   const int sh = pt < 1 ? 1 : pt;
   double d0 = 0.0, d1 = 0.0;
 #pragma unroll
   for (int j = 0; j < 11; j++) {
-    d0 = fma(c_fd1_shift[sh][j], f(j - 5 - sh, 0), d0);
+    d0 = fma(fd[sh * 11 + j], f(j - 5 - sh, 0), d0);
     d1 = fma(c_fd1_shift[0][j], f(0, j - 5), d1);
   }
   d0 *= invdx0;
@@ -96,12 +98,12 @@ __device__ __forceinline__ void vrow_source(const GridDims &g, int vr, int i2, i
 template <int TX, int PL>
 constexpr int interior2_lds() {
   constexpr int NT = NRPYELL_BLOCK, TY = NT / TX, BOX = (TX + 2 * NG) * (TY + 2 * NG), NR = (BOX + NT - 1) / NT;
-  return 2 * PL * NR * NT + 24 * TY + NT / 64;
+  return 2 * PL * NR * NT + 24 * TY + NT / 64 + NRPYELL_FD_TABLE;
 }
 constexpr int bc2_lds() {
   constexpr int NT = NRPYELL_BLOCK, TYB = NRPYELL_PAIR_BC_ROWS, NR1 = TYB + 2 * NG;
   constexpr int NRB = ((TYB + 4 * NG) * 17 + NT - 1) / NT;
-  return 2 * NRB * NT + NR1 * (24 + 10 + 24 + 2 + 12) + TYB * (10 + 30 + 8 + 10 + 2 + 12);
+  return 2 * NRB * NT + NR1 * (24 + 10 + 24 + 2 + 12) + TYB * (10 + 30 + 8 + 10 + 2 + 12) + NRPYELL_FD_TABLE;
 }
 
 template <int PAIR, int TX, int PL>
@@ -119,6 +121,7 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
   double(*s1)[NR * NT] = reinterpret_cast<double(*)[NR * NT]>(lds + PL * NR * NT);
   double(*s_geo)[TY][4] = reinterpret_cast<double(*)[TY][4]>(lds + 2 * PL * NR * NT);
   double *s_red = lds + 2 * PL * NR * NT + 24 * TY;
+  double *s_fd = s_red + NT / 64;
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -182,8 +185,9 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
   const int bq = min(tid / (5 * TY), PL - 1), brow = (tid / 5) % TY, bpt = tid % 5 + 1;
   const bool b_on = tid < NBI && i1s + brow < g.N1 + NG;
   double b_yu = 0.0, b_xv0 = 0.0;
-  double geo_r[NGEO];
+  double geo_r[NGEO], fdv = 0.0;
   if (last_tile) {
+    fdv = load_fd_table(tid);
     const unsigned bp = (unsigned)(i2a + bq) * s2 + (unsigned)min(i1s + brow, g.N1 + NG - 1) * s1s;
     b_yu = ld32(a.yn, bp + ilast + bpt);
     b_xv0 = ld32(a.in, nt + bp + ilast);
@@ -209,6 +213,7 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
 #pragma unroll
     for (int r = 0; r < NGEO; r++)
       if (tid + NT * r < 24 * TY) (&s_geo[0][0][0])[tid + NT * r] = geo_r[r];
+    if (tid < NRPYELL_FD_TABLE) s_fd[tid] = fdv;
   }
   __syncthreads();
 
@@ -219,9 +224,9 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
       const int cl = ilast - i0s + NG, by = brow + NG;  // box column of ilast, box row
       const double *box = sx[bq];
       const double Ap = radiation_A([&](int d0, int d1) { return box[(by + d1) * BX + cl + bpt + d0]; }, bpt,
-                                    s_geo[bpt][brow], a.cOB_u, g.invdx0, g.invdx1);
+                                    s_geo[bpt][brow], a.cOB_u, g.invdx0, g.invdx1, s_fd);
       const double A0 = radiation_A([&](int d0, int d1) { return box[(by + d1) * BX + cl + d0]; }, 0,
-                                    s_geo[0][brow], a.cOB_u, g.invdx0, g.invdx1);
+                                    s_geo[0][brow], a.cOB_u, g.invdx0, g.invdx1, s_fd);
       const double rhs0 = fma(-eta, box[by * BX + cl], b_xv0);
       const double rhs = fma(s_geo[bpt][brow][3], rhs0 - A0, Ap);
       s1[bq][by * BX + cl + bpt] = next_input<S0>(rhs, dt, b_yu);
@@ -360,6 +365,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   double(*srto)[2][5] = reinterpret_cast<double(*)[2][5]>(take(TYB * 10));  // running total at the outputs
   double(*sk1)[2] = reinterpret_cast<double(*)[2]>(take(TYB * 2));
   double(*sA1)[2][6] = reinterpret_cast<double(*)[2][6]>(take(TYB * 12));
+  double *s_fd = take(NRPYELL_FD_TABLE);
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x;
@@ -430,6 +436,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     const unsigned pa = (unsigned)wrap2_near(g, i2 + kk) * s2 + (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + ilast;
     larm[r] = f == 0 ? ld32(a.in, pa) : f == 1 ? ld32(a.in, nt + pa) : ld32(a.yn, pa);
   }
+  const double lfd = load_fd_table(tid);
   double lkl = 0.0;
   if (tid < TYB * 8) {
     const int tr = tid / 8, c = tid % 8;
@@ -455,6 +462,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   for (int r = 0; r < NARMR; r++)
     if (tid + NT * r < NARM) (&sarm[0][0][0])[tid + NT * r] = larm[r];
   if (tid < TYB * 8) (&skl[0][0])[tid] = lkl;
+  if (tid < NRPYELL_FD_TABLE) s_fd[tid] = lfd;
   __syncthreads();
   // box value at virtual row vr (absolute) and column c (absolute)
   auto BX_U = [&](int vr, int c) { return bxu[(vr - (r0 - 2 * NG)) * NC + (c - c_lo)]; };
@@ -498,9 +506,9 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
       const int col = ilast + pt;
       sA0[e][gf][pt] = gf == 0
           ? radiation_A([&](int d0, int d1) { return BX_U(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_u,
-                        g.invdx0, g.invdx1)
+                        g.invdx0, g.invdx1, s_fd)
           : radiation_A([&](int d0, int d1) { return BX_V(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_v,
-                        g.invdx0, g.invdx1);
+                        g.invdx0, g.invdx1, s_fd);
     }
   }
   __syncthreads();
@@ -545,7 +553,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     if (r0 + tr < g.N1 + NG) {
       const double(*R)[12] = gf == 0 ? ru1 : rv1;
       sA1[tr][gf][pt] = radiation_A([&](int d0, int d1) { return R[e + d1][6 + pt + d0]; }, pt, sgeo[e][pt],
-                                    gf == 0 ? a.cOB_u : a.cOB_v, g.invdx0, g.invdx1);
+                                    gf == 0 ? a.cOB_u : a.cOB_v, g.invdx0, g.invdx1, s_fd);
     }
   }
   __syncthreads();
@@ -572,7 +580,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
 // One configuration: the nbc_rows*N2 BC blocks first, then the interior blocks in XCD-aware
 // order.
 template <int PAIR, int TX, int PL>
-__global__ void __launch_bounds__(NRPYELL_BLOCK) substep2_kernel(SubstepArgs<double> a) {
+__global__ void __launch_bounds__(NRPYELL_BLOCK, 3) substep2_kernel(SubstepArgs<double> a) {
   constexpr int NL = interior2_lds<TX, PL>() > bc2_lds() ? interior2_lds<TX, PL>() : bc2_lds();
   __shared__ double lds[NL];
   const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
