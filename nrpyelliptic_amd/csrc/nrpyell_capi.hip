@@ -798,7 +798,8 @@ struct nrpyell_solver {
     launch_substep_pair(a, 3, st);
   }
   // the pair kernel's BC blocks cover NRPYELL_PAIR_BC_ROWS rows each
-  void pair_layout(SubstepArgs<double> &a) const {
+  template <typename T>
+  void pair_layout(SubstepArgs<T> &a) const {
     a.nbc_rows = (g.N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS;
     a.nbc_mul = div_mul((unsigned)a.nbc_rows);
   }
@@ -810,6 +811,24 @@ struct nrpyell_solver {
   }
   // RK4 steps in a graph: the pair path needs an even count (y_n back in y_n_gfs)
   int graph_steps64() const { return pair_path ? opt.steps_per_graph + (opt.steps_per_graph & 1) : opt.steps_per_graph; }
+  // ... and in the FP32 phase also a multiple of check_every (steps_per_graph is one)
+  int graph_steps32() const { return pair_path && (opt.steps_per_graph & 1) ? 2 * opt.steps_per_graph : opt.steps_per_graph; }
+  // The FP32 state of the pair path alternates between y32 and ke32 like the FP64 one.
+  float *ycur32(int j) const { return j == 0 ? y32 : ke32; }
+  // One FP32 RK4 step in two launches, from ycur32(j) to ycur32(1 - j) (no fused residual:
+  // the FP32 phase checks the residual in FP64 every check_every steps).
+  void step32_pair(hipStream_t st, int j) {
+    SubstepArgs<float> a = args<float>(1, ycur32(j), ko32, nullptr, rt32, src32, kc32, geo32);
+    a.in = ycur32(j);
+    a.yn = ycur32(j);
+    a.out = ko32;
+    pair_layout(a);
+    launch_substep_pair(a, 1, st);
+    a.in = ko32;
+    a.yn = ycur32(j);
+    a.out = ycur32(1 - j);
+    launch_substep_pair(a, 3, st);
+  }
   // One FP64 RK4 step; with fuse_check the stage-1 kernel also reduces the residual L2 of
   // y_n (before the step) into the convergence state.
   void step64(hipStream_t st, bool fuse_check = false) {
@@ -854,9 +873,10 @@ struct nrpyell_solver {
   void step32(hipStream_t st) {
     for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, src32, kc32, geo32), s, st);
   }
-  // residual of the current y_n (FP64 or the FP32 shadow) and a convergence-state update
-  void check(hipStream_t st, bool from32, int steps_per_check, double target) {
-    const void *u = from32 ? (const void *)y32 : (const void *)gd->gridfuncs.y_n_gfs;
+  // residual of the current y_n (FP64, or the FP32 shadow in u32, y32 by default) and a
+  // convergence-state update
+  void check(hipStream_t st, bool from32, int steps_per_check, double target, const float *u32 = nullptr) {
+    const void *u = from32 ? (const void *)(u32 ? u32 : y32) : (const void *)gd->gridfuncs.y_n_gfs;
     const long npart = launch_residual_l2(g, u, from32, gd->gridfuncs.auxevol_gfs, coef64, w2d, partial,
                                           &dstate->done, nullptr, st);
     launch_l2_finalize(partial, npart, volume, dstate, steps_per_check, target, l2_out, st);
@@ -1041,9 +1061,12 @@ static hipGraphExec_t capture(nrpyell_solver *s, bool fp32, int nsteps, double t
   HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
   for (int k = 1; k <= nsteps; k++) {
     if (fp32) {
-      // FP32 bulk phase: the residual is evaluated in FP64 from the FP32 state every check_every steps
-      s->step32(s->stream);
-      if (k % s->opt.check_every == 0) s->check(s->stream, true, s->opt.check_every, target);
+      // FP32 bulk phase: the residual is evaluated in FP64 from the FP32 state every check_every
+      // steps; on the pair path the state after step k is in ycur32(k & 1) (nsteps is even)
+      if (s->pair_path) s->step32_pair(s->stream, (k - 1) & 1);
+      else s->step32(s->stream);
+      if (k % s->opt.check_every == 0)
+        s->check(s->stream, true, s->opt.check_every, target, s->pair_path ? s->ycur32(k & 1) : s->y32);
     } else if (s->pair_path) {
       // FP64, two substeps per launch: y alternates between the two arrays (nsteps is even)
       s->step64_pair(s->stream, (k - 1) & 1, true);
@@ -1107,7 +1130,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       if (!s->graph32 || !s->key32.same(k32)) {
         if (s->graph32) (void)hipGraphExecDestroy(s->graph32);
         s->graph32 = nullptr;
-        s->graph32 = capture(s, true, s->opt.steps_per_graph, s->opt.fp32_switch_log10);
+        s->graph32 = capture(s, true, s->graph_steps32(), s->opt.fp32_switch_log10);
         s->key32 = k32;
       }
       reset_state(s);
@@ -1135,7 +1158,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
           best = h.last_log10;
           since_best = 0;
         } else {
-          since_best += s->opt.steps_per_graph;
+          since_best += s->graph_steps32();
         }
         if (since_best >= s->opt.fp32_plateau_steps) break;
         if (h.last_log10 > best + 2.0) break;
@@ -1144,7 +1167,9 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       res->fp32_steps = steps;
       std::vector<int> h32s(s->hstate->hist_step, s->hstate->hist_step + s->hstate->nhist);
       std::vector<double> h32l(s->hstate->hist_log10, s->hstate->hist_log10 + s->hstate->nhist);
-      launch_convert_f2d(2 * nt, s->y32, f.y_n_gfs, s->stream);
+      // a check that ended the phase leaves the state of its step count (pair path: in ycur32 of
+      // that parity; whole graphs run an even number of steps)
+      launch_convert_f2d(2 * nt, s->pair_path ? s->ycur32(steps & 1) : s->y32, f.y_n_gfs, s->stream);
       reset_state(s);
       s->check(s->stream, false, 0, s->opt.log10_target);
       read_state(s);

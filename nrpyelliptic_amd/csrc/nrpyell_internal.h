@@ -174,7 +174,8 @@ void launch_fold(const GridDims &g, const double *aux, const double *coef, const
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 // two RK4 substeps (first_stage 1: stages 1+2; 3: stages 3+4) in one launch (nrpyell_substep2.hip)
-void launch_substep_pair(const SubstepArgs<double> &a, int first_stage, hipStream_t st);
+template <typename T>
+void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st);
 template <typename T>
 void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
                             int tx, int pl, int stage, hipStream_t st);

@@ -1,5 +1,6 @@
-// nrpyell_substep2.hip -- two RK4 substeps per launch (stages 1+2, then stages 3+4): the FP64
-// path of the resident solver with the radiation outer BC.
+// nrpyell_substep2.hip -- two RK4 substeps per launch (stages 1+2, then stages 3+4): the path of
+// the resident solver with the radiation outer BC, in FP64 and in the FP32 bulk phase of the
+// mixed-precision solve (templated on the arithmetic type T).
 //
 // RK4 on  u_t = v - eta u,  v_t = c^2 (lap u + AA^2 / (8 (psi+u)^7))  couples neighbouring points
 // only through the stencil of u: the next stage's u is pointwise,
@@ -42,10 +43,10 @@
 
 // Stage s's contribution to the next stage's input, y + a_s dt k_s, with the FMA placement of
 // stage_update (MoL_step_forward_in_time.c:44-45, :88-89, :129-130).
-template <int STAGE>
-__device__ __forceinline__ double next_input(double k, double dt, double yn) {
+template <typename T, int STAGE>
+__device__ __forceinline__ T next_input(T k, T dt, T yn) {
   if (STAGE == 3) return fma(dt, k, yn);
-  return fma(0.5, dt * k, yn);
+  return fma((T)0.5, dt * k, yn);
 }
 
 // Radiation term A = -c (d_r f + f / r) at outer layer pt (pt = 0: the last interior point) of
@@ -54,19 +55,18 @@ __device__ __forceinline__ double next_input(double k, double dt, double yn) {
 // interior, d_1 the centred one; geo = (1/r, dxx0/dr, dxx1/dr, (r_0/r)^3). fd is the block's LDS
 // copy of c_fd1_shift (load_fd_table): pt differs between lanes, and a lane-indexed read of the
 // constant table is a global-memory round trip in the middle of the block's dependency chain.
-template <class F>
-__device__ __forceinline__ double radiation_A(F &&f, int pt, const double geo[4], double c, double invdx0,
-                                              double invdx1, const double *fd) {
+template <typename T, class F>
+__device__ __forceinline__ T radiation_A(F &&f, int pt, const T geo[4], T c, T invdx0, T invdx1, const T *fd) {
   const int sh = pt < 1 ? 1 : pt;
-  double d0 = 0.0, d1 = 0.0;
+  T d0 = 0, d1 = 0;
 #pragma unroll
   for (int j = 0; j < 11; j++) {
     d0 = fma(fd[sh * 11 + j], f(j - 5 - sh, 0), d0);
-    d1 = fma(c_fd1_shift[0][j], f(0, j - 5), d1);
+    d1 = fma((T)c_fd1_shift[0][j], f(0, j - 5), d1);
   }
   d0 *= invdx0;
   d1 *= invdx1;
-  const double dr = geo[1] * d0 + geo[2] * d1;
+  const T dr = geo[1] * d0 + geo[2] * d1;
   return -c * (dr + f(0, 0) * geo[0]);
 }
 
@@ -93,12 +93,12 @@ __device__ __forceinline__ void vrow_source(const GridDims &g, int vr, int i2, i
 // ---------------------------------------------------------------------------------------
 // Interior blocks.
 // ---------------------------------------------------------------------------------------
-// LDS (doubles) of an interior block and of a BC block; the kernel holds one buffer of the
-// larger size and each body lays its arrays out in it.
+// LDS (elements of T) of an interior block and of a BC block; the kernel holds one buffer of the
+// larger size and each body lays its arrays out in it (the block sums have their own array).
 template <int TX, int PL>
 constexpr int interior2_lds() {
   constexpr int NT = NRPYELL_BLOCK, TY = NT / TX, BOX = (TX + 2 * NG) * (TY + 2 * NG), NR = (BOX + NT - 1) / NT;
-  return 2 * PL * NR * NT + 24 * TY + NT / 64 + NRPYELL_FD_TABLE;
+  return 2 * PL * NR * NT + 24 * TY + NRPYELL_FD_TABLE;
 }
 constexpr int bc2_lds() {
   constexpr int NT = NRPYELL_BLOCK, TYB = NRPYELL_PAIR_BC_ROWS, NR1 = TYB + 2 * NG;
@@ -106,9 +106,9 @@ constexpr int bc2_lds() {
   return 2 * NRB * NT + NR1 * (24 + 10 + 24 + 2 + 12) + TYB * (10 + 30 + 8 + 10 + 2 + 12) + NRPYELL_FD_TABLE;
 }
 
-template <int PAIR, int TX, int PL>
-__device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, const int tile, const int pgroup,
-                                               const unsigned bid, const unsigned nblk, double *lds) {
+template <typename T, int PAIR, int TX, int PL>
+__device__ __forceinline__ void interior2_body(const SubstepArgs<T> &a, const int tile, const int pgroup,
+                                               const unsigned bid, const unsigned nblk, T *lds, double *s_red) {
   constexpr int S0 = PAIR;
   constexpr int NT = NRPYELL_BLOCK, TY = NT / TX;
   constexpr int BX = TX + 2 * NG, BY = TY + 2 * NG, BOX = BX * BY, NR = (BOX + NT - 1) / NT;
@@ -117,11 +117,10 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
   constexpr int NGEO = (24 * TY + NT - 1) / NT;
   static_assert(NBI <= NT, "one outer point per thread");
   // stage-s u and stage-(s+1) u on the box of each plane, the last tile's geometry, block sums
-  double(*sx)[NR * NT] = reinterpret_cast<double(*)[NR * NT]>(lds);
-  double(*s1)[NR * NT] = reinterpret_cast<double(*)[NR * NT]>(lds + PL * NR * NT);
-  double(*s_geo)[TY][4] = reinterpret_cast<double(*)[TY][4]>(lds + 2 * PL * NR * NT);
-  double *s_red = lds + 2 * PL * NR * NT + 24 * TY;
-  double *s_fd = s_red + NT / 64;
+  T(*sx)[NR * NT] = reinterpret_cast<T(*)[NR * NT]>(lds);
+  T(*s1)[NR * NT] = reinterpret_cast<T(*)[NR * NT]>(lds + PL * NR * NT);
+  T(*s_geo)[TY][4] = reinterpret_cast<T(*)[TY][4]>(lds + 2 * PL * NR * NT);
+  T *s_fd = lds + 2 * PL * NR * NT + 24 * TY;
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
@@ -134,16 +133,16 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
   const bool last_tile = t0 == a.ntile0 - 1;  // block-uniform
   const unsigned nt = (unsigned)g.ntot, s1s = g.G0, s2 = (unsigned)(g.G0 * g.G1);
   const bool fuse_l2 = (PAIR == 1) && (a.l2_partial != nullptr);
-  const double eta = a.eta, dt = a.dt;
+  const T eta = a.eta, dt = a.dt;
   const int done = *a.done;
 
   // ---- loads: unconditional, from clamped indices (values a stencil never reaches are loaded
   // and not read). For stages 1+2 the stage input is y itself.
   const int i0c = min(max(i0, NG), ilast), i1c = min(i1, g.N1 + NG - 1);
   const unsigned k2 = (unsigned)i1c * s1s + i0c;
-  double bxu[PL][NR], bxv[PL][NR], byu[PL][NR];
-  double wxu[NW], wxv[NW], wyu[NW];
-  double yv[PL], psi[PL], AAc[PL], rtu[PL], rtv[PL], K[5];
+  T bxu[PL][NR], bxv[PL][NR], byu[PL][NR];
+  T wxu[NW], wxv[NW], wyu[NW];
+  T yv[PL], psi[PL], AAc[PL], rtu[PL], rtv[PL], K[5];
   double w2 = 0.0;
 #pragma unroll
   for (int q = 0; q < PL; q++) {
@@ -165,29 +164,29 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
         wxv[jj] = ld32(a.in, nt + pw);
         wyu[jj] = PAIR == 1 ? wxu[jj] : ld32(a.yn, pw);
       }
-      double kw;
+      T kw;
       ld2x32(a.coef, 6 * k2, K[0], K[1]);
       ld2x32(a.coef, 6 * k2 + 2, K[2], K[3]);
       ld2x32(a.coef, 6 * k2 + 4, K[4], kw);
-      if (fuse_l2) w2 = kw;
+      if (fuse_l2) w2 = (double)kw;
     }
     const unsigned p = (unsigned)(i2a + q) * s2 + k2;
     yv[q] = PAIR == 1 ? wxv[NG + q] : ld32(a.yn, nt + p);
     ld2x32(a.src, 2 * p, psi[q], AAc[q]);
-    rtu[q] = 0.0;
-    rtv[q] = 0.0;
+    rtu[q] = 0;
+    rtv[q] = 0;
     if (PAIR == 3) {
-      rtu[q] = ld32((const double *)a.rt, p);
-      rtv[q] = ld32((const double *)a.rt, nt + p);
+      rtu[q] = ld32((const T *)a.rt, p);
+      rtv[q] = ld32((const T *)a.rt, nt + p);
     }
   }
   // the last tile's outer points: plane bq, row brow, layer bpt; y_u there and v at ilast
   const int bq = min(tid / (5 * TY), PL - 1), brow = (tid / 5) % TY, bpt = tid % 5 + 1;
   const bool b_on = tid < NBI && i1s + brow < g.N1 + NG;
-  double b_yu = 0.0, b_xv0 = 0.0;
-  double geo_r[NGEO], fdv = 0.0;
+  T b_yu = 0, b_xv0 = 0;
+  T geo_r[NGEO], fdv = 0;
   if (last_tile) {
-    fdv = load_fd_table(tid);
+    fdv = (T)load_fd_table(tid);
     const unsigned bp = (unsigned)(i2a + bq) * s2 + (unsigned)min(i1s + brow, g.N1 + NG - 1) * s1s;
     b_yu = ld32(a.yn, bp + ilast + bpt);
     b_xv0 = ld32(a.in, nt + bp + ilast);
@@ -206,7 +205,7 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
 #pragma unroll
     for (int r = 0; r < NR; r++) {
       sx[q][tid + NT * r] = bxu[q][r];
-      s1[q][tid + NT * r] = next_input<S0>(fma(-eta, bxu[q][r], bxv[q][r]), dt, byu[q][r]);
+      s1[q][tid + NT * r] = next_input<T, S0>(fma(-eta, bxu[q][r], bxv[q][r]), dt, byu[q][r]);
     }
   }
   if (last_tile) {
@@ -222,14 +221,14 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
   if (last_tile) {
     if (b_on) {
       const int cl = ilast - i0s + NG, by = brow + NG;  // box column of ilast, box row
-      const double *box = sx[bq];
-      const double Ap = radiation_A([&](int d0, int d1) { return box[(by + d1) * BX + cl + bpt + d0]; }, bpt,
-                                    s_geo[bpt][brow], a.cOB_u, g.invdx0, g.invdx1, s_fd);
-      const double A0 = radiation_A([&](int d0, int d1) { return box[(by + d1) * BX + cl + d0]; }, 0,
-                                    s_geo[0][brow], a.cOB_u, g.invdx0, g.invdx1, s_fd);
-      const double rhs0 = fma(-eta, box[by * BX + cl], b_xv0);
-      const double rhs = fma(s_geo[bpt][brow][3], rhs0 - A0, Ap);
-      s1[bq][by * BX + cl + bpt] = next_input<S0>(rhs, dt, b_yu);
+      const T *box = sx[bq];
+      const T Ap = radiation_A<T>([&](int d0, int d1) { return box[(by + d1) * BX + cl + bpt + d0]; }, bpt,
+                                  s_geo[bpt][brow], a.cOB_u, (T)g.invdx0, (T)g.invdx1, s_fd);
+      const T A0 = radiation_A<T>([&](int d0, int d1) { return box[(by + d1) * BX + cl + d0]; }, 0,
+                                  s_geo[0][brow], a.cOB_u, (T)g.invdx0, (T)g.invdx1, s_fd);
+      const T rhs0 = fma(-eta, box[by * BX + cl], b_xv0);
+      const T rhs = fma(s_geo[bpt][brow][3], rhs0 - A0, Ap);
+      s1[bq][by * BX + cl + bpt] = next_input<T, S0>(rhs, dt, b_yu);
     }
     __syncthreads();
   }
@@ -242,43 +241,43 @@ __device__ __forceinline__ void interior2_body(const SubstepArgs<double> &a, con
     if (valid) {
       const int i2 = i2a + q;
       const unsigned p = (unsigned)i2 * s2 + k2;
-      double xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+      T xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
       // stage s
-      const double uu = wxu[NG + q], vv = wxv[NG + q];
+      const T uu = wxu[NG + q], vv = wxv[NG + q];
 #pragma unroll
       for (int k = 1; k <= 5; k++) {
         xm[k] = sx[q][cb - k]; xp[k] = sx[q][cb + k];
         ym[k] = sx[q][cb - k * BX]; yp[k] = sx[q][cb + k * BX];
         zm[k] = wxu[NG + q - k]; zp[k] = wxu[NG + q + k];
       }
-      double ku0, kv0;
-      point_rhs<double>(uu, vv, psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, eta, ku0, kv0);
-      if (fuse_l2) l2acc += w2 * kv0 * kv0;  // w / c^4 x (c^2 R)^2
+      T ku0, kv0;
+      point_rhs<T>(uu, vv, psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, eta, ku0, kv0);
+      if (fuse_l2) l2acc += w2 * (double)kv0 * (double)kv0;  // w / c^4 x (c^2 R)^2
       // stage s+1
-      const double u1 = s1[q][cb];
-      const double v1 = next_input<S0>(kv0, dt, yv[q]);
+      const T u1 = s1[q][cb];
+      const T v1 = next_input<T, S0>(kv0, dt, yv[q]);
 #pragma unroll
       for (int k = 1; k <= 5; k++) {
         xm[k] = s1[q][cb - k]; xp[k] = s1[q][cb + k];
         ym[k] = s1[q][cb - k * BX]; yp[k] = s1[q][cb + k * BX];
-        zm[k] = next_input<S0>(fma(-eta, wxu[NG + q - k], wxv[NG + q - k]), dt, wyu[NG + q - k]);
-        zp[k] = next_input<S0>(fma(-eta, wxu[NG + q + k], wxv[NG + q + k]), dt, wyu[NG + q + k]);
+        zm[k] = next_input<T, S0>(fma(-eta, wxu[NG + q - k], wxv[NG + q - k]), dt, wyu[NG + q - k]);
+        zp[k] = next_input<T, S0>(fma(-eta, wxu[NG + q + k], wxv[NG + q + k]), dt, wyu[NG + q + k]);
       }
-      double ku1, kv1;
-      point_rhs<double>(u1, v1, psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, eta, ku1, kv1);
-      double ou, ov;
+      T ku1, kv1;
+      point_rhs<T>(u1, v1, psi[q], AAc[q], K, xm, xp, ym, yp, zm, zp, eta, ku1, kv1);
+      T ou, ov;
       if (PAIR == 1) {
-        double ru = (1.0 / 6.0) * (dt * ku0), rv = (1.0 / 6.0) * (dt * kv0);
-        ru = fma(1.0 / 3.0, dt * ku1, ru);
-        rv = fma(1.0 / 3.0, dt * kv1, rv);
-        ou = fma(0.5, dt * ku1, wyu[NG + q]);
-        ov = fma(0.5, dt * kv1, yv[q]);
+        T ru = (T)(1.0 / 6.0) * (dt * ku0), rv = (T)(1.0 / 6.0) * (dt * kv0);
+        ru = fma((T)(1.0 / 3.0), dt * ku1, ru);
+        rv = fma((T)(1.0 / 3.0), dt * kv1, rv);
+        ou = fma((T)0.5, dt * ku1, wyu[NG + q]);
+        ov = fma((T)0.5, dt * kv1, yv[q]);
         st32(a.rt, p, ru);
         st32(a.rt, nt + p, rv);
       } else {
-        const double ru = fma(1.0 / 3.0, dt * ku0, rtu[q]), rv = fma(1.0 / 3.0, dt * kv0, rtv[q]);
-        ou = wyu[NG + q] + fma(1.0 / 6.0, dt * ku1, ru);
-        ov = yv[q] + fma(1.0 / 6.0, dt * kv1, rv);
+        const T ru = fma((T)(1.0 / 3.0), dt * ku0, rtu[q]), rv = fma((T)(1.0 / 3.0), dt * kv0, rtv[q]);
+        ou = wyu[NG + q] + fma((T)(1.0 / 6.0), dt * ku1, ru);
+        ov = yv[q] + fma((T)(1.0 / 6.0), dt * kv1, rv);
       }
       st32(a.out, p, ou);
       st32(a.out, nt + p, ov);
@@ -337,8 +336,8 @@ __device__ __forceinline__ void r1_stencil_point(int t, int ilast, int &e, int &
   }
 }
 
-template <int PAIR>
-__device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int rb, const int plane, double *lds) {
+template <typename T, int PAIR>
+__device__ __forceinline__ void bc2_body(const SubstepArgs<T> &a, const int rb, const int plane, T *lds) {
   constexpr int S0 = PAIR;
   constexpr int NT = NRPYELL_BLOCK, TYB = NRPYELL_PAIR_BC_ROWS;
   constexpr int NVR = TYB + 4 * NG, NC = 17, NBOX = NVR * NC, NRB = (NBOX + NT - 1) / NT;
@@ -350,33 +349,33 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   constexpr int NGE = NR1 * 24, NGER = (NGE + NT - 1) / NT;  // geometry of the R1 rows
   constexpr int NARM = TYB * 30, NARMR = (NARM + NT - 1) / NT;  // i2 arms at (ilast, tile rows)
   static_assert(TYB * 12 <= NT && TYB * 10 <= NT && TYB * 8 <= NT, "one tile-row item per thread");
-  double *bxu = lds, *bxv = lds + NRB * NT;  // [vr - (r0-10)][c - (ilast-11)]
-  double *q = lds + 2 * NRB * NT;
-  auto take = [&](int n) { double *r = q; q += n; return r; };
-  double(*sgeo)[6][4] = reinterpret_cast<double(*)[6][4]>(take(NR1 * 24));  // at each R1 row's source row
-  double(*syo)[2][5] = reinterpret_cast<double(*)[2][5]>(take(NR1 * 10));   // y at the outer points of R1
-  double(*srti)[2][5] = reinterpret_cast<double(*)[2][5]>(take(TYB * 10));  // (stages 3+4) running total there
-  double(*sarm)[10][3] = reinterpret_cast<double(*)[10][3]>(take(TYB * 30));  // stage-s u, v, y_u on i2 arms at ilast
-  double(*skl)[8] = reinterpret_cast<double(*)[8]>(take(TYB * 8));          // coefficients, source at (ilast, row)
-  double(*ru1)[12] = reinterpret_cast<double(*)[12]>(take(NR1 * 12));       // stage-(s+1) u on R1
-  double(*rv1)[12] = reinterpret_cast<double(*)[12]>(take(NR1 * 12));       // stage-(s+1) v on R1
-  double(*sk0)[2] = reinterpret_cast<double(*)[2]>(take(NR1 * 2));          // stage-s k at (ilast, R1 row)
-  double(*sA0)[2][6] = reinterpret_cast<double(*)[2][6]>(take(NR1 * 12));   // stage-s radiation terms, R1 rows
-  double(*srto)[2][5] = reinterpret_cast<double(*)[2][5]>(take(TYB * 10));  // running total at the outputs
-  double(*sk1)[2] = reinterpret_cast<double(*)[2]>(take(TYB * 2));
-  double(*sA1)[2][6] = reinterpret_cast<double(*)[2][6]>(take(TYB * 12));
-  double *s_fd = take(NRPYELL_FD_TABLE);
+  T *bxu = lds, *bxv = lds + NRB * NT;  // [vr - (r0-10)][c - (ilast-11)]
+  T *q = lds + 2 * NRB * NT;
+  auto take = [&](int n) { T *r = q; q += n; return r; };
+  T(*sgeo)[6][4] = reinterpret_cast<T(*)[6][4]>(take(NR1 * 24));  // at each R1 row's source row
+  T(*syo)[2][5] = reinterpret_cast<T(*)[2][5]>(take(NR1 * 10));   // y at the outer points of R1
+  T(*srti)[2][5] = reinterpret_cast<T(*)[2][5]>(take(TYB * 10));  // (stages 3+4) running total there
+  T(*sarm)[10][3] = reinterpret_cast<T(*)[10][3]>(take(TYB * 30));  // stage-s u, v, y_u on i2 arms at ilast
+  T(*skl)[8] = reinterpret_cast<T(*)[8]>(take(TYB * 8));          // coefficients, source at (ilast, row)
+  T(*ru1)[12] = reinterpret_cast<T(*)[12]>(take(NR1 * 12));       // stage-(s+1) u on R1
+  T(*rv1)[12] = reinterpret_cast<T(*)[12]>(take(NR1 * 12));       // stage-(s+1) v on R1
+  T(*sk0)[2] = reinterpret_cast<T(*)[2]>(take(NR1 * 2));          // stage-s k at (ilast, R1 row)
+  T(*sA0)[2][6] = reinterpret_cast<T(*)[2][6]>(take(NR1 * 12));   // stage-s radiation terms, R1 rows
+  T(*srto)[2][5] = reinterpret_cast<T(*)[2][5]>(take(TYB * 10));  // running total at the outputs
+  T(*sk1)[2] = reinterpret_cast<T(*)[2]>(take(TYB * 2));
+  T(*sA1)[2][6] = reinterpret_cast<T(*)[2][6]>(take(TYB * 12));
+  T *s_fd = take(NRPYELL_FD_TABLE);
 
   const GridDims &g = a.g;
   const int tid = threadIdx.x;
   const unsigned nt = (unsigned)g.ntot, s1s = g.G0, s2 = (unsigned)(g.G0 * g.G1);
   const int ilast = g.N0 + NG - 1, c_lo = ilast - 11;
   const int r0 = NG + rb * TYB, i2 = NG + plane;
-  const double eta = a.eta, dt = a.dt;
+  const T eta = a.eta, dt = a.dt, ix0 = (T)g.invdx0, ix1 = (T)g.invdx1;
   const int done = *a.done;
 
   // ---- loads
-  double lxu[NRB], lxv[NRB];
+  T lxu[NRB], lxv[NRB];
 #pragma unroll
   for (int r = 0; r < NRB; r++) {
     const int k = min(tid + NT * r, NBOX - 1), e = k / NC, c = k % NC;
@@ -387,7 +386,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     lxv[r] = ld32(a.in, nt + p);
   }
   // this thread's stage-s stencil points on R1 (columns ilast-6 .. ilast)
-  double iK[NIR][5], ipsi[NIR], iAAc[NIR], izm[NIR][6], izp[NIR][6], iyu[NIR], iyv[NIR];
+  T iK[NIR][5], ipsi[NIR], iAAc[NIR], izm[NIR][6], izp[NIR][6], iyu[NIR], iyv[NIR];
 #pragma unroll
   for (int it = 0; it < NIR; it++) {
     int e, col;
@@ -395,7 +394,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     int row, pl, dir;
     vrow_source(g, r0 - NG + e, i2, row, pl, dir);
     const unsigned k2 = (unsigned)row * s1s + col, p = (unsigned)pl * s2 + k2;
-    double kw;
+    T kw;
     ld2x32(a.coef, 6 * k2, iK[it][0], iK[it][1]);
     ld2x32(a.coef, 6 * k2 + 2, iK[it][2], iK[it][3]);
     ld2x32(a.coef, 6 * k2 + 4, iK[it][4], kw);
@@ -405,10 +404,10 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
       izm[it][k] = ld32(a.in, (unsigned)wrap2_near(g, pl - k) * s2 + k2);
       izp[it][k] = ld32(a.in, (unsigned)wrap2_near(g, pl + k) * s2 + k2);
     }
-    iyu[it] = PAIR == 1 ? 0.0 : ld32(a.yn, p);
-    iyv[it] = PAIR == 1 ? 0.0 : ld32(a.yn, nt + p);
+    iyu[it] = PAIR == 1 ? (T)0 : ld32(a.yn, p);
+    iyv[it] = PAIR == 1 ? (T)0 : ld32(a.yn, nt + p);
   }
-  double lgeo[NGER];
+  T lgeo[NGER];
 #pragma unroll
   for (int r = 0; r < NGER; r++) {
     const int k = min(tid + NT * r, NGE - 1), e = k / 24, pt = (k / 4) % 6, c = k % 4;
@@ -416,7 +415,7 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     vrow_source(g, r0 - NG + e, i2, row, pl, dir);
     lgeo[r] = ld32(a.geo, ((unsigned)pt * g.G1 + row) * 4 + c);
   }
-  double lyo[NOR], lrt = 0.0;
+  T lyo[NOR], lrt = 0;
 #pragma unroll
   for (int r = 0; r < NOR; r++) {
     const int k = min(tid + NT * r, NO0 - 1), e = k / 10, gf = (k % 10) / 5, pt = k % 5 + 1;
@@ -426,9 +425,9 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   }
   if (PAIR == 3) {
     const int k = min(tid, TYB * 10 - 1), tr = k / 10, gf = (k % 10) / 5, pt = k % 5 + 1;
-    lrt = ld32((const double *)a.rt, i2 * s2 + (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + (ilast + pt) + gf * nt);
+    lrt = ld32((const T *)a.rt, i2 * s2 + (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + (ilast + pt) + gf * nt);
   }
-  double larm[NARMR];
+  T larm[NARMR];
 #pragma unroll
   for (int r = 0; r < NARMR; r++) {
     const int k = min(tid + NT * r, NARM - 1), tr = k / 30, j = (k / 3) % 10, f = k % 3;
@@ -436,8 +435,8 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     const unsigned pa = (unsigned)wrap2_near(g, i2 + kk) * s2 + (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + ilast;
     larm[r] = f == 0 ? ld32(a.in, pa) : f == 1 ? ld32(a.in, nt + pa) : ld32(a.yn, pa);
   }
-  const double lfd = load_fd_table(tid);
-  double lkl = 0.0;
+  const T lfd = (T)load_fd_table(tid);
+  T lkl = 0;
   if (tid < TYB * 8) {
     const int tr = tid / 8, c = tid % 8;
     const unsigned k2 = (unsigned)min(r0 + tr, g.N1 + NG - 1) * s1s + ilast;
@@ -478,17 +477,17 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
       const int vr = r0 - NG + e;
       int row, pl, dir;
       vrow_source(g, vr, i2, row, pl, dir);
-      double xm[6], xp[6], ym[6], yp[6];
-      const double uu = BX_U(vr, col), vv = BX_V(vr, col);
+      T xm[6], xp[6], ym[6], yp[6];
+      const T uu = BX_U(vr, col), vv = BX_V(vr, col);
 #pragma unroll
       for (int k = 1; k <= 5; k++) {
         xm[k] = BX_U(vr, col - k); xp[k] = BX_U(vr, col + k);
         ym[k] = BX_U(vr - dir * k, col); yp[k] = BX_U(vr + dir * k, col);  // the source's orientation
       }
-      double ku, kv;
-      point_rhs<double>(uu, vv, ipsi[it], iAAc[it], iK[it], xm, xp, ym, yp, izm[it], izp[it], eta, ku, kv);
-      ru1[e][col - (ilast - 6)] = next_input<S0>(ku, dt, PAIR == 1 ? uu : iyu[it]);
-      rv1[e][col - (ilast - 6)] = next_input<S0>(kv, dt, PAIR == 1 ? vv : iyv[it]);
+      T ku, kv;
+      point_rhs<T>(uu, vv, ipsi[it], iAAc[it], iK[it], xm, xp, ym, yp, izm[it], izp[it], eta, ku, kv);
+      ru1[e][col - (ilast - 6)] = next_input<T, S0>(ku, dt, PAIR == 1 ? uu : iyu[it]);
+      rv1[e][col - (ilast - 6)] = next_input<T, S0>(kv, dt, PAIR == 1 ? vv : iyv[it]);
       if (col == ilast) {
         sk0[e][0] = ku;
         sk0[e][1] = kv;
@@ -505,10 +504,10 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
       vrow_source(g, vr, i2, row, pl, dir);
       const int col = ilast + pt;
       sA0[e][gf][pt] = gf == 0
-          ? radiation_A([&](int d0, int d1) { return BX_U(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_u,
-                        g.invdx0, g.invdx1, s_fd)
-          : radiation_A([&](int d0, int d1) { return BX_V(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_v,
-                        g.invdx0, g.invdx1, s_fd);
+          ? radiation_A<T>([&](int d0, int d1) { return BX_U(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_u,
+                           ix0, ix1, s_fd)
+          : radiation_A<T>([&](int d0, int d1) { return BX_V(vr + dir * d1, col + d0); }, pt, sgeo[e][pt], a.cOB_v,
+                           ix0, ix1, s_fd);
     }
   }
   __syncthreads();
@@ -518,13 +517,13 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     const int t = tid + NT * it;
     if (t < NO0) {
       const int e = t / 10, gf = (t % 10) / 5, pt = t % 5 + 1;
-      const double rhs = fma(sgeo[e][pt][3], sk0[e][gf] - sA0[e][gf][0], sA0[e][gf][pt]);
-      const double y = syo[e][gf][pt - 1];
-      (gf == 0 ? ru1 : rv1)[e][6 + pt] = next_input<S0>(rhs, dt, y);
+      const T rhs = fma(sgeo[e][pt][3], sk0[e][gf] - sA0[e][gf][0], sA0[e][gf][pt]);
+      const T y = syo[e][gf][pt - 1];
+      (gf == 0 ? ru1 : rv1)[e][6 + pt] = next_input<T, S0>(rhs, dt, y);
       const int tr = e - NG;
       if (tr >= 0 && tr < TYB)
         srto[tr][gf][pt - 1] =
-            PAIR == 1 ? (1.0 / 6.0) * (dt * rhs) : fma(1.0 / 3.0, dt * rhs, srti[tr][gf][pt - 1]);
+            PAIR == 1 ? (T)(1.0 / 6.0) * (dt * rhs) : fma((T)(1.0 / 3.0), dt * rhs, srti[tr][gf][pt - 1]);
     }
   }
   __syncthreads();
@@ -532,18 +531,18 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   // ---- stage s+1 at (ilast, tile rows): the RHS from the stage-(s+1) u on R1 and on the i2 arm
   if (tid < TYB && r0 + tid < g.N1 + NG) {
     const int e = tid + NG;
-    double xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
+    T xm[6], xp[6], ym[6], yp[6], zm[6], zp[6];
 #pragma unroll
     for (int k = 1; k <= 5; k++) {
-      const double *m = sarm[tid][k - 1], *pp = sarm[tid][4 + k];
-      zm[k] = next_input<S0>(fma(-eta, m[0], m[1]), dt, PAIR == 1 ? m[0] : m[2]);
-      zp[k] = next_input<S0>(fma(-eta, pp[0], pp[1]), dt, PAIR == 1 ? pp[0] : pp[2]);
+      const T *m = sarm[tid][k - 1], *pp = sarm[tid][4 + k];
+      zm[k] = next_input<T, S0>(fma(-eta, m[0], m[1]), dt, PAIR == 1 ? m[0] : m[2]);
+      zp[k] = next_input<T, S0>(fma(-eta, pp[0], pp[1]), dt, PAIR == 1 ? pp[0] : pp[2]);
       xm[k] = ru1[e][6 - k]; xp[k] = ru1[e][6 + k];
       ym[k] = ru1[e - k][6]; yp[k] = ru1[e + k][6];
     }
-    const double K[5] = {skl[tid][0], skl[tid][1], skl[tid][2], skl[tid][3], skl[tid][4]};
-    double ku, kv;
-    point_rhs<double>(ru1[e][6], rv1[e][6], skl[tid][6], skl[tid][7], K, xm, xp, ym, yp, zm, zp, eta, ku, kv);
+    const T K[5] = {skl[tid][0], skl[tid][1], skl[tid][2], skl[tid][3], skl[tid][4]};
+    T ku, kv;
+    point_rhs<T>(ru1[e][6], rv1[e][6], skl[tid][6], skl[tid][7], K, xm, xp, ym, yp, zm, zp, eta, ku, kv);
     sk1[tid][0] = ku;
     sk1[tid][1] = kv;
   }
@@ -551,9 +550,9 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
   if (tid < TYB * 12) {
     const int tr = tid / 12, gf = (tid % 12) / 6, pt = tid % 6, e = tr + NG;
     if (r0 + tr < g.N1 + NG) {
-      const double(*R)[12] = gf == 0 ? ru1 : rv1;
-      sA1[tr][gf][pt] = radiation_A([&](int d0, int d1) { return R[e + d1][6 + pt + d0]; }, pt, sgeo[e][pt],
-                                    gf == 0 ? a.cOB_u : a.cOB_v, g.invdx0, g.invdx1, s_fd);
+      const T(*R)[12] = gf == 0 ? ru1 : rv1;
+      sA1[tr][gf][pt] = radiation_A<T>([&](int d0, int d1) { return R[e + d1][6 + pt + d0]; }, pt, sgeo[e][pt],
+                                       gf == 0 ? a.cOB_u : a.cOB_v, ix0, ix1, s_fd);
     }
   }
   __syncthreads();
@@ -562,14 +561,14 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
     const int tr = tid / 10, gf = (tid % 10) / 5, pt = tid % 5 + 1, row = r0 + tr, e = tr + NG;
     if (row < g.N1 + NG) {
       const unsigned p = (unsigned)i2 * s2 + (unsigned)row * s1s + (ilast + pt);
-      const double rhs = fma(sgeo[e][pt][3], sk1[tr][gf] - sA1[tr][gf][0], sA1[tr][gf][pt]);
-      const double rt = srto[tr][gf][pt - 1], y = syo[e][gf][pt - 1];
-      double o;
+      const T rhs = fma(sgeo[e][pt][3], sk1[tr][gf] - sA1[tr][gf][0], sA1[tr][gf][pt]);
+      const T rt = srto[tr][gf][pt - 1], y = syo[e][gf][pt - 1];
+      T o;
       if (PAIR == 1) {
-        st32(a.rt, gf * nt + p, fma(1.0 / 3.0, dt * rhs, rt));
-        o = fma(0.5, dt * rhs, y);
+        st32(a.rt, gf * nt + p, fma((T)(1.0 / 3.0), dt * rhs, rt));
+        o = fma((T)0.5, dt * rhs, y);
       } else {
-        o = y + fma(1.0 / 6.0, dt * rhs, rt);
+        o = y + fma((T)(1.0 / 6.0), dt * rhs, rt);
       }
       st32(a.out, gf * nt + p, o);
       for_each_image_in_planes(g, ilast + pt, row, i2, [&](long d) { st32(a.out, gf * nt + (unsigned)d, o); });
@@ -579,39 +578,43 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<double> &a, const int
 
 // One configuration: the nbc_rows*N2 BC blocks first, then the interior blocks in XCD-aware
 // order.
-template <int PAIR, int TX, int PL>
-__global__ void __launch_bounds__(NRPYELL_BLOCK, 3) substep2_kernel(SubstepArgs<double> a) {
+template <typename T, int PAIR, int TX, int PL>
+__global__ void __launch_bounds__(NRPYELL_BLOCK, 3) substep2_kernel(SubstepArgs<T> a) {
   constexpr int NL = interior2_lds<TX, PL>() > bc2_lds() ? interior2_lds<TX, PL>() : bc2_lds();
-  __shared__ double lds[NL];
+  __shared__ T lds[NL];
+  __shared__ double s_red[NRPYELL_BLOCK / 64];
   const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
   if (L < nb) {
     const unsigned pln = div_by(L, a.nbc_mul);
-    bc2_body<PAIR>(a, (int)(L - pln * (unsigned)a.nbc_rows), (int)pln, lds);
+    bc2_body<T, PAIR>(a, (int)(L - pln * (unsigned)a.nbc_rows), (int)pln, lds);
     return;
   }
   const unsigned ni = gridDim.x - nb, q = xcd_logical(L - nb, ni), ng = (unsigned)(a.g.N2 / PL);
   const unsigned tile = div_by(q, a.ng_mul);
-  interior2_body<PAIR, TX, PL>(a, (int)tile, (int)(q - tile * ng), q, ni, lds);
+  interior2_body<T, PAIR, TX, PL>(a, (int)tile, (int)(q - tile * ng), q, ni, lds, s_red);
 }
 
-template <int PAIR>
-static void launch_pair(const SubstepArgs<double> &a, hipStream_t st) {
+template <typename T, int PAIR>
+static void launch_pair(const SubstepArgs<T> &a, hipStream_t st) {
   dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(NRPYELL_BLOCK);
   if (a.pl == 2) {
     switch (a.tx) {
-      case 64: hipLaunchKernelGGL((substep2_kernel<PAIR, 64, 2>), grd, blk, 0, st, a); break;
-      default: hipLaunchKernelGGL((substep2_kernel<PAIR, 32, 2>), grd, blk, 0, st, a); break;
+      case 64: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 64, 2>), grd, blk, 0, st, a); break;
+      default: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 32, 2>), grd, blk, 0, st, a); break;
     }
     return;
   }
   switch (a.tx) {
-    case 16: hipLaunchKernelGGL((substep2_kernel<PAIR, 16, 1>), grd, blk, 0, st, a); break;
-    case 64: hipLaunchKernelGGL((substep2_kernel<PAIR, 64, 1>), grd, blk, 0, st, a); break;
-    default: hipLaunchKernelGGL((substep2_kernel<PAIR, 32, 1>), grd, blk, 0, st, a); break;
+    case 16: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 16, 1>), grd, blk, 0, st, a); break;
+    case 64: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 64, 1>), grd, blk, 0, st, a); break;
+    default: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 32, 1>), grd, blk, 0, st, a); break;
   }
 }
 
-void launch_substep_pair(const SubstepArgs<double> &a, int first_stage, hipStream_t st) {
-  if (first_stage == 1) launch_pair<1>(a, st);
-  else launch_pair<3>(a, st);
+template <typename T>
+void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st) {
+  if (first_stage == 1) launch_pair<T, 1>(a, st);
+  else launch_pair<T, 3>(a, st);
 }
+template void launch_substep_pair<double>(const SubstepArgs<double> &, int, hipStream_t);
+template void launch_substep_pair<float>(const SubstepArgs<float> &, int, hipStream_t);

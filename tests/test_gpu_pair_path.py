@@ -79,3 +79,29 @@ def test_pair_path_option_checks():
             Q.solver(substeps_per_launch=2, outer_bc="sommerfeld6")
     finally:
         Q.close()
+
+
+@pytest.mark.parametrize("check_every,steps_per_graph", [(10, 50), (5, 25), (3, 9)])
+def test_pair_path_mixed_precision_matches_single(check_every, steps_per_graph):
+    """The FP32 bulk phase runs the same two-substep kernel in float: the mixed-precision solve
+    takes the same FP32 and FP64 steps with the same history and the same final u as with one
+    substep per launch. Odd check intervals end the FP32 phase on odd step counts, i.e. with the
+    FP32 state in the pair path's second array."""
+    P = runtime.Problem(config.axisymmetric_spinning(Nxx=(64, 64, 8)))
+    try:
+        out = []
+        for spl in (1, 2):
+            P.zero_guess()
+            S = P.solver(precision=32, log10_target=-9.0, max_steps=8000, substeps_per_launch=spl,
+                         check_every=check_every, steps_per_graph=steps_per_graph)
+            r = S.solve()
+            out.append((r.steps, r.fp32_steps, r.log10_l2, P.get("y_n_gfs"), S.history()))
+            S.close()
+        (s1, f1, l1, y1, h1), (s2, f2, l2, y2, h2) = out
+        print(f"check_every {check_every}: {s1} steps, {f1} in FP32, log10 L2 {l1:.3f}", flush=True)
+        assert f1 > 0 and s1 > f1
+        assert (s1, f1, l1) == (s2, f2, l2)
+        assert np.array_equal(y1, y2)
+        assert np.array_equal(h1[0], h2[0]) and np.array_equal(h1[1], h2[1])
+    finally:
+        P.close()
