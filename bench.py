@@ -220,16 +220,17 @@ def nn_seeded_paths(P, n_held_out=256):
     training set). Per path: steps and seconds to log10 L2 <= -10 (seeding included), max |u - u*|
     against the reference's own u* (tests/golden/deep_equal_mass_boosted_128x128x16.npz: the
     reference's C relaxed to log10 L2 = -14), and the accuracy-matched time: the seconds of the
-    first residual target in -10, -10.5, ... whose u is within 1e-10 of u*. A fourth path keeps
-    the tuned damping only down to -9 and finishes at the configured one
-    (learned.solve_damping_schedule): the strong damping leaves long-wavelength error the masked
-    residual hardly sees.
+    first residual target in -10, -10.5, ... whose u is within 1e-10 of u*. A fourth path runs
+    the damping schedule autotune.ScheduleTuner picks on the same seed-901 configuration for that
+    accuracy bar (learned.solve_damping_schedule: a strong damping down to a switch residual,
+    then a weaker one; the strong damping leaves long-wavelength error the masked residual hardly
+    sees).
 
     Held-out (seed 900, n configurations at 128x128x16): learned.evaluate_batched, both starts
     on the batched FP64 solver, the network's at the damping EtaTuner gives it in FP64."""
     import numpy as np
     from nrpyelliptic_amd import config, learned, runtime
-    from nrpyelliptic_amd.autotune import EtaTuner
+    from nrpyelliptic_amd.autotune import EtaTuner, ScheduleTuner
     from nrpyelliptic_amd.mlp import InitialGuessMLP
     w = os.path.join(ROOT, "nrpyelliptic_amd", "data", "mlp_initial_guess.npz")
     if not os.path.exists(w):
@@ -245,16 +246,21 @@ def nn_seeded_paths(P, n_held_out=256):
     f_nn32 = EtaTuner(cache_path=None, precision=32).choose(calib, nn_start, "nn")
     f_nn64 = EtaTuner(cache_path=None, precision=64).choose(calib, nn_start, "nn")
     f_zero = EtaTuner(cache_path=None, precision=64).choose(calib, lambda Q: Q.zero_guess(), "zero")
-    calib.close()
     tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sched = ScheduleTuner(cache_path=None, precision=32).choose(calib, nn_start, "nn")
+    sched_s = time.perf_counter() - t0
+    calib.close()
 
     eta0 = P.gd.params.eta_damping
 
     def run(nn, factor, target):
         if nn == "schedule":
-            if target > -9.5:
-                return None  # the switch to the configured damping happens at -9
-            steps, fp32, sec, seed_s, conv, used = learned.solve_damping_schedule(net, P, factor, -9.0, target)
+            f_hi, switch, f_lo = factor
+            if target > switch - 0.5:
+                return None  # the switch to the weaker damping comes first
+            steps, fp32, sec, seed_s, conv, used = learned.solve_damping_schedule(net, P, f_hi, switch, target,
+                                                                                    f_lo=f_lo)
             u = P.get("y_n_gfs")[: P.ntot].reshape(G2, G1, G0)[5:-5, 5:-5, 5:-5]
             return {"steps": steps, "fp32_steps": fp32, "seconds": sec + seed_s, "seed_seconds": seed_s,
                     "converged": conv, "nn_used": used, "max_abs_u_minus_ustar": float(np.max(np.abs(u - u_star)))}
@@ -278,9 +284,9 @@ def nn_seeded_paths(P, n_held_out=256):
                 "log10_l2": r.log10_l2, "converged": bool(r.converged), "nn_used": bool(used),
                 "max_abs_u_minus_ustar": float(np.max(np.abs(u - u_star)))}
 
-    # the damping schedule: tuned eta down to -9, then the configured eta (learned.solve_damping_schedule)
+    # the damping schedule ScheduleTuner chose (learned.solve_damping_schedule)
     paths = {"plain_fp64": (False, 1.0), "nn_mixed_configured_eta": (True, 1.0),
-             "nn_mixed_tuned_eta": (True, f_nn32), "nn_mixed_damping_schedule": ("schedule", f_nn32)}
+             "nn_mixed_tuned_eta": (True, f_nn32), "nn_mixed_damping_schedule": ("schedule", sched["schedule"])}
     bench = {}
     for name, (nn, f) in paths.items():
         row = run(nn, f, -10.0) or {}
@@ -302,6 +308,13 @@ def nn_seeded_paths(P, n_held_out=256):
         net, config.random_batch(n_held_out, seed=900, grids=False, Nxx=(128, 128, 16)), f_nn64, f_zero,
         log=lambda s: None)
     return {"eta_factors": {"nn_fp32_fp64": f_nn32, "nn_fp64": f_nn64, "zero": f_zero},
+            "damping_schedule": {"eta_factor_before_switch": sched["schedule"][0],
+                                 "switch_log10": sched["schedule"][1],
+                                 "eta_factor_after_switch": sched["schedule"][2],
+                                 "calibration": "ScheduleTuner on the seed-901 configuration: fewest steps until "
+                                                "max|u - u*| <= 1e-10 (u* its zero-guess solve to -14)",
+                                 "calibration_target_log10": sched["target"], "calibration_steps": sched["steps"],
+                                 "calibration_seconds": sched_s},
             "calibration": "EtaTuner on a seed-901 configuration of the 128x128x16 grid: fewest steps to -10 among "
                            "the eta factors whose u is within half the reference path's error of a solve to -14, "
                            "in the precision the path runs",

@@ -1,5 +1,5 @@
-"""Autotuners of the relaxation: the fused kernel's tile shape (TileTuner) and the damping
-strength eta (EtaTuner).
+"""Autotuners of the relaxation: the fused kernel's tile shape (TileTuner), the damping
+strength eta (EtaTuner) and, for an absolute accuracy bar, the damping schedule (ScheduleTuner).
 
 Tile width of the fused RK4 substep kernel:
 
@@ -198,6 +198,134 @@ class EtaTuner:
                     json.dump(self.table, fh, indent=1, sort_keys=True)
                 os.replace(tmp, self.cache_path)
         return float(self.table[k]["factor"])
+
+
+# (f_hi, switch log10 L2, f_lo): multiples of the configured eta before and after the switch;
+# f_hi == f_lo is one damping throughout
+SCHEDULES = ((1.0, -9.0, 1.0), (1.4, -9.0, 1.4), (2.0, -9.0, 2.0), (2.8, -9.0, 2.8),
+             (2.8, -9.0, 1.0), (2.8, -9.0, 1.4), (2.8, -9.0, 2.0), (2.0, -9.0, 1.4), (4.0, -9.0, 1.4),
+             (4.0, -8.0, 2.0))
+ACCURACY_TARGETS = tuple(-10.0 - 0.5 * k for k in range(10))  # -10 .. -14.5
+
+
+def _schedule_key(prob: Problem, start: str, tol: float, precision: int) -> str:
+    p = prob.gd.params
+    return f"{p.Nxx0}x{p.Nxx1}x{p.Nxx2}_A{p.AMAX:g}_w{p.SINHWAA:g}_{start}_tol{tol:g}_p{precision}"
+
+
+class ScheduleTuner:
+    """Chooses the damping schedule of a start for accuracy: the fewest RK4 steps until
+    max |u - u*| <= tol, with u* the converged solution.
+
+    EtaTuner's constant factor minimises the steps to the residual target at no loss of accuracy
+    against the reference path at that target. When the bar is instead an absolute accuracy (u
+    within tol of the converged u), the residual target has to go deeper, and there a weaker
+    damping late in the relaxation pays: the strong damping that removes the guess's grid-scale
+    error fast relaxes the long-wavelength modes (rate c^2 lambda / eta once overdamped) slowly.
+
+    `choose(prob, start, name)` calibrates on `prob` (a configuration of the grid, not the one to
+    be measured): u* = a solve from the zero guess at the configured eta to log10 L2 = -14; then
+    for each schedule (f_hi, switch, f_lo) of `schedules` the relaxation from `start(prob)` runs at
+    f_hi x the configured eta (in `precision`, 32 for the FP32 -> FP64 path) down to the switch,
+    continues at f_lo x it (Problem.set_eta_damping keeps u and u_t) and is continued through the
+    residual targets -10, -10.5, ... (the state (u, v) carries the trajectory, so each
+    continuation is the single longer solve) until the first whose u is within tol of u*. The
+    schedule with the fewest steps wins (ties: the earlier in `schedules`); the result --
+    schedule, residual target and steps -- is cached per (grid, start, tol, precision), and the
+    problem's eta and state are restored. `cache_path=None` keeps the table in memory only."""
+
+    def __init__(self, cache_path: Optional[str] = None, schedules=SCHEDULES, tol: float = 1e-10,
+                 targets: Sequence[float] = ACCURACY_TARGETS, max_steps: int = 40000, precision: int = 32):
+        self.cache_path = cache_path
+        self.schedules = tuple(tuple(float(x) for x in s) for s in schedules)
+        self.tol = tol
+        self.targets = tuple(targets)
+        self.max_steps = max_steps
+        self.precision = precision
+        self.measured = 0
+        self.table: Dict[str, dict] = {}
+        if cache_path and os.path.exists(cache_path):
+            with open(cache_path) as f:
+                self.table = json.load(f)
+
+    def _relax(self, prob: Problem, target: float, precision: int):
+        s = Solver(prob, log10_target=target, max_steps=self.max_steps, precision=precision)
+        try:
+            r = s.solve()
+        except NrpyellError:
+            return None  # NaN
+        finally:
+            s.close()
+        return r if r.converged else None
+
+    def trajectory(self, prob: Problem, start: Callable[[Problem], object], schedule, u_star: np.ndarray,
+                   precision: Optional[int] = None):
+        """(first residual target whose u is within tol of u*, steps to it), or None; the first
+        solve in `precision` (the tuner's by default), the continuations in FP64."""
+        f_hi, switch, f_lo = schedule
+        eta0 = prob.gd.params.eta_damping
+        m = _interior(prob)
+        try:
+            prob.gd.params.eta_damping = f_hi * eta0
+            start(prob)  # after eta: a seed writes v = eta u
+            steps, precision = 0, self.precision if precision is None else precision
+            if f_lo != f_hi:
+                r = self._relax(prob, switch, precision)
+                if r is None:
+                    return None
+                steps, precision = steps + int(r.steps), 64
+                prob.set_eta_damping(f_lo * eta0)
+            for t in self.targets:
+                r = self._relax(prob, t, precision)
+                if r is None:
+                    return None
+                steps, precision = steps + int(r.steps), 64
+                err = float(np.max(np.abs(prob.get("y_n_gfs")[: prob.ntot][m] - u_star)))
+                if err <= self.tol:
+                    return t, steps
+            return None
+        finally:
+            prob.gd.params.eta_damping = eta0
+
+    def measure(self, prob: Problem, start: Callable[[Problem], object]) -> dict:
+        eta0 = prob.gd.params.eta_damping
+        saved = prob.get("y_n_gfs")
+        try:
+            prob.zero_guess()
+            star = self._relax(prob, -14.0, 64)
+            if star is None:
+                return {}
+            u_star = prob.get("y_n_gfs")[: prob.ntot][_interior(prob)]
+            out = {}
+            for sch in self.schedules:
+                got = self.trajectory(prob, start, sch, u_star)
+                if got is not None:
+                    out[sch] = got
+        finally:
+            prob.gd.params.eta_damping = eta0
+            prob.set("y_n_gfs", saved)
+        self.measured += 1
+        return out
+
+    def choose(self, prob: Problem, start: Callable[[Problem], object], start_name: str) -> dict:
+        """{"schedule": (f_hi, switch, f_lo), "target": residual target, "steps": steps} on the
+        calibration problem; the configured eta throughout (1, -9, 1) when nothing qualifies."""
+        k = _schedule_key(prob, start_name, self.tol, self.precision)
+        if k not in self.table:
+            m = self.measure(prob, start)
+            entry = {"schedule": [1.0, -9.0, 1.0], "target": None, "steps": None}
+            if m:
+                best = min(m, key=lambda sch: (m[sch][1], self.schedules.index(sch)))
+                entry = {"schedule": list(best), "target": m[best][0], "steps": m[best][1]}
+            entry["measured"] = {f"{a:g}/{b:g}/{c:g}": {"target": v[0], "steps": v[1]} for (a, b, c), v in m.items()}
+            self.table[k] = entry
+            if self.cache_path:
+                tmp = self.cache_path + ".tmp"
+                with open(tmp, "w") as fh:
+                    json.dump(self.table, fh, indent=1, sort_keys=True)
+                os.replace(tmp, self.cache_path)
+        e = self.table[k]
+        return {"schedule": tuple(e["schedule"]), "target": e["target"], "steps": e["steps"]}
 
 
 def _interior(prob: Problem) -> np.ndarray:

@@ -698,10 +698,18 @@ struct nrpyell_mlp {
   bf16_t *dZt[MLP_HIDDEN + 1] = {};  // dZ_l^T [256 x M], l = 1..5
   float *dy = nullptr;               // per-row output gradient
   double *loss = nullptr;
+  // seeding workspace (features, per-row scale and point index, network output), grown on demand
+  // and kept between seeds so a seed allocates nothing after the first one on a grid this large
+  long seed_cap = 0;
+  bf16_t *seedX = nullptr;
+  double *seed_scale = nullptr;
+  long *seed_idx = nullptr;
+  float *seed_y = nullptr;
   hipStream_t st = nullptr;
   ~nrpyell_mlp() {
     (void)hipFree(p); (void)hipFree(g); (void)hipFree(m); (void)hipFree(v); (void)hipFree(part);
     (void)hipFree(pw); (void)hipFree(pws); (void)hipFree(loss);
+    release_seed();
     release();
     if (st) (void)hipStreamDestroy(st);
   }
@@ -712,6 +720,20 @@ struct nrpyell_mlp {
     (void)hipFree(dy);
     dy = nullptr;
     cap = 0;
+  }
+  void release_seed() {
+    (void)hipFree(seedX); (void)hipFree(seed_scale); (void)hipFree(seed_idx); (void)hipFree(seed_y);
+    seedX = nullptr; seed_scale = nullptr; seed_idx = nullptr; seed_y = nullptr;
+    seed_cap = 0;
+  }
+  void ensure_seed(long rows) {
+    if (rows <= seed_cap) return;
+    release_seed();  // a failed allocation below leaves seed_cap 0: the next seed allocates again
+    seedX = mlp_alloc<bf16_t>((size_t)rows * MLP_K0);
+    seed_scale = mlp_alloc<double>((size_t)rows);
+    seed_idx = mlp_alloc<long>((size_t)rows);
+    seed_y = mlp_alloc<float>((size_t)rows);
+    seed_cap = rows;
   }
   void ensure(int M) {
     if (M <= cap) return;
@@ -1053,14 +1075,11 @@ extern "C" int nrpyell_mlp_seed(nrpyell_mlp *s, griddata_struct *gd, int smooth_
     }
     const GridDims g = dims_from_params(&gd->params);
     const long n = (long)(g.G0 - NRPYELL_NG) * g.N1 * g.N2, rows = (n + 63) / 64 * 64;
-    DevBuf<bf16_t> Xb(rows * MLP_K0);
-    DevBuf<double> scaleb(rows);
-    DevBuf<long> idxb(rows);
-    DevBuf<float> yb(rows);
-    bf16_t *X = Xb.p;
-    double *scale = scaleb.p;
-    long *idx = idxb.p;
-    float *y = yb.p;
+    s->ensure_seed(rows);
+    bf16_t *X = s->seedX;
+    double *scale = s->seed_scale;
+    long *idx = s->seed_idx;
+    float *y = s->seed_y;
     int rc = nrpyell_mlp_features(gd, X, scale, idx);
     if (rc != NRPYELL_OK) return rc;
     const int chunk = 1 << 18;  // rows per forward pass (multiple of 64)

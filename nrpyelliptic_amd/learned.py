@@ -106,13 +106,13 @@ def evaluate_batched(net: InitialGuessMLP, configs, f_nn: float, f_zero: float =
 
 def seed_with_fallback(net: InitialGuessMLP, P: runtime.Problem, smooth_passes: int = 2):
     """Zero-guess residual vs NN-guess residual; keep the NN guess only if its residual is lower."""
-    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    P.zero_guess()
     r0 = P.log10_residual()
     net.seed(P, smooth_passes)
     r1 = P.log10_residual()
     used = r1 < r0
     if not used:
-        P.set("y_n_gfs", np.zeros(2 * P.ntot))
+        P.zero_guess()
     return used, r0, r1
 
 
@@ -132,13 +132,14 @@ def seed_with_tuned_damping(net: InitialGuessMLP, P: runtime.Problem, f_nn: floa
 
 def solve_damping_schedule(net: InitialGuessMLP, P: runtime.Problem, f_hi: float, switch_log10: float = -9.0,
                            target: float = -12.0, precision: int = 32, smooth_passes: int = 2,
-                           max_steps: int = 40000):
-    """The network's start relaxed in two phases: at f_hi x the configured eta (the damping tuned
-    to that start: fast on the grid-scale error the guess leaves) down to switch_log10, then at the
-    configured eta (which relaxes the long-wavelength error a strong damping leaves behind, the
-    error the masked residual hardly sees) down to target, with u and u_t = v - eta u carried
-    across the switch (Problem.set_eta_damping). Returns (steps, fp32 steps, solver seconds, seed
-    seconds, converged, network guess used); P's eta is the configured one afterwards."""
+                           max_steps: int = 40000, f_lo: float = 1.0):
+    """The network's start relaxed in two phases: at f_hi x the configured eta (a strong damping:
+    fast on the grid-scale error the guess leaves) down to switch_log10, then at f_lo x the
+    configured eta (a weaker one, which relaxes the long-wavelength error a strong damping leaves
+    behind, the error the masked residual hardly sees) down to target, with u and
+    u_t = v - eta u carried across the switch (Problem.set_eta_damping). autotune.ScheduleTuner
+    picks (f_hi, switch_log10, f_lo) for accuracy. Returns (steps, fp32 steps, solver seconds,
+    seed seconds, converged, network guess used); P's eta is the configured one afterwards."""
     eta0 = P.gd.params.eta_damping
     t0 = time.perf_counter()
     used, _ = seed_with_tuned_damping(net, P, f_hi, 1.0, smooth_passes)
@@ -147,7 +148,7 @@ def solve_damping_schedule(net: InitialGuessMLP, P: runtime.Problem, f_hi: float
         S = P.solver(precision=precision, log10_target=switch_log10, max_steps=max_steps)
         r1 = S.solve()
         S.close()
-        P.set_eta_damping(eta0)
+        P.set_eta_damping(f_lo * eta0)
         S = P.solver(log10_target=target, max_steps=max_steps)
         r2 = S.solve()
         S.close()

@@ -63,3 +63,51 @@ def test_cached_per_grid_start_and_target(tmp_path):
     assert len(table) == 3 and all("factor" in v and "reference_error" in v for v in table.values())
     t3 = autotune.EtaTuner(cache_path=path)  # a new tuner reads the table back
     assert t3.choose(P, None, "nn") == 2.0 and t3.measured == 0
+
+
+# ---- ScheduleTuner: fewest steps until u is within tol of u* ----
+
+def _stuner(result, **kw):
+    t = autotune.ScheduleTuner(cache_path=None, **kw)
+    calls = []
+
+    def measure(prob, start):
+        calls.append(prob)
+        return result
+    t.measure = measure
+    return t, calls
+
+
+def test_schedule_with_fewest_steps_to_the_accuracy_bar():
+    res = {(1.0, -9.0, 1.0): (-12.5, 4887), (2.8, -9.0, 2.8): (-14.0, 2855),
+           (2.8, -9.0, 1.4): (-11.5, 2005), (2.8, -9.0, 1.0): (-12.0, 2776)}
+    t, _ = _stuner(res)
+    got = t.choose(_prob(), None, "nn")
+    assert got == {"schedule": (2.8, -9.0, 1.4), "target": -11.5, "steps": 2005}
+
+
+def test_schedule_ties_go_to_the_earlier_candidate_and_failure_keeps_the_configured_eta():
+    a, b = autotune.SCHEDULES[1], autotune.SCHEDULES[5]
+    t, _ = _stuner({b: (-12.0, 100), a: (-12.5, 100)})
+    assert t.choose(_prob(), None, "x")["schedule"] == a
+    t2, _ = _stuner({})
+    assert t2.choose(_prob(), None, "x") == {"schedule": (1.0, -9.0, 1.0), "target": None, "steps": None}
+
+
+def test_schedule_cached_per_grid_start_tolerance_and_precision(tmp_path):
+    res = {(2.0, -9.0, 2.0): (-12.5, 2292)}
+    path = str(tmp_path / "sched.json")
+    t, calls = _stuner(res)
+    t.cache_path = path
+    P = _prob()
+    assert t.choose(P, None, "nn")["steps"] == 2292 and t.choose(P, None, "nn")["steps"] == 2292
+    assert len(calls) == 1
+    t.choose(P, None, "zero")
+    t.tol = 1e-9
+    t.choose(P, None, "nn")
+    assert len(calls) == 3
+    table = json.load(open(path))
+    assert len(table) == 3 and all(v["schedule"] == [2.0, -9.0, 2.0] for v in table.values())
+    t2 = autotune.ScheduleTuner(cache_path=path)
+    t2.measure = lambda prob, start: (_ for _ in ()).throw(AssertionError("re-measured"))
+    assert t2.choose(P, None, "nn")["schedule"] == (2.0, -9.0, 2.0)
