@@ -243,7 +243,9 @@ def nn_seeded_paths(P, n_held_out=256):
     t0 = time.perf_counter()
     calib = runtime.Problem(config.random_batch(1, seed=901, grids=False, Nxx=(128, 128, 16))[0])
     nn_start = lambda Q: learned.seed_with_fallback(net, Q)  # noqa: E731
-    f_nn32 = EtaTuner(cache_path=None, precision=32).choose(calib, nn_start, "nn")
+    tuner32 = EtaTuner(cache_path=None, precision=32, fp32_switches=(-6.0, -7.0, -8.0))
+    f_nn32 = tuner32.choose(calib, nn_start, "nn")
+    sw_nn32 = tuner32.fp32_switch_for(calib, "nn")
     f_nn64 = EtaTuner(cache_path=None, precision=64).choose(calib, nn_start, "nn")
     f_zero = EtaTuner(cache_path=None, precision=64).choose(calib, lambda Q: Q.zero_guess(), "zero")
     tune_s = time.perf_counter() - t0
@@ -274,7 +276,8 @@ def nn_seeded_paths(P, n_held_out=256):
                 used = False
                 P.gd.params.eta_damping = factor * eta0
                 P.zero_guess()
-            S = P.solver(precision=32 if nn else 64, log10_target=target, max_steps=40000)
+            kw = {"fp32_switch_log10": sw_nn32} if nn and factor != 1.0 else {}
+            S = P.solver(precision=32 if nn else 64, log10_target=target, max_steps=40000, **kw)
             r = S.solve()
             S.close()
         finally:
@@ -308,6 +311,7 @@ def nn_seeded_paths(P, n_held_out=256):
         net, config.random_batch(n_held_out, seed=900, grids=False, Nxx=(128, 128, 16)), f_nn64, f_zero,
         log=lambda s: None)
     return {"eta_factors": {"nn_fp32_fp64": f_nn32, "nn_fp64": f_nn64, "zero": f_zero},
+            "fp32_switch_log10_tuned_eta": sw_nn32,
             "damping_schedule": {"eta_factor_before_switch": sched["schedule"][0],
                                  "switch_log10": sched["schedule"][1],
                                  "eta_factor_after_switch": sched["schedule"][2],
@@ -317,7 +321,8 @@ def nn_seeded_paths(P, n_held_out=256):
                                  "calibration_seconds": sched_s},
             "calibration": "EtaTuner on a seed-901 configuration of the 128x128x16 grid: fewest steps to -10 among "
                            "the eta factors whose u is within half the reference path's error of a solve to -14, "
-                           "in the precision the path runs",
+                           "in the precision the path runs; for the FP32 -> FP64 path also the deepest FP32 -> FP64 "
+                           "switch (log10 L2 -6, -7, -8) with no more steps within the same margin",
             "calibration_seconds": tune_s, "bench_problem": bench, "held_out_batched": held}
 
 

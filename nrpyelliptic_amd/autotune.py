@@ -123,10 +123,18 @@ class EtaTuner:
     whose error is at most `margin` x e_ref is cached per (grid, start, target, precision), 1.0 if
     none qualifies; configurations of the grid with other punctures
     and foci reuse it. The problem's eta and state are restored afterwards. `cache_path=None`
-    keeps the table in memory only."""
+    keeps the table in memory only.
+
+    With precision 32 and more than one level in `fp32_switches`, the chosen factor's solve is
+    repeated with the FP32 -> FP64 switch at each level (log10 L2; the first level is the one the
+    factors were measured at): the deepest level whose solve takes no more steps and stays within
+    the same accuracy margin is cached too (fp32_switch_for), since an FP32 step costs less than an
+    FP64 one and the FP32 state's rounding is what a deeper switch risks."""
 
     def __init__(self, cache_path: Optional[str] = None, factors: Sequence[float] = ETA_FACTORS,
-                 target: float = -10.0, max_steps: int = 40000, margin: float = 0.5, precision: int = 64):
+                 target: float = -10.0, max_steps: int = 40000, margin: float = 0.5, precision: int = 64,
+                 fp32_switches: Sequence[float] = (-6.0,)):
+        self.fp32_switches = tuple(float(x) for x in fp32_switches)
         self.precision = precision  # of the solves from `start` (the path the factor will serve)
         self.cache_path = cache_path
         self.factors = tuple(factors)
@@ -140,11 +148,12 @@ class EtaTuner:
                 self.table = json.load(f)
 
     def _solve(self, prob: Problem, eta: float, start: Callable[[Problem], object], target: float,
-               precision: int = 64):
+               precision: int = 64, fp32_switch: Optional[float] = None):
         """(steps, interior u) of one solve, or None if it does not converge."""
         prob.gd.params.eta_damping = eta
         start(prob)  # after eta: a seed writes v = eta u
-        s = Solver(prob, log10_target=target, max_steps=self.max_steps, precision=precision)
+        kw = {} if fp32_switch is None else {"fp32_switch_log10": fp32_switch}
+        s = Solver(prob, log10_target=target, max_steps=self.max_steps, precision=precision, **kw)
         try:
             r = s.solve()
         except NrpyellError:
@@ -168,15 +177,28 @@ class EtaTuner:
             u_star = star[1]
             out = {"reference_steps": ref[0], "reference_error": float(np.max(np.abs(ref[1] - u_star))),
                    "factors": {}}
+            sw0 = self.fp32_switches[0] if self.precision == 32 else None
             for f in self.factors:
-                got = self._solve(prob, f * eta0, start, self.target, self.precision)
+                got = self._solve(prob, f * eta0, start, self.target, self.precision, sw0)
                 if got is not None:
                     out["factors"][f] = (got[0], float(np.max(np.abs(got[1] - u_star))))
+            if self.precision == 32 and len(self.fp32_switches) > 1:
+                ok = self._qualifying(out)
+                if ok:
+                    best = min(ok, key=lambda f: (ok[f], f))
+                    out["switches"] = {sw0: out["factors"][best]}
+                    for sw in self.fp32_switches[1:]:
+                        got = self._solve(prob, best * eta0, start, self.target, self.precision, sw)
+                        if got is not None:
+                            out["switches"][sw] = (got[0], float(np.max(np.abs(got[1] - u_star))))
         finally:
             prob.gd.params.eta_damping = eta0
             prob.set("y_n_gfs", saved)
         self.measured += 1
         return out
+
+    def _qualifying(self, m: dict) -> dict:
+        return {f: v[0] for f, v in m["factors"].items() if v[1] <= self.margin * m["reference_error"]}
 
     def choose(self, prob: Problem, start: Callable[[Problem], object], start_name: str) -> float:
         """Factor of the configured eta: fewest steps at no loss of accuracy (1.0 by default)."""
@@ -184,20 +206,34 @@ class EtaTuner:
         if k not in self.table:
             m = self.measure(prob, start)
             best = 1.0
+            switch = self.fp32_switches[0]
             if m:
-                ok = {f: v[0] for f, v in m["factors"].items() if v[1] <= self.margin * m["reference_error"]}
+                ok = self._qualifying(m)
                 if ok:
                     best = min(ok, key=lambda f: (ok[f], f))
-            self.table[k] = {"factor": best,
+                sws = m.get("switches") or {}
+                if switch in sws:
+                    steps0 = sws[switch][0]
+                    good = [sw for sw, v in sws.items() if v[0] <= steps0 and v[1] <= self.margin * m["reference_error"]]
+                    switch = min(good)
+            self.table[k] = {"factor": best, "fp32_switch": switch,
                              "reference_steps": m.get("reference_steps"), "reference_error": m.get("reference_error"),
                              "factors": {f"{f:g}": {"steps": v[0], "error": v[1]}
-                                         for f, v in (m.get("factors") or {}).items()}}
+                                         for f, v in (m.get("factors") or {}).items()},
+                             "fp32_switches": {f"{sw:g}": {"steps": v[0], "error": v[1]}
+                                               for sw, v in (m.get("switches") or {}).items()}}
             if self.cache_path:
                 tmp = self.cache_path + ".tmp"
                 with open(tmp, "w") as fh:
                     json.dump(self.table, fh, indent=1, sort_keys=True)
                 os.replace(tmp, self.cache_path)
         return float(self.table[k]["factor"])
+
+    def fp32_switch_for(self, prob: Problem, start_name: str) -> float:
+        """The FP32 -> FP64 switch level cached with the factor (after choose; the first level of
+        fp32_switches by default)."""
+        e = self.table.get(_eta_key(prob, start_name, self.target, self.precision), {})
+        return float(e.get("fp32_switch", self.fp32_switches[0]))
 
 
 # (f_hi, switch log10 L2, f_lo): multiples of the configured eta before and after the switch;

@@ -111,3 +111,19 @@ def test_schedule_cached_per_grid_start_tolerance_and_precision(tmp_path):
     t2 = autotune.ScheduleTuner(cache_path=path)
     t2.measure = lambda prob, start: (_ for _ in ()).throw(AssertionError("re-measured"))
     assert t2.choose(P, None, "nn")["schedule"] == (2.0, -9.0, 2.0)
+
+
+def test_deepest_fp32_switch_with_no_more_steps_within_the_margin():
+    res = {"reference_steps": 3144, "reference_error": 2e-8,
+           "factors": {1.0: (2900, 3e-9), 2.8: (1059, 8e-9)},
+           "switches": {-6.0: (1059, 8e-9), -7.0: (1059, 9.5e-9), -8.0: (1059, 1.5e-8), -9.0: (1328, 2e-9)}}
+    t, _ = _tuner(res, precision=32, fp32_switches=(-6.0, -7.0, -8.0, -9.0))
+    P = _prob()
+    assert t.choose(P, None, "nn") == 2.8
+    # -8 is over the 0.5 x 2e-8 margin, -9 takes more steps: -7
+    assert t.fp32_switch_for(P, "nn") == -7.0
+    t2, _ = _tuner(dict(res, switches={-6.0: (1059, 8e-9), -7.0: (1100, 5e-9)}), precision=32,
+                   fp32_switches=(-6.0, -7.0))
+    assert t2.choose(P, None, "nn") == 2.8 and t2.fp32_switch_for(P, "nn") == -6.0
+    t3, _ = _tuner({}, precision=32, fp32_switches=(-6.0, -7.0))
+    assert t3.choose(P, None, "nn") == 1.0 and t3.fp32_switch_for(P, "nn") == -6.0
