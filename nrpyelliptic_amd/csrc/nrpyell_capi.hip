@@ -1494,13 +1494,15 @@ struct nrpyell_batch {
   struct Group {
     int tx = 32, pl = 1;
     std::vector<int> members;
-    SubstepArgs<double> *d_args[5] = {};
+    SubstepArgs<double> *d_args[5] = {};   // one substep per launch: per stage 1..4
+    SubstepArgs<double> *d_pair[4] = {};   // two per launch: [2 j + l], l = 0 (stages 1+2), 1 (3+4), y in ycur(j)
     int *d_block_cfg = nullptr, *d_block_first = nullptr;  // sized for every member
     int nblocks = 0;                                       // blocks of the members in the graph
     hipStream_t side = nullptr;                            // the group's branch while capturing
     hipEvent_t joined = nullptr;
   };
   std::vector<Group> groups;
+  bool pair = false;  // every member runs two substeps per launch (as its single solver would)
   SolverDeviceState **d_states = nullptr;
   double *d_heads = nullptr, *h_heads = nullptr;
   hipGraphExec_t graph = nullptr;
@@ -1510,6 +1512,7 @@ struct nrpyell_batch {
     if (graph) (void)hipGraphExecDestroy(graph);
     for (auto &g : groups) {
       for (auto *p : g.d_args) (void)hipFree(p);
+      for (auto *p : g.d_pair) (void)hipFree(p);
       (void)hipFree(g.d_block_cfg);
       (void)hipFree(g.d_block_first);
       if (g.side) (void)hipStreamDestroy(g.side);
@@ -1522,12 +1525,24 @@ struct nrpyell_batch {
     for (auto *s : solvers) nrpyell_solver_destroy(s);
     if (stream) (void)hipStreamDestroy(stream);
   }
-  void step(hipStream_t st) {
+  // One RK4 step of a group: step k of a graph has y in ycur(k & 1) on the pair path.
+  void group_step(const Group &g, int k, hipStream_t st) const {
+    if (pair) {
+      const int j = k & 1;
+      launch_substep_pair_batched(g.d_pair[2 * j], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl, 1, st);
+      launch_substep_pair_batched(g.d_pair[2 * j + 1], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl, 3, st);
+      return;
+    }
     for (int stage = 1; stage <= 4; stage++)
-      for (auto &g : groups)
-        if (g.nblocks > 0)
-          launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl,
-                                         stage, st);
+      launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl, stage,
+                                     st);
+  }
+  // RK4 steps per graph: even on the pair path, so every graph starts with y in y_n_gfs
+  int graph_steps() const { return pair ? opt.steps_per_graph + (opt.steps_per_graph & 1) : opt.steps_per_graph; }
+  // blocks of one configuration in a launch
+  int blocks_of(const nrpyell_solver *s) const {
+    const int rows = pair ? NRPYELL_PAIR_BC_ROWS : NRPYELL_BC_ROWS;
+    return s->ntile0 * s->ntile1 * (s->g.N2 / s->pl) + (s->g.N1 + rows - 1) / rows * s->g.N2;
   }
   // Block maps over the configurations with active[k] set, and a graph of steps_per_graph
   // steps over them: converged configurations drop out of the launches.
@@ -1542,8 +1557,7 @@ struct nrpyell_batch {
         in_graph++;
         nrpyell_solver *s = solvers[k];
         first[j] = (int)bcfg.size();
-        const int nb = s->ntile0 * s->ntile1 * (s->g.N2 / s->pl) +
-                       (s->g.N1 + NRPYELL_BC_ROWS - 1) / NRPYELL_BC_ROWS * s->g.N2;
+        const int nb = blocks_of(s);
         for (int q = 0; q < nb; q++) bcfg.push_back(j);
       }
       if (!g.d_block_cfg) {  // first build: every member is active, the buffers get their full size
@@ -1562,7 +1576,9 @@ struct nrpyell_batch {
     for (auto &g : groups) nlive += g.nblocks > 0;
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     if (nlive <= 1) {
-      for (int k = 0; k < opt.steps_per_graph; k++) step(stream);
+      for (int k = 0; k < graph_steps(); k++)
+        for (auto &g : groups)
+          if (g.nblocks > 0) group_step(g, k, stream);
     } else {
       // the groups' configurations are independent: one branch per group (its steps in order,
       // on a stream forked from the capture and joined at the end), so the GPU can overlap the
@@ -1574,10 +1590,7 @@ struct nrpyell_batch {
         if (!g.side) HIPCHK(hipStreamCreateWithFlags(&g.side, hipStreamNonBlocking));
         if (!g.joined) HIPCHK(hipEventCreateWithFlags(&g.joined, hipEventDisableTiming));
         HIPCHK(hipStreamWaitEvent(g.side, fork, 0));
-        for (int k = 0; k < opt.steps_per_graph; k++)
-          for (int stage = 1; stage <= 4; stage++)
-            launch_substep_batched<double>(g.d_args[stage], g.d_block_cfg, g.d_block_first, g.nblocks, g.tx, g.pl,
-                                           stage, g.side);
+        for (int k = 0; k < graph_steps(); k++) group_step(g, k, g.side);
         HIPCHK(hipEventRecord(g.joined, g.side));
         HIPCHK(hipStreamWaitEvent(stream, g.joined, 0));
       }
@@ -1615,6 +1628,8 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
       b->solvers.push_back(s);
     }
     HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    b->pair = true;
+    for (auto *s : b->solvers) b->pair = b->pair && s->pair_path;
     std::map<std::pair<int, int>, std::vector<int>> by_tile;
     for (int k = 0; k < n; k++) by_tile[{b->solvers[k]->tx, b->solvers[k]->pl}].push_back(k);
     for (auto &kv : by_tile) {
@@ -1640,6 +1655,30 @@ extern "C" int nrpyell_batch_create(griddata_struct **gds, const double *dts, co
         }
         g.d_args[stage] = upload(args.data(), args.size());
       }
+      if (b->pair) {
+        // as step64_pair with the fused check: launch 1 writes the residual partials of y_n,
+        // launch 2's block 0 reduces them
+        for (int j = 0; j < 2; j++)
+          for (int l = 0; l < 2; l++) {
+            std::vector<SubstepArgs<double>> args(m);
+            for (int i = 0; i < m; i++) {
+              nrpyell_solver *s = b->solvers[g.members[i]];
+              MoL_gridfunctions_struct &f = s->gd->gridfuncs;
+              SubstepArgs<double> a = s->args<double>(1, s->ycur(j), f.k_odd_gfs, nullptr,
+                                                      f.y_nplus1_running_total_gfs, s->src64, s->kc64, s->geo64);
+              a.in = l == 0 ? s->ycur(j) : f.k_odd_gfs;
+              a.yn = s->ycur(j);
+              a.out = l == 0 ? f.k_odd_gfs : s->ycur(1 - j);
+              s->pair_layout(a);
+              a.l2_partial = s->partial;
+              a.state = s->dstate;
+              a.l2_volume = s->volume;
+              a.l2_target = b->opt.log10_target;
+              args[i] = a;
+            }
+            g.d_pair[2 * j + l] = upload(args.data(), args.size());
+          }
+      }
       b->groups.push_back(g);
     }
     std::vector<SolverDeviceState *> st(n);
@@ -1662,7 +1701,7 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
     std::vector<char> active(b->n, 1);
     while (true) {
       HIPCHK(hipGraphLaunch(b->graph, b->stream));
-      launched += b->opt.steps_per_graph;
+      launched += b->graph_steps();
       b->read_heads();
       int n_active = 0;
       for (int k = 0; k < b->n; k++) {
@@ -1673,7 +1712,13 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
       // compact when a quarter of the launched configurations have finished
       if (4 * n_active <= 3 * b->in_graph) b->rebuild(active);
     }
-    for (auto *s : b->solvers) s->fill_ghosts(b->stream);
+    for (int k = 0; k < b->n; k++) {
+      nrpyell_solver *s = b->solvers[k];
+      // pair path: a configuration stopped by its check holds the state it checked in the array
+      // of that step count's parity; whole graphs run an even number of steps
+      if (b->pair) s->settle(b->stream, b->h_heads[4 * k] != 0.0 ? ((int)b->h_heads[4 * k + 1] & 1) : 0);
+      s->fill_ghosts(b->stream);
+    }
     HIPCHK(hipStreamSynchronize(b->stream));
     const auto t1 = std::chrono::steady_clock::now();
     *seconds = std::chrono::duration<double>(t1 - t0).count();

@@ -176,6 +176,9 @@ void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 // two RK4 substeps (first_stage 1: stages 1+2; 3: stages 3+4) in one launch (nrpyell_substep2.hip)
 template <typename T>
 void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st);
+// the same for many configurations in one launch (block maps as launch_substep_batched)
+void launch_substep_pair_batched(const SubstepArgs<double> *cfgs, const int *block_cfg, const int *block_first,
+                                 int nblocks, int tx, int pl, int first_stage, hipStream_t st);
 template <typename T>
 void launch_substep_batched(const SubstepArgs<T> *cfgs, const int *block_cfg, const int *block_first, int nblocks,
                             int tx, int pl, int stage, hipStream_t st);

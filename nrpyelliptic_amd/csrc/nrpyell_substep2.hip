@@ -594,6 +594,57 @@ __global__ void __launch_bounds__(NRPYELL_BLOCK, 3) substep2_kernel(SubstepArgs<
   interior2_body<T, PAIR, TX, PL>(a, (int)tile, (int)(q - tile * ng), q, ni, lds, s_red);
 }
 
+// Many configurations (uneven grids) in one launch, as substep_batched_kernel does for one
+// substep: logical block q works for configuration block_cfg[q], as that configuration's block
+// q - block_first[cfg] (its BC blocks first, then its interior blocks); a configuration that has
+// converged returns at once (its own done flag). Block 0 of a configuration's interior blocks
+// of the stages-3+4 launch reduces that configuration's residual partials.
+template <int PAIR, int TX, int PL>
+__global__ void __launch_bounds__(NRPYELL_BLOCK, 3)
+    substep2_batched_kernel(const SubstepArgs<double> *__restrict__ cfgs, const int *__restrict__ block_cfg,
+                            const int *__restrict__ block_first) {
+  constexpr int NL = interior2_lds<TX, PL>() > bc2_lds() ? interior2_lds<TX, PL>() : bc2_lds();
+  __shared__ double lds[NL];
+  __shared__ double s_red[NRPYELL_BLOCK / 64];
+  const unsigned q = xcd_logical(blockIdx.x, gridDim.x);
+  const int c = block_cfg[q];
+  const SubstepArgs<double> a = cfgs[c];
+  if (*a.done) return;  // leave before issuing any load
+  const unsigned local = q - block_first[c], nb = (unsigned)(a.nbc_rows * a.g.N2);
+  if (local < nb) {
+    const unsigned pln = div_by(local, a.nbc_mul);
+    bc2_body<double, PAIR>(a, (int)(local - pln * (unsigned)a.nbc_rows), (int)pln, lds);
+    return;
+  }
+  const unsigned li = local - nb, ng = (unsigned)(a.g.N2 / PL);
+  const unsigned tile = div_by(li, a.ng_mul);
+  interior2_body<double, PAIR, TX, PL>(a, (int)tile, (int)(li - tile * ng), li, a.ntile0 * a.ntile1 * ng, lds, s_red);
+}
+
+template <int PAIR>
+static void launch_pair_batched(const SubstepArgs<double> *cfgs, const int *block_cfg, const int *block_first,
+                                int nblocks, int tx, int pl, hipStream_t st) {
+  dim3 grd(nblocks), blk(NRPYELL_BLOCK);
+  if (pl == 2) {
+    switch (tx) {
+      case 64: hipLaunchKernelGGL((substep2_batched_kernel<PAIR, 64, 2>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+      default: hipLaunchKernelGGL((substep2_batched_kernel<PAIR, 32, 2>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    }
+    return;
+  }
+  switch (tx) {
+    case 16: hipLaunchKernelGGL((substep2_batched_kernel<PAIR, 16, 1>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    case 64: hipLaunchKernelGGL((substep2_batched_kernel<PAIR, 64, 1>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+    default: hipLaunchKernelGGL((substep2_batched_kernel<PAIR, 32, 1>), grd, blk, 0, st, cfgs, block_cfg, block_first); break;
+  }
+}
+
+void launch_substep_pair_batched(const SubstepArgs<double> *cfgs, const int *block_cfg, const int *block_first,
+                                 int nblocks, int tx, int pl, int first_stage, hipStream_t st) {
+  if (first_stage == 1) launch_pair_batched<1>(cfgs, block_cfg, block_first, nblocks, tx, pl, st);
+  else launch_pair_batched<3>(cfgs, block_cfg, block_first, nblocks, tx, pl, st);
+}
+
 template <typename T, int PAIR>
 static void launch_pair(const SubstepArgs<T> &a, hipStream_t st) {
   dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(NRPYELL_BLOCK);
