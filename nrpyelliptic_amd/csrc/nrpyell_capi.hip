@@ -702,6 +702,7 @@ struct nrpyell_solver {
   // FP32 shadow state
   float *y32 = nullptr, *ko32 = nullptr, *ke32 = nullptr, *rt32 = nullptr, *src32 = nullptr;
   hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
+  hipGraphExec_t graph_step = nullptr;  // the drop-in MoL_step_forward_in_time call
   // what a captured graph bakes into its kernel arguments; a graph is recaptured when any of it
   // changes between solves (eta_damping or the outer-boundary wave speeds edited in the params)
   struct GraphKey {
@@ -711,7 +712,7 @@ struct nrpyell_solver {
       return eta == o.eta && dt == o.dt && cu == o.cu && cv == o.cv && target == o.target && steps == o.steps;
     }
   };
-  GraphKey key64, key32;
+  GraphKey key64, key32, key_step;
   // residual history of the FP32 phase of the last mixed-precision solve (its checks count FP32
   // steps); the FP64 phase's entries follow it, offset by fp32_offset steps
   std::vector<int> hist32_step;
@@ -722,6 +723,7 @@ struct nrpyell_solver {
   ~nrpyell_solver() {
     if (graph64) (void)hipGraphExecDestroy(graph64);
     if (graph32) (void)hipGraphExecDestroy(graph32);
+    if (graph_step) (void)hipGraphExecDestroy(graph_step);
     (void)hipFree(coef64); (void)hipFree(kc64); (void)hipFree(kc32); (void)hipFree(src64);
     (void)hipFree(geo64); (void)hipFree(geo32); (void)hipFree(w2d); (void)hipFree(partial);
     (void)hipFree(l2_out); (void)hipFree(dstate); (void)hipFree(bad);
@@ -862,7 +864,7 @@ struct nrpyell_solver {
   // from auxevol: the only reads of those arrays. Raises `bad` (cleared first) when the wave
   // speed varies along xx2.
   void refresh_tables(hipStream_t st) {
-    HIPCHK(hipMemsetAsync(bad, 0, sizeof(int), st));
+    launch_set_int(bad, 0, st);  // a kernel, not a memset: this runs inside the drop-in step's graph
     launch_coef(g, &gd->rfmstruct, coef64, nullptr, st);
     launch_fold(g, gd->gridfuncs.auxevol_gfs, coef64, w2d, kc64, kc32, src64, bad, st);
     HIPCHK(hipGetLastError());
@@ -1453,14 +1455,33 @@ extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL d
         if (nrpyell_solver_create(griddata, dt, &o, &s) != NRPYELL_OK) fatal("MoL_step_forward_in_time", g_last_error);
         g_step_cache[griddata] = {s, gf, griddata->rfmstruct.f0_of_xx0, griddata->bcstruct.inner_bc_array,
                                   griddata->bcstruct.bc_info.num_inner_boundary_points, key};
-      } else {
-        s->refresh_tables(s->stream);
       }
     }
     s->dt = dt;
-    HIPCHK(hipMemsetAsync(&s->dstate->done, 0, sizeof(int), s->stream));
-    s->step64(s->stream);
-    s->fill_ghosts(s->stream);
+    // one graph per solver: table refresh (the rfm and auxevol arrays are re-read every call),
+    // the RK4 step (two launches of two substeps where the grid allows, then y back to
+    // y_n_gfs), the phi ghost planes; recaptured when dt, eta or the outer wave speeds change
+    const nrpyell_solver::GraphKey k = s->graph_key(false, 0.0);
+    if (!s->graph_step || !s->key_step.same(k)) {
+      if (s->graph_step) (void)hipGraphExecDestroy(s->graph_step);
+      s->graph_step = nullptr;
+      hipGraph_t gr;
+      HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      s->refresh_tables(s->stream);
+      launch_set_int(&s->dstate->done, 0, s->stream);
+      if (s->pair_path) {
+        s->step64_pair(s->stream, 0, false);
+        s->settle(s->stream, 1);
+      } else {
+        s->step64(s->stream);
+      }
+      s->fill_ghosts(s->stream);
+      HIPCHK(hipStreamEndCapture(s->stream, &gr));
+      HIPCHK(hipGraphInstantiate(&s->graph_step, gr, nullptr, nullptr, 0));
+      HIPCHK(hipGraphDestroy(gr));
+      s->key_step = k;
+    }
+    HIPCHK(hipGraphLaunch(s->graph_step, s->stream));
     HIPCHK(hipGetLastError());
     int hbad = 0;
     HIPCHK(hipMemcpyAsync(&hbad, s->bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));

@@ -162,6 +162,7 @@ struct SubstepArgs {
 
 
 // kernel launchers (nrpyell_kernels.hip)
+void launch_set_int(int *p, int v, hipStream_t st);
 void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in, double *out,
                       double eta, hipStream_t st);
 void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in,

@@ -364,6 +364,14 @@ void launch_l2_finalize(const double *partial, long npart, double volume, Solver
 // Drop-in boundary kernels.
 // ---------------------------------------------------------------------------------------
 // apply_bcs_inner_only.c:18-26: gfs[dst] = parity[0] * gfs[src] for both evolved gfs.
+// *p = v on the stream: a kernel where a memset would do, because this is recorded into graphs
+// (the drop-in MoL step's), and memset nodes are not used in them.
+__global__ void set_int_kernel(int *p, int v) {
+  if (threadIdx.x == 0) *p = v;
+}
+
+void launch_set_int(int *p, int v, hipStream_t st) { hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, st, p, v); }
+
 __global__ void inner_gather_kernel(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;

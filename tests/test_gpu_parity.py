@@ -123,8 +123,9 @@ def test_l2_norm(pair):
 
 @pytest.mark.parametrize("nsteps", [1, 20])
 def test_mol_step_matches_oracle(pair, nsteps):
-    """MoL_step_forward_in_time (4 fused kernels) vs the oracle's reference-order RK4; y_n at
-    every point including all ghost zones."""
+    """MoL_step_forward_in_time (the fused kernels: two launches of two substeps where the grid
+    allows, else four) vs the oracle's reference-order RK4; y_n at every point including all ghost
+    zones."""
     P, O = pair
     use_oracle_inputs(P, O)
     P.set("y_n_gfs", np.zeros(2 * P.ntot))
@@ -134,6 +135,28 @@ def test_mol_step_matches_oracle(pair, nsteps):
         O.mol_step(st)
     got = P.get("y_n_gfs")
     assert rel_maxnorm(got, st["y_n"]) <= 1e-12
+
+
+def test_mol_step_follows_dt_and_eta_changes(pair):
+    """MoL_step_forward_in_time replays one captured graph per cached solver; a call with another
+    dt, or after the damping in the params changed, must step with the new values (the graph is
+    recaptured), as the reference's step reads them on every call."""
+    P, O = pair
+    use_oracle_inputs(P, O)
+    P.set("y_n_gfs", np.zeros(2 * P.ntot))
+    st = O.new_state()
+    eta0 = P.gd.params.eta_damping
+    try:
+        for k, (dt, eta) in enumerate([(O.dt, eta0), (0.5 * O.dt, eta0), (0.5 * O.dt, eta0), (0.5 * O.dt, 2.0 * eta0),
+                                       (O.dt, 2.0 * eta0)]):
+            P.gd.params.eta_damping = eta
+            O.g.eta = eta
+            P.mol_step(dt)
+            O.mol_step(st, dt=dt)
+            assert rel_maxnorm(P.get("y_n_gfs"), st["y_n"]) <= 1e-12, k
+    finally:
+        P.gd.params.eta_damping = eta0
+        O.g.eta = eta0
 
 
 @pytest.mark.parametrize("tile", [(32, 1), (16, 1), (64, 2), (32, 2)], ids=lambda t: f"tx{t[0]}_pl{t[1]}")
