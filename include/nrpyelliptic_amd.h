@@ -193,7 +193,8 @@ void apply_bcs_outerradiation_and_inner(const paramstruct *NRPYELL_RESTRICT para
 /* :8 / apply_bcs_inner_only.c:11. HIP gather kernel. */
 void apply_bcs_inner_only(const paramstruct *NRPYELL_RESTRICT params, const bc_struct *NRPYELL_RESTRICT bcstruct,
                           REAL *NRPYELL_RESTRICT gfs);
-/* :5 / MoL_step_forward_in_time.c:7. One RK4 step as 4 fused HIP kernels.
+/* :5 / MoL_step_forward_in_time.c:7. One RK4 step of fused HIP kernels (two launches of two
+ * substeps where the grid allows, else four), replayed as one captured graph per grid.
  * Precondition (always true for data produced by this path or the reference's zero
  * initial guess): y_n ghost values equal their parity images. After the call y_n is
  * exact at every point; k_odd/k_even/y_nplus1_running_total are scratch. */
