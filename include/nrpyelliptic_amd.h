@@ -354,6 +354,12 @@ int nrpyell_set_eta_damping(griddata_struct *gd, double eta);
 int nrpyell_solver_substeps_per_launch(nrpyell_solver *s);
 /* Residual history of the last solve: (steps completed, log10 L2) pairs; returns the count. */
 int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10_l2);
+/* Block placement of the solver's two-substep kernel: out[L] (L < max) = the block that the
+ * L-th workgroup of a launch computes (BC blocks first, then interior blocks in tile order);
+ * returns the launch's block count, or 0 when the kernel runs its own order (one-substep path,
+ * or a launch that does not fit one round of resident workgroups). No counterpart in the
+ * reference (a property of this GPU path): for tools that inspect the placement. */
+int nrpyell_solver_block_order(const nrpyell_solver *s, unsigned short *out, int max);
 
 /* Batched FP64 solver: n configurations, each with its own griddata (any grid shape), relaxed
  * together -- one launch per RK4 substep per tile width covers every configuration, and each

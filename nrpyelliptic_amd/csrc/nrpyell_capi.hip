@@ -6,6 +6,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <map>
 #include <memory>
@@ -703,6 +704,7 @@ struct nrpyell_solver {
   float *y32 = nullptr, *ko32 = nullptr, *ke32 = nullptr, *rt32 = nullptr, *src32 = nullptr;
   hipGraphExec_t graph64 = nullptr, graph32 = nullptr;
   hipGraphExec_t graph_step = nullptr;  // the drop-in MoL_step_forward_in_time call
+  BlockOrder order{};                    // placement of the two-substep kernel's blocks (n == 0: none)
   // what a captured graph bakes into its kernel arguments; a graph is recaptured when any of it
   // changes between solves (eta_damping or the outer-boundary wave speeds edited in the params)
   struct GraphKey {
@@ -793,11 +795,11 @@ struct nrpyell_solver {
       a.l2_volume = volume;
       a.l2_target = opt.log10_target;
     }
-    launch_substep_pair(a, 1, st);
+    launch_substep_pair(a, 1, st, &order);
     a.in = f.k_odd_gfs;
     a.yn = ycur(j);
     a.out = ycur(1 - j);
-    launch_substep_pair(a, 3, st);
+    launch_substep_pair(a, 3, st, &order);
   }
   // the pair kernel's BC blocks cover NRPYELL_PAIR_BC_ROWS rows each
   template <typename T>
@@ -825,11 +827,11 @@ struct nrpyell_solver {
     a.yn = ycur32(j);
     a.out = ko32;
     pair_layout(a);
-    launch_substep_pair(a, 1, st);
+    launch_substep_pair(a, 1, st, &order);
     a.in = ko32;
     a.yn = ycur32(j);
     a.out = ycur32(1 - j);
-    launch_substep_pair(a, 3, st);
+    launch_substep_pair(a, 3, st, &order);
   }
   // One FP64 RK4 step; with fuse_check the stage-1 kernel also reduces the residual L2 of
   // y_n (before the step) into the convergence state.
@@ -884,6 +886,102 @@ struct nrpyell_solver {
     launch_l2_finalize(partial, npart, volume, dstate, steps_per_check, target, l2_out, st);
   }
 };
+
+// Placement of the two-substep kernel's blocks (BlockOrder). Workgroups are dealt to the XCDs
+// round robin (workgroup L to XCD L % 8) and within an XCD to its CUs round robin (its k-th
+// workgroup to CU k % P, P = CUs per XCD), as the hardware-id register shows on MI355X. When the
+// whole launch is resident at once, each CU's time is set by the blocks it holds, and the blocks
+// differ: per-block timestamps at 128x128x16 give BC blocks and last interior tiles (radiation
+// terms, outer layers) the longest lives, first tiles the shortest; the kernel's own order happened
+// to put the last tile columns on the CUs that also hold BC blocks. Each XCD keeps the blocks the
+// kernel's own order gives it (whole tile columns: L2 reuse of the i2 arms and i1 halos). Its BC
+// blocks keep the first positions (dispatched first, one per CU); the interior blocks go heaviest
+// first to the CU with the least load and a free position, then pairwise swaps of interior blocks
+// lower the most loaded CU while they can. Relative weights: BC 1.25, last tile 1.1, other tiles
+// 1.0, first tile 0.85. A CU's blocks are then dispatched last tile first, first tile next, other
+// tiles after: the order matters as much as the split, measured at 128x128x16 where this gives
+// the CUs holding three blocks a BC block, a first tile and another tile, in that order, and the
+// CUs holding two a last tile and another. Placements measured there (us per RK4 step): the
+// kernel's own order 21.5; BC blocks placed like interior blocks, on the CUs that hold two, 20.6;
+// the same split as here with the first tile after the other tile 21.3; a weight of +0.15 for
+// tiles on the first or last i1 row 20.3-20.45; this one 20.05-20.1.
+static void pair_block_order(nrpyell_solver *s) {
+  s->order.n = 0;
+  if (!s->pair_path) return;
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int nbc_rows = (s->g.N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS;
+  const int ng = s->g.N2 / s->pl, nb = nbc_rows * s->g.N2, ni = s->ntile0 * s->ntile1 * ng, n = nb + ni;
+  const int P = ncu / 8;
+  // one round of resident blocks (3 per CU: __launch_bounds__(256, 3), 37 KB of LDS each)
+  if (ncu % 8 != 0 || P <= 0 || n > NRPYELL_ORDER_MAX || n > 3 * ncu || nb % 8 || ni % 8) return;
+  const int per = ni / 8, nx = n / 8;
+  struct Item {
+    double w;
+    int logical, rank;  // rank: dispatch order within a CU (0 last tile, 1 first tile, 2 other)
+  };
+  const int nbx = nb / 8;  // BC blocks per XCD
+  if (nbx > P) return;
+  for (int x = 0; x < 8; x++) {
+    std::vector<Item> mem;
+    for (int q = x * per; q < (x + 1) * per; q++) {
+      const int t0 = (q / ng) % s->ntile0;
+      const int rank = t0 == s->ntile0 - 1 ? 0 : t0 == 0 ? 1 : 2;
+      mem.push_back({rank == 0 ? 1.1 : rank == 1 ? 0.85 : 1.0, nb + q, rank});
+    }
+    std::stable_sort(mem.begin(), mem.end(), [](const Item &a, const Item &b) { return a.w > b.w; });
+    std::vector<std::vector<Item>> on(P);
+    std::vector<double> load(P, 0.0);
+    std::vector<int> cap(P, 0);
+    for (int j = nbx; j < nx; j++) cap[j % P]++;  // positions left after the BC blocks
+    for (int k = 0; k < nbx; k++) {
+      load[k] = 1.25;                                   // position k, CU k
+      s->order.set(8 * k + x, (unsigned)(8 * k + x));  // BC block 8 k + x: its own order
+    }
+    for (const Item &m : mem) {
+      int best = -1;
+      for (int c = 0; c < P; c++)
+        if ((int)on[c].size() < cap[c] &&
+            (best < 0 || load[c] < load[best] || (load[c] == load[best] && cap[c] < cap[best])))
+          best = c;
+      on[best].push_back(m);
+      load[best] += m.w;
+    }
+    for (int it = 0; it < 4096; it++) {
+      const int cm = (int)(std::max_element(load.begin(), load.end()) - load.begin());
+      double gain = 1e-9;
+      int bi = -1, bc = -1, bj = -1;
+      for (int i = 0; i < (int)on[cm].size(); i++)
+        for (int c = 0; c < P; c++)
+          for (int j = 0; c != cm && j < (int)on[c].size(); j++) {
+            const double d = on[cm][i].w - on[c][j].w;
+            const double g = load[cm] - std::max(load[cm] - d, load[c] + d);
+            if (g > gain) { gain = g; bi = i; bc = c; bj = j; }
+          }
+      if (bi < 0) break;
+      const double d = on[cm][bi].w - on[bc][bj].w;
+      std::swap(on[cm][bi], on[bc][bj]);
+      load[cm] -= d;
+      load[bc] += d;
+    }
+    for (int c = 0; c < P; c++) {
+      std::stable_sort(on[c].begin(), on[c].end(), [](const Item &a, const Item &b) {
+        return a.rank != b.rank ? a.rank < b.rank : a.logical < b.logical;
+      });
+      int k = 0;
+      for (int j = c < nbx ? c + P : c; j < nx; j += P) s->order.set(8 * j + x, (unsigned)on[c][k++].logical);
+    }
+  }
+  s->order.n = n;
+}
+
+extern "C" int nrpyell_solver_block_order(const nrpyell_solver *s, unsigned short *out, int max_entries) {
+  if (!s) return 0;
+  const int n = !out ? 0 : s->order.n < max_entries ? s->order.n : max_entries;
+  for (int i = 0; i < n; i++) out[i] = (unsigned short)s->order.at(i);
+  return s->order.n;
+}
 
 static void reset_state(nrpyell_solver *s) {
   s->hist32_step.clear();
@@ -999,6 +1097,7 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
         return NRPYELL_EINVAL;
       }
     }
+    pair_block_order(s.get());
     HIPCHK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     const long n2 = (long)g.G0 * g.G1;
     s->coef64 = dalloc<double>(5 * n2);
@@ -1328,12 +1427,12 @@ extern "C" int nrpyell_solver_time_substep(nrpyell_solver *s, int stage, int n, 
         a.l2_partial = nullptr;  // time the steady-state launch, not the block-0 reduction
       }
       const int first = stage == 5 ? 1 : 3;
-      for (int k = 0; k < 3; k++) launch_substep_pair(a, first, s->stream);
+      for (int k = 0; k < 3; k++) launch_substep_pair(a, first, s->stream, &s->order);
       hipEvent_t e0, e1;
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
       HIPCHK(hipEventRecord(e0, s->stream));
-      for (int k = 0; k < n; k++) launch_substep_pair(a, first, s->stream);
+      for (int k = 0; k < n; k++) launch_substep_pair(a, first, s->stream, &s->order);
       HIPCHK(hipEventRecord(e1, s->stream));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0;

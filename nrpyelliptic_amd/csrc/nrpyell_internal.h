@@ -175,8 +175,26 @@ void launch_fold(const GridDims &g, const double *aux, const double *coef, const
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 // two RK4 substeps (first_stage 1: stages 1+2; 3: stages 3+4) in one launch (nrpyell_substep2.hip)
+// Placement of the two-substep kernel's blocks for a launch that fits the GPU in one round:
+// workgroup L runs as logical block q[L] (< nbc_rows * N2: a BC block, else interior block q[L] -
+// that). Built on the host (pair_block_order) so that the CUs' loads even out; n == 0: the
+// kernel's own order (xcd_logical). Passed by value in the kernel arguments, so a block reads its
+// entry with the other arguments.
+#define NRPYELL_ORDER_MAX 1024
+struct BlockOrder {
+  int n;
+  unsigned q2[NRPYELL_ORDER_MAX / 2];  // entry L in bits 16 (L & 1) .. 16 (L & 1) + 15 of q2[L / 2]
+  __host__ __device__ unsigned at(unsigned L) const {
+    const unsigned w = q2[(L < NRPYELL_ORDER_MAX ? L : NRPYELL_ORDER_MAX - 1) >> 1];
+    return (L & 1) ? w >> 16 : w & 0xffffu;
+  }
+  void set(unsigned L, unsigned v) {
+    unsigned &w = q2[L >> 1];
+    w = (L & 1) ? (w & 0xffffu) | (v << 16) : (w & 0xffff0000u) | v;
+  }
+};
 template <typename T>
-void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st);
+void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st, const BlockOrder *ord = nullptr);
 // the same for many configurations in one launch (block maps as launch_substep_batched)
 void launch_substep_pair_batched(const SubstepArgs<double> *cfgs, const int *block_cfg, const int *block_first,
                                  int nblocks, int tx, int pl, int first_stage, hipStream_t st);

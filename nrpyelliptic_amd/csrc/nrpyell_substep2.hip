@@ -576,20 +576,24 @@ __device__ __forceinline__ void bc2_body(const SubstepArgs<T> &a, const int rb, 
   }
 }
 
-// One configuration: the nbc_rows*N2 BC blocks first, then the interior blocks in XCD-aware
-// order.
+// One configuration. Without a placement table: the nbc_rows*N2 BC blocks first, then the
+// interior blocks in XCD-aware order. With one (ord.n == gridDim.x): workgroup L is logical block
+// ord.q[L], the BC blocks being logical 0 .. nb-1.
 template <typename T, int PAIR, int TX, int PL>
-__global__ void __launch_bounds__(NRPYELL_BLOCK, 3) substep2_kernel(SubstepArgs<T> a) {
+__global__ void __launch_bounds__(NRPYELL_BLOCK, 3) substep2_kernel(SubstepArgs<T> a, const BlockOrder ord) {
   constexpr int NL = interior2_lds<TX, PL>() > bc2_lds() ? interior2_lds<TX, PL>() : bc2_lds();
   __shared__ T lds[NL];
   __shared__ double s_red[NRPYELL_BLOCK / 64];
-  const unsigned L = blockIdx.x, nb = (unsigned)(a.nbc_rows * a.g.N2);
+  const unsigned nb = (unsigned)(a.nbc_rows * a.g.N2), ni = gridDim.x - nb;
+  const bool ordered = ord.n == (int)gridDim.x;
+  const unsigned v = ord.at(blockIdx.x);  // unconditional: a scalar load issued with the other arguments
+  const unsigned L = ordered ? v : blockIdx.x;
   if (L < nb) {
     const unsigned pln = div_by(L, a.nbc_mul);
     bc2_body<T, PAIR>(a, (int)(L - pln * (unsigned)a.nbc_rows), (int)pln, lds);
     return;
   }
-  const unsigned ni = gridDim.x - nb, q = xcd_logical(L - nb, ni), ng = (unsigned)(a.g.N2 / PL);
+  const unsigned q = ordered ? L - nb : xcd_logical(L - nb, ni), ng = (unsigned)(a.g.N2 / PL);
   const unsigned tile = div_by(q, a.ng_mul);
   interior2_body<T, PAIR, TX, PL>(a, (int)tile, (int)(q - tile * ng), q, ni, lds, s_red);
 }
@@ -646,26 +650,28 @@ void launch_substep_pair_batched(const SubstepArgs<double> *cfgs, const int *blo
 }
 
 template <typename T, int PAIR>
-static void launch_pair(const SubstepArgs<T> &a, hipStream_t st) {
+static void launch_pair(const SubstepArgs<T> &a, hipStream_t st, const BlockOrder &o) {
   dim3 grd(a.nbc_rows * a.g.N2 + a.ntile0 * a.ntile1 * (a.g.N2 / a.pl)), blk(NRPYELL_BLOCK);
   if (a.pl == 2) {
     switch (a.tx) {
-      case 64: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 64, 2>), grd, blk, 0, st, a); break;
-      default: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 32, 2>), grd, blk, 0, st, a); break;
+      case 64: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 64, 2>), grd, blk, 0, st, a, o); break;
+      default: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 32, 2>), grd, blk, 0, st, a, o); break;
     }
     return;
   }
   switch (a.tx) {
-    case 16: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 16, 1>), grd, blk, 0, st, a); break;
-    case 64: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 64, 1>), grd, blk, 0, st, a); break;
-    default: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 32, 1>), grd, blk, 0, st, a); break;
+    case 16: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 16, 1>), grd, blk, 0, st, a, o); break;
+    case 64: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 64, 1>), grd, blk, 0, st, a, o); break;
+    default: hipLaunchKernelGGL((substep2_kernel<T, PAIR, 32, 1>), grd, blk, 0, st, a, o); break;
   }
 }
 
 template <typename T>
-void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st) {
-  if (first_stage == 1) launch_pair<T, 1>(a, st);
-  else launch_pair<T, 3>(a, st);
+void launch_substep_pair(const SubstepArgs<T> &a, int first_stage, hipStream_t st, const BlockOrder *ord) {
+  static const BlockOrder none{};
+  const BlockOrder &o = ord ? *ord : none;
+  if (first_stage == 1) launch_pair<T, 1>(a, st, o);
+  else launch_pair<T, 3>(a, st, o);
 }
-template void launch_substep_pair<double>(const SubstepArgs<double> &, int, hipStream_t);
-template void launch_substep_pair<float>(const SubstepArgs<float> &, int, hipStream_t);
+template void launch_substep_pair<double>(const SubstepArgs<double> &, int, hipStream_t, const BlockOrder *);
+template void launch_substep_pair<float>(const SubstepArgs<float> &, int, hipStream_t, const BlockOrder *);

@@ -102,6 +102,7 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_timed_steps.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.nrpyell_solver_substeps_per_launch.argtypes = [vp]
+    L.nrpyell_solver_block_order.argtypes = [vp, C.POINTER(C.c_ushort), C.c_int]
     L.nrpyell_set_eta_damping.argtypes = [C.POINTER(abi.griddata_struct), C.c_double]
     L.nrpyell_batch_create.argtypes = [C.POINTER(C.POINTER(abi.griddata_struct)), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options),
@@ -497,6 +498,14 @@ class Solver:
     def substeps_per_launch(self) -> int:
         """1 or 2: RK4 substeps per launch of the FP64 steps."""
         return lib().nrpyell_solver_substeps_per_launch(self.h)
+
+    def block_order(self) -> np.ndarray:
+        """Placement of the two-substep kernel's blocks: entry L is the block (BC blocks first,
+        then interior blocks in tile order) that workgroup L of a launch computes; empty when the
+        kernel runs its own order."""
+        buf = (C.c_ushort * 1024)()
+        n = lib().nrpyell_solver_block_order(self.h, buf, 1024)
+        return np.array(buf[:n], dtype=np.int64)
 
     def time_substep(self, stage: int, n: int) -> float:
         v = C.c_double()

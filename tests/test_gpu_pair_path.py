@@ -63,6 +63,41 @@ def test_pair_path_solve_matches_single():
         P.close()
 
 
+def test_pair_block_order_places_every_block_once():
+    """The two-substep kernel's placement table at 128x128x16, tiles 32 x 8 x 2 planes (512
+    interior blocks after 128 BC blocks of 16 rows, one round of resident workgroups on 256 CUs):
+    a permutation of the launch's blocks; the BC blocks keep their positions and every interior
+    block its XCD (workgroup L runs on XCD L % 8, which the kernel's own order gives tiles
+    (L % 8) * 64 .. + 63); and the XCD's k-th workgroup runs on its CU k % 32, so the CUs
+    holding three blocks get a BC block, a first-column tile and a middle tile, in that order,
+    and those holding two a last-column tile and a middle tile."""
+    P = runtime.Problem(config.equal_mass_boosted())
+    try:
+        S = P.solver(substeps_per_launch=2, tile_i0=32, planes=2)
+        o = S.block_order()
+        S.close()
+        ntile0, ntile1, ng = 128 // 32, 128 // 8, 16 // 2
+        nb, ni = (128 // 16) * 16, ntile0 * ntile1 * ng
+        assert len(o) == nb + ni and np.array_equal(np.sort(o), np.arange(nb + ni))
+        assert np.array_equal(o[:nb], np.arange(nb))
+        L = np.arange(nb, nb + ni)
+        assert np.array_equal((o[L] - nb) // (ni // 8), L % 8)
+
+        def kind(b):
+            if b < nb:
+                return "B"
+            t0 = ((b - nb) // ng) % ntile0
+            return "F" if t0 == 0 else "L" if t0 == ntile0 - 1 else "M"
+
+        for x in range(8):
+            cus = [""] * 32
+            for j, b in enumerate(o[x::8]):
+                cus[j % 32] += kind(int(b))
+            assert cus == ["BFM"] * 16 + ["LM"] * 16, (x, cus)
+    finally:
+        P.close()
+
+
 def test_pair_path_option_checks():
     P = runtime.Problem(config.axisymmetric_spinning())  # Nxx2 = 4
     try:
