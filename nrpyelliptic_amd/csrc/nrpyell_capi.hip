@@ -80,6 +80,17 @@ static T *dalloc(size_t n) {
   HIPCHK(hipMalloc(&p, n * sizeof(T)));
   return (T *)p;
 }
+// Pinned, mapped host memory for a small result that a kernel writes and the host reads after a
+// stream synchronisation, without a device-to-host copy; *dev is the address kernels use.
+template <typename T>
+static T *halloc_mapped(size_t n, T **dev) {
+  void *h = nullptr, *d = nullptr;
+  HIPCHK(hipHostMalloc(&h, n * sizeof(T), hipHostMallocMapped | hipHostMallocCoherent));
+  memset(h, 0, n * sizeof(T));
+  HIPCHK(hipHostGetDevicePointer(&d, h, 0));
+  *dev = (T *)d;
+  return (T *)h;
+}
 template <typename T>
 static T *upload(const T *h, size_t n) {
   T *d = dalloc<T>(n);
@@ -493,9 +504,10 @@ struct L2Weights {
   std::vector<double> key;
   double *w2d = nullptr;
   double *partial = nullptr;
-  double *result = nullptr;
+  double *result = nullptr;       // device address of result_host
+  double *result_host = nullptr;  // pinned, mapped
   double volume = 0.0;
-  ~L2Weights() { (void)hipFree(w2d); (void)hipFree(partial); (void)hipFree(result); }
+  ~L2Weights() { (void)hipFree(w2d); (void)hipFree(partial); (void)hipHostFree(result_host); }
 };
 static std::vector<std::unique_ptr<L2Weights>> g_l2_cache;
 
@@ -519,7 +531,7 @@ static L2Weights &l2_weights(const paramstruct *p, REAL *const xx[3], double rad
   w->w2d = upload(h.data(), h.size());
   const long nblk = (long)((g.N0 + 63) / 64) * ((g.N1 + 3) / 4) * g.N2;
   w->partial = dalloc<double>(nblk);
-  w->result = dalloc<double>(1);
+  w->result_host = halloc_mapped<double>(1, &w->result);
   if (g_l2_cache.size() > 32) g_l2_cache.erase(g_l2_cache.begin());
   g_l2_cache.push_back(std::move(w));
   return *g_l2_cache.back();
@@ -534,9 +546,8 @@ extern "C" REAL L2_norm_of_gf(const paramstruct *params, const int gf_index, con
     const long npart = launch_l2_partial(g, in_gf + (long)gf_index * g.ntot, w.w2d, w.partial, default_stream());
     launch_l2_finalize(w.partial, npart, w.volume, nullptr, 0, 0.0, w.result, default_stream());
     HIPCHK(hipGetLastError());
-    double out = 0.0;
-    HIPCHK(hipMemcpy(&out, w.result, sizeof(double), hipMemcpyDeviceToHost));
-    return out;
+    HIPCHK(hipStreamSynchronize(default_stream()));
+    return *(volatile double *)w.result_host;
   });
 }
 
@@ -722,13 +733,14 @@ struct nrpyell_solver {
   int fp32_offset = 0;
   double *cached_aux = nullptr;
   int *bad = nullptr;         // set by the fold kernel when the wave speed varies along xx2
+  int *bad_host = nullptr;    // the same flag as the host reads it (pinned, mapped; bad is its device address)
   ~nrpyell_solver() {
     if (graph64) (void)hipGraphExecDestroy(graph64);
     if (graph32) (void)hipGraphExecDestroy(graph32);
     if (graph_step) (void)hipGraphExecDestroy(graph_step);
     (void)hipFree(coef64); (void)hipFree(kc64); (void)hipFree(kc32); (void)hipFree(src64);
     (void)hipFree(geo64); (void)hipFree(geo32); (void)hipFree(w2d); (void)hipFree(partial);
-    (void)hipFree(l2_out); (void)hipFree(dstate); (void)hipFree(bad);
+    (void)hipFree(l2_out); (void)hipFree(dstate); (void)hipHostFree(bad_host);
     if (hstate) (void)hipHostFree(hstate);
     (void)hipFree(y32); (void)hipFree(ko32); (void)hipFree(ke32); (void)hipFree(rt32); (void)hipFree(src32);
     if (stream) (void)hipStreamDestroy(stream);
@@ -807,12 +819,6 @@ struct nrpyell_solver {
     a.nbc_rows = (g.N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS;
     a.nbc_mul = div_mul((unsigned)a.nbc_rows);
   }
-  // After a pair-path run that left y_n in ycur(j): bring it back to y_n_gfs.
-  void settle(hipStream_t st, int j) {
-    if (j == 1)
-      HIPCHK(hipMemcpyAsync(gd->gridfuncs.y_n_gfs, gd->gridfuncs.k_even_gfs, 2 * g.ntot * sizeof(double),
-                            hipMemcpyDeviceToDevice, st));
-  }
   // RK4 steps in a graph: the pair path needs an even count (y_n back in y_n_gfs)
   int graph_steps64() const { return pair_path ? opt.steps_per_graph + (opt.steps_per_graph & 1) : opt.steps_per_graph; }
   // ... and in the FP32 phase also a multiple of check_every (steps_per_graph is one)
@@ -865,8 +871,9 @@ struct nrpyell_solver {
   // Metric coefficients from the rfm arrays and the c^2-folded coefficient and source tables
   // from auxevol: the only reads of those arrays. Raises `bad` (cleared first) when the wave
   // speed varies along xx2.
-  void refresh_tables(hipStream_t st) {
-    launch_set_int(bad, 0, st);  // a kernel, not a memset: this runs inside the drop-in step's graph
+  // also_clear: another flag to zero with the wave-speed check's (the drop-in step's done flag)
+  void refresh_tables(hipStream_t st, int *also_clear = nullptr) {
+    launch_set_int(bad, 0, st, also_clear);  // a kernel, not a memset: this runs inside the drop-in step's graph
     launch_coef(g, &gd->rfmstruct, coef64, nullptr, st);
     launch_fold(g, gd->gridfuncs.auxevol_gfs, coef64, w2d, kc64, kc32, src64, bad, st);
     HIPCHK(hipGetLastError());
@@ -874,6 +881,14 @@ struct nrpyell_solver {
   // The fused kernels keep only the interior phi planes current; this fills the ghost planes
   // of y_n from them before the state is handed back.
   void fill_ghosts(hipStream_t st) { launch_fill_ghost_planes<double>(g, gd->gridfuncs.y_n_gfs, 2, st); }
+  // y_n_gfs handed back after steps that left the state in array j (pair path: 1 = k_even_gfs,
+  // copied back with its ghost planes filled in the same pass; 0 = y_n_gfs, ghost planes filled)
+  void finish(hipStream_t st, int j) {
+    if (j == 1)
+      launch_settle_fill(g, gd->gridfuncs.k_even_gfs, gd->gridfuncs.y_n_gfs, 2, st);
+    else
+      fill_ghosts(st);
+  }
   void step32(hipStream_t st) {
     for (int s = 1; s <= 4; s++) launch_substep<float>(args<float>(s, y32, ko32, ke32, rt32, src32, kc32, geo32), s, st);
   }
@@ -1113,12 +1128,10 @@ extern "C" int nrpyell_solver_create(griddata_struct *gd, REAL dt, const nrpyell
     s->kc64 = dalloc<double>(6 * n2);
     s->kc32 = dalloc<float>(6 * n2);
     s->src64 = dalloc<double>(2 * g.ntot);
-    s->bad = dalloc<int>(1);
+    s->bad_host = halloc_mapped<int>(1, &s->bad);
     s->refresh_tables(s->stream);
-    int hbad = 0;
-    HIPCHK(hipMemcpyAsync(&hbad, s->bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (hbad) {
+    if (*(volatile int *)s->bad_host) {
       set_last_error("the wave speed gridfunction varies along xx2; the solver needs c = c(xx0, xx1) as "
                      "wavespeed_gf_all_points sets it");
       return NRPYELL_EINVAL;
@@ -1188,11 +1201,11 @@ extern "C" int nrpyell_solver_steps(nrpyell_solver *s, int n) {
     reset_state(s);
     if (s->pair_path) {
       for (int k = 0; k < n; k++) s->step64_pair(s->stream, k & 1, false);
-      s->settle(s->stream, n & 1);
+      s->finish(s->stream, n & 1);
     } else {
       for (int k = 0; k < n; k++) s->step64(s->stream);
+      s->finish(s->stream, 0);
     }
-    s->fill_ghosts(s->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s->stream));
     return NRPYELL_OK;
@@ -1284,6 +1297,7 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
     }
     const std::vector<int> keep_s = s->hist32_step;
     const std::vector<double> keep_l = s->hist32_log10;
+    int at = 0;  // the array holding the final state (pair path)
     if (s->hstate->done == 0) {
       const nrpyell_solver::GraphKey k64 = s->graph_key(false, s->opt.log10_target);
       if (!s->graph64 || !s->key64.same(k64)) {
@@ -1305,9 +1319,9 @@ extern "C" int nrpyell_solver_solve(nrpyell_solver *s, nrpyell_solve_result *res
       steps += s->hstate->done ? s->hstate->steps : s->hstate->nchecks;
       // pair path: a check that ended the solve left its state in the array of its parity;
       // otherwise every graph ran an even number of steps
-      if (s->pair_path) s->settle(s->stream, s->hstate->done ? (s->hstate->steps & 1) : 0);
+      if (s->pair_path) at = s->hstate->done ? (s->hstate->steps & 1) : 0;
     }
-    s->fill_ghosts(s->stream);
+    s->finish(s->stream, at);
     HIPCHK(hipStreamSynchronize(s->stream));
     const auto t1 = std::chrono::steady_clock::now();
     s->hist32_step = keep_s;
@@ -1355,8 +1369,7 @@ extern "C" int nrpyell_solver_timed_steps(nrpyell_solver *s, int n, double *ms_o
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    if (s->pair_path) s->settle(s->stream, (n % per) & 1);
-    s->fill_ghosts(s->stream);
+    s->finish(s->stream, s->pair_path ? (n % per) & 1 : 0);
     read_state(s);
     *ms_out = ms;
     if (log10_out) *log10_out = s->hstate->last_log10;
@@ -1566,15 +1579,14 @@ extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL d
       s->graph_step = nullptr;
       hipGraph_t gr;
       HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      s->refresh_tables(s->stream);
-      launch_set_int(&s->dstate->done, 0, s->stream);
+      s->refresh_tables(s->stream, &s->dstate->done);
       if (s->pair_path) {
         s->step64_pair(s->stream, 0, false);
-        s->settle(s->stream, 1);
+        s->finish(s->stream, 1);
       } else {
         s->step64(s->stream);
+        s->finish(s->stream, 0);
       }
-      s->fill_ghosts(s->stream);
       HIPCHK(hipStreamEndCapture(s->stream, &gr));
       HIPCHK(hipGraphInstantiate(&s->graph_step, gr, nullptr, nullptr, 0));
       HIPCHK(hipGraphDestroy(gr));
@@ -1582,10 +1594,8 @@ extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL d
     }
     HIPCHK(hipGraphLaunch(s->graph_step, s->stream));
     HIPCHK(hipGetLastError());
-    int hbad = 0;
-    HIPCHK(hipMemcpyAsync(&hbad, s->bad, sizeof(int), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
-    if (hbad)
+    if (*(volatile int *)s->bad_host)
       fatal("MoL_step_forward_in_time",
             "the wave speed gridfunction varies along xx2; the fused step needs c = c(xx0, xx1) as "
             "wavespeed_gf_all_points sets it");
@@ -1836,8 +1846,7 @@ extern "C" int nrpyell_batch_solve(nrpyell_batch *b, nrpyell_solve_result *res, 
       nrpyell_solver *s = b->solvers[k];
       // pair path: a configuration stopped by its check holds the state it checked in the array
       // of that step count's parity; whole graphs run an even number of steps
-      if (b->pair) s->settle(b->stream, b->h_heads[4 * k] != 0.0 ? ((int)b->h_heads[4 * k + 1] & 1) : 0);
-      s->fill_ghosts(b->stream);
+      s->finish(b->stream, b->pair && b->h_heads[4 * k] != 0.0 ? ((int)b->h_heads[4 * k + 1] & 1) : 0);
     }
     HIPCHK(hipStreamSynchronize(b->stream));
     const auto t1 = std::chrono::steady_clock::now();

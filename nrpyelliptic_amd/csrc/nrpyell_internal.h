@@ -162,7 +162,8 @@ struct SubstepArgs {
 
 
 // kernel launchers (nrpyell_kernels.hip)
-void launch_set_int(int *p, int v, hipStream_t st);
+// *p = v (and *p2 = v when p2 is given): a kernel, so that it can sit in a captured graph
+void launch_set_int(int *p, int v, hipStream_t st, int *p2 = nullptr);
 void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in, double *out,
                       double eta, hipStream_t st);
 void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in,
@@ -170,6 +171,8 @@ void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const doubl
 void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st);
 template <typename T>
 void launch_fill_ghost_planes(const GridDims &g, T *y, int ngf, hipStream_t st);
+// dst = src with dst's ghost phi planes filled from their sources (copy + fill_ghost_planes)
+void launch_settle_fill(const GridDims &g, const double *src, double *dst, int ngf, hipStream_t st);
 void launch_fold(const GridDims &g, const double *aux, const double *coef, const double *w2d, double *kc64,
                  float *kc32, double *src, int *bad, hipStream_t st);
 template <typename T>

@@ -366,11 +366,14 @@ void launch_l2_finalize(const double *partial, long npart, double volume, Solver
 // apply_bcs_inner_only.c:18-26: gfs[dst] = parity[0] * gfs[src] for both evolved gfs.
 // *p = v on the stream: a kernel where a memset would do, because this is recorded into graphs
 // (the drop-in MoL step's), and memset nodes are not used in them.
-__global__ void set_int_kernel(int *p, int v) {
+__global__ void set_int_kernel(int *p, int *p2, int v) {
   if (threadIdx.x == 0) *p = v;
+  if (threadIdx.x == 1 && p2) *p2 = v;
 }
 
-void launch_set_int(int *p, int v, hipStream_t st) { hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, st, p, v); }
+void launch_set_int(int *p, int v, hipStream_t st, int *p2) {
+  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, st, p, p2, v);
+}
 
 __global__ void inner_gather_kernel(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -680,6 +683,28 @@ void launch_fill_ghost_planes(const GridDims &g, T *y, int ngf, hipStream_t st) 
 }
 template void launch_fill_ghost_planes<double>(const GridDims &, double *, int, hipStream_t);
 template void launch_fill_ghost_planes<float>(const GridDims &, float *, int, hipStream_t);
+
+// dst = src at every point, except that a point of a ghost phi plane takes src at its source
+// (ghost_source): a device copy followed by fill_ghost_planes_kernel on dst, in one pass. The
+// sources lie in interior planes, where the copy leaves dst equal to src.
+template <typename T>
+__global__ void settle_fill_kernel(GridDims g, const T *__restrict__ src, T *__restrict__ dst, int ngf) {
+  const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.ntot) return;
+  const int plane = g.G0 * g.G1;
+  const int i2 = (int)(k / plane), r = (int)(k - (long)i2 * plane), i1 = r / g.G0, i0 = r - i1 * g.G0;
+  long from = k;
+  if (i2 < NG || i2 >= g.N2 + NG) {
+    int s0, s1, s2;
+    ghost_source(g, i0, i1, i2, s0, s1, s2);
+    from = idx3(g, s0, s1, s2);
+  }
+  for (int f = 0; f < ngf; f++) dst[f * g.ntot + k] = src[f * g.ntot + from];
+}
+void launch_settle_fill(const GridDims &g, const double *src, double *dst, int ngf, hipStream_t st) {
+  hipLaunchKernelGGL(settle_fill_kernel<double>, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, g, src,
+                     dst, ngf);
+}
 
 __global__ void fill_kernel(long n, double v, double *out) {
   const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
