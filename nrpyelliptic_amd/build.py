@@ -25,6 +25,9 @@ HIPCC_FLAGS = [
     "-ffp-contract=off",
     # keep the compiler from emitting packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # register-pressure tracking of the AMDGPU scheduler: the stages-3+4 two-substep kernel at
+    # 168 VGPRs spills 6 instead of 10 and runs 9.5 instead of 9.8 us at 128x128x16
+    "-mllvm", "-amdgpu-use-amdgpu-trackers",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     "-Wl,-Bsymbolic",
     "-Wl,--no-undefined",
