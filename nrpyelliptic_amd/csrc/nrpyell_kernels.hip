@@ -39,18 +39,11 @@ __device__ inline double fd2_exact(const double m[6], const double p[6], double 
                  fma(1.0 / 3150.0, m[5] + p[5], -fma(5.0 / 1008.0, m[4] + p[4], fma(5.0 / 21.0, m[2] + p[2], c0)))));
 }
 
-// Flat-space Laplacian in SinhSymTP plus AA^2/(8 (psi+u)^7), reference expression tree.
-__device__ inline double lap_source_exact(const GridDims &g, const RfmPtrs &r, const double *u, long p, int i0,
-                                          int i1, double psi, double AA) {
-  double m0[6], p0[6], m1[6], p1[6], m2[6], p2[6];
-  const long s1 = g.G0, s2 = (long)g.G0 * g.G1;
-#pragma unroll
-  for (int k = 1; k <= 5; k++) {
-    m0[k] = u[p - k]; p0[k] = u[p + k];
-    m1[k] = u[p - k * s1]; p1[k] = u[p + k * s1];
-    m2[k] = u[p - k * s2]; p2[k] = u[p + k * s2];
-  }
-  const double uu = u[p];
+// Flat-space Laplacian in SinhSymTP plus AA^2/(8 (psi+u)^7), reference expression tree, from the
+// stencil arms m*/p* (u at offsets -k / +k along x0, x1, x2) and the centre value uu.
+__device__ inline double lap_source_exact(const GridDims &g, const RfmPtrs &r, const double m0[6], const double p0[6],
+                                          const double m1[6], const double p1[6], const double m2[6],
+                                          const double p2[6], double uu, int i0, int i1, double psi, double AA) {
   const double c0 = (5269.0 / 1800.0) * uu;
   const double dD0 = g.invdx0 * fd1_exact(m0, p0);
   const double dD1 = g.invdx1 * fd1_exact(m1, p1);
@@ -80,15 +73,63 @@ __device__ inline double lap_source_exact(const GridDims &g, const RfmPtrs &r, c
   return e;
 }
 
+// The drop-in kernels work on 64 x 4 tiles of one plane: the tile's u with its 5-point halos in
+// x0 and x1 is staged in LDS (the x0 and x1 arms of every point are read from there), the x2
+// arms come from global memory. Only where the values come from differs from a per-point
+// gather: each point's arithmetic is the reference's expression tree.
+#define EX_TX 64
+#define EX_TY 4
+#define EX_BX (EX_TX + 2 * NG)
+#define EX_BY (EX_TY + 2 * NG)
+
+struct ExactTile {
+  int i0, i1, i2;
+  long p;
+  bool valid;
+};
+
+// Stage the tile's box (rows i1s-5 .. i1s+8, columns i0s-5 .. i0s+68, clamped to the grid: a
+// clamped value is never read by a point of the grid) and wait for the whole block.
+__device__ __forceinline__ ExactTile exact_tile(const GridDims &g, const double *u, double *box) {
+  const int i0s = NG + blockIdx.x * EX_TX, i1s = NG + blockIdx.y * EX_TY, i2 = NG + blockIdx.z;
+  const long base = (long)i2 * g.G0 * g.G1;
+  for (int k = threadIdx.y * EX_TX + threadIdx.x; k < EX_BX * EX_BY; k += EX_TX * EX_TY) {
+    const int bx = k % EX_BX, by = k / EX_BX;
+    const int j0 = min(i0s - NG + bx, g.G0 - 1), j1 = min(i1s - NG + by, g.G1 - 1);
+    box[k] = u[base + (long)j1 * g.G0 + j0];
+  }
+  __syncthreads();
+  ExactTile t;
+  t.i0 = i0s + (int)threadIdx.x;
+  t.i1 = i1s + (int)threadIdx.y;
+  t.i2 = i2;
+  t.p = idx3(g, t.i0, t.i1, i2);
+  t.valid = t.i0 < g.N0 + NG && t.i1 < g.N1 + NG;
+  return t;
+}
+
+__device__ __forceinline__ double exact_point(const GridDims &g, const RfmPtrs &r, const double *u, const double *box,
+                                              const ExactTile &t, double psi, double AA) {
+  double m0[6], p0[6], m1[6], p1[6], m2[6], p2[6];
+  const int c = ((int)threadIdx.y + NG) * EX_BX + (int)threadIdx.x + NG;
+  const long s2 = (long)g.G0 * g.G1;
+#pragma unroll
+  for (int k = 1; k <= 5; k++) {
+    m0[k] = box[c - k]; p0[k] = box[c + k];
+    m1[k] = box[c - k * EX_BX]; p1[k] = box[c + k * EX_BX];
+    m2[k] = u[t.p - k * s2]; p2[k] = u[t.p + k * s2];
+  }
+  return lap_source_exact(g, r, m0, p0, m1, p1, m2, p2, box[c], t.i0, t.i1, psi, AA);
+}
+
 __global__ void __launch_bounds__(256) rhs_exact_kernel(GridDims g, RfmPtrs r, const double *aux, const double *in,
                                                         double *out, double eta) {
-  const int i0 = NG + blockIdx.x * blockDim.x + threadIdx.x;
-  const int i1 = NG + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i2 = NG + blockIdx.z;
-  if (i0 >= g.N0 + NG || i1 >= g.N1 + NG) return;
-  const long p = idx3(g, i0, i1, i2), nt = g.ntot;
+  __shared__ double box[EX_BX * EX_BY];
+  const ExactTile t = exact_tile(g, in, box);
+  if (!t.valid) return;
+  const long p = t.p, nt = g.ntot;
   const double *u = in, *v = in + nt;
-  const double e = lap_source_exact(g, r, u, p, i0, i1, aux[PSI_BACKGROUNDGF * nt + p], aux[ADD_TIMES_AUUGF * nt + p]);
+  const double e = exact_point(g, r, u, box, t, aux[PSI_BACKGROUNDGF * nt + p], aux[ADD_TIMES_AUUGF * nt + p]);
   const double c = aux[WAVESPEEDGF * nt + p];
   out[p] = fma(-eta, u[p], v[p]);
   out[nt + p] = (c * c) * e;
@@ -96,19 +137,18 @@ __global__ void __launch_bounds__(256) rhs_exact_kernel(GridDims g, RfmPtrs r, c
 
 __global__ void __launch_bounds__(256) residual_exact_kernel(GridDims g, RfmPtrs r, const double *aux,
                                                              const double *in, double *out) {
-  const int i0 = NG + blockIdx.x * blockDim.x + threadIdx.x;
-  const int i1 = NG + blockIdx.y * blockDim.y + threadIdx.y;
-  const int i2 = NG + blockIdx.z;
-  if (i0 >= g.N0 + NG || i1 >= g.N1 + NG) return;
-  const long p = idx3(g, i0, i1, i2), nt = g.ntot;
-  out[p] = lap_source_exact(g, r, in, p, i0, i1, aux[PSI_BACKGROUNDGF * nt + p], aux[ADD_TIMES_AUUGF * nt + p]);
+  __shared__ double box[EX_BX * EX_BY];
+  const ExactTile t = exact_tile(g, in, box);
+  if (!t.valid) return;
+  const long p = t.p, nt = g.ntot;
+  out[p] = exact_point(g, r, in, box, t, aux[PSI_BACKGROUNDGF * nt + p], aux[ADD_TIMES_AUUGF * nt + p]);
 }
 
 void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in, double *out,
                       double eta, hipStream_t st) {
   RfmPtrs r{rfm->f0_of_xx0, rfm->f0_of_xx0__D0, rfm->f0_of_xx0__DD00, rfm->f1_of_xx1, rfm->f1_of_xx1__D1,
             rfm->f2_of_xx0_xx1, rfm->f2_of_xx0_xx1__D0, rfm->f3_of_xx0, rfm->f3_of_xx0__D0};
-  dim3 blk(64, 4), grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2);
+  dim3 blk(EX_TX, EX_TY), grd((g.N0 + EX_TX - 1) / EX_TX, (g.N1 + EX_TY - 1) / EX_TY, g.N2);
   if (out)
     hipLaunchKernelGGL(rhs_exact_kernel, grd, blk, 0, st, g, r, aux, in, out, eta);
 }
@@ -117,7 +157,7 @@ void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const doubl
                            double *out, hipStream_t st) {
   RfmPtrs r{rfm->f0_of_xx0, rfm->f0_of_xx0__D0, rfm->f0_of_xx0__DD00, rfm->f1_of_xx1, rfm->f1_of_xx1__D1,
             rfm->f2_of_xx0_xx1, rfm->f2_of_xx0_xx1__D0, rfm->f3_of_xx0, rfm->f3_of_xx0__D0};
-  dim3 blk(64, 4), grd((g.N0 + 63) / 64, (g.N1 + 3) / 4, g.N2);
+  dim3 blk(EX_TX, EX_TY), grd((g.N0 + EX_TX - 1) / EX_TX, (g.N1 + EX_TY - 1) / EX_TY, g.N2);
   hipLaunchKernelGGL(residual_exact_kernel, grd, blk, 0, st, g, r, aux, in, out);
 }
 
