@@ -360,6 +360,11 @@ int nrpyell_solver_history(nrpyell_solver *s, int max, int *steps, double *log10
  * or a launch that does not fit one round of resident workgroups). No counterpart in the
  * reference (a property of this GPU path): for tools that inspect the placement. */
 int nrpyell_solver_block_order(const nrpyell_solver *s, unsigned short *out, int max);
+/* The same placement as planned for a grid of N0 x N1 x N2 interior points with tiles of tile_i0
+ * (16, 32, 64) x 256/tile_i0 points x `planes` (1, or 2 with tile_i0 >= 32) on a GPU of ncu CUs
+ * in 8 XCDs, without a GPU: out[L] as above; returns the block count, 0 for the kernel's own
+ * order, NRPYELL_EINVAL for invalid arguments. No counterpart in the reference. */
+int nrpyell_block_order_plan(int ncu, int N0, int N1, int N2, int tile_i0, int planes, unsigned short *out, int max);
 
 /* Batched FP64 solver: n configurations, each with its own griddata (any grid shape), relaxed
  * together -- one launch per RK4 substep per tile width covers every configuration, and each

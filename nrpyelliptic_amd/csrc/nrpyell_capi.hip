@@ -920,14 +920,11 @@ struct nrpyell_solver {
 // kernel's own order 21.5; BC blocks placed like interior blocks, on the CUs that hold two, 20.6;
 // the same split as here with the first tile after the other tile 21.3; a weight of +0.15 for
 // tiles on the first or last i1 row 20.3-20.45; this one 20.05-20.1.
-static void pair_block_order(nrpyell_solver *s) {
-  s->order.n = 0;
-  if (!s->pair_path) return;
-  int dev = 0, ncu = 0;
-  HIPCHK(hipGetDevice(&dev));
-  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int nbc_rows = (s->g.N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS;
-  const int ng = s->g.N2 / s->pl, nb = nbc_rows * s->g.N2, ni = s->ntile0 * s->ntile1 * ng, n = nb + ni;
+// plan_block_order fills `order` for a launch of nb BC blocks and ntile0 x ntile1 x ng interior
+// blocks on a GPU of ncu CUs (order.n = 0: the kernel's own order).
+static void plan_block_order(int ncu, int nb, int ntile0, int ntile1, int ng, BlockOrder &order) {
+  order = BlockOrder{};
+  const int ni = ntile0 * ntile1 * ng, n = nb + ni;
   const int P = ncu / 8;
   // one round of resident blocks (3 per CU: __launch_bounds__(256, 3), 37 KB of LDS each)
   if (ncu % 8 != 0 || P <= 0 || n > NRPYELL_ORDER_MAX || n > 3 * ncu || nb % 8 || ni % 8) return;
@@ -941,8 +938,8 @@ static void pair_block_order(nrpyell_solver *s) {
   for (int x = 0; x < 8; x++) {
     std::vector<Item> mem;
     for (int q = x * per; q < (x + 1) * per; q++) {
-      const int t0 = (q / ng) % s->ntile0;
-      const int rank = t0 == s->ntile0 - 1 ? 0 : t0 == 0 ? 1 : 2;
+      const int t0 = (q / ng) % ntile0;
+      const int rank = t0 == ntile0 - 1 ? 0 : t0 == 0 ? 1 : 2;
       mem.push_back({rank == 0 ? 1.1 : rank == 1 ? 0.85 : 1.0, nb + q, rank});
     }
     std::stable_sort(mem.begin(), mem.end(), [](const Item &a, const Item &b) { return a.w > b.w; });
@@ -951,8 +948,8 @@ static void pair_block_order(nrpyell_solver *s) {
     std::vector<int> cap(P, 0);
     for (int j = nbx; j < nx; j++) cap[j % P]++;  // positions left after the BC blocks
     for (int k = 0; k < nbx; k++) {
-      load[k] = 1.25;                                   // position k, CU k
-      s->order.set(8 * k + x, (unsigned)(8 * k + x));  // BC block 8 k + x: its own order
+      load[k] = 1.25;                                // position k, CU k
+      order.set(8 * k + x, (unsigned)(8 * k + x));  // BC block 8 k + x: its own order
     }
     for (const Item &m : mem) {
       int best = -1;
@@ -985,10 +982,36 @@ static void pair_block_order(nrpyell_solver *s) {
         return a.rank != b.rank ? a.rank < b.rank : a.logical < b.logical;
       });
       int k = 0;
-      for (int j = c < nbx ? c + P : c; j < nx; j += P) s->order.set(8 * j + x, (unsigned)on[c][k++].logical);
+      for (int j = c < nbx ? c + P : c; j < nx; j += P) order.set(8 * j + x, (unsigned)on[c][k++].logical);
     }
   }
-  s->order.n = n;
+  order.n = n;
+}
+
+static void pair_block_order(nrpyell_solver *s) {
+  s->order = BlockOrder{};
+  if (!s->pair_path) return;
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int nbc_rows = (s->g.N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS;
+  plan_block_order(ncu, nbc_rows * s->g.N2, s->ntile0, s->ntile1, s->g.N2 / s->pl, s->order);
+}
+
+extern "C" int nrpyell_block_order_plan(int ncu, int N0, int N1, int N2, int tile_i0, int planes, unsigned short *out,
+                                        int max_entries) {
+  if (ncu <= 0 || N0 <= 0 || N1 <= 0 || N2 <= 0 || (tile_i0 != 16 && tile_i0 != 32 && tile_i0 != 64) ||
+      !(planes == 1 || (planes == 2 && tile_i0 >= 32)) || N2 % planes != 0 || max_entries < 0) {
+    set_last_error("nrpyell_block_order_plan: invalid arguments");
+    return NRPYELL_EINVAL;
+  }
+  const int ty = NRPYELL_BLOCK / tile_i0;
+  BlockOrder o;
+  plan_block_order(ncu, (N1 + NRPYELL_PAIR_BC_ROWS - 1) / NRPYELL_PAIR_BC_ROWS * N2, (N0 + tile_i0 - 1) / tile_i0,
+                   (N1 + ty - 1) / ty, N2 / planes, o);
+  const int n = !out ? 0 : o.n < max_entries ? o.n : max_entries;
+  for (int i = 0; i < n; i++) out[i] = (unsigned short)o.at(i);
+  return o.n;
 }
 
 extern "C" int nrpyell_solver_block_order(const nrpyell_solver *s, unsigned short *out, int max_entries) {

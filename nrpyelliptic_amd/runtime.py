@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
     L.nrpyell_solver_history.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.nrpyell_solver_substeps_per_launch.argtypes = [vp]
     L.nrpyell_solver_block_order.argtypes = [vp, C.POINTER(C.c_ushort), C.c_int]
+    L.nrpyell_block_order_plan.argtypes = [C.c_int] * 6 + [C.POINTER(C.c_ushort), C.c_int]
     L.nrpyell_set_eta_damping.argtypes = [C.POINTER(abi.griddata_struct), C.c_double]
     L.nrpyell_batch_create.argtypes = [C.POINTER(C.POINTER(abi.griddata_struct)), C.POINTER(C.c_double),
                                        C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int, C.POINTER(solve_options),
@@ -144,6 +145,18 @@ def default_options(**kw) -> solve_options:
             v = names[k][v]
         setattr(o, k, v)
     return o
+
+
+def block_order_plan(ncu: int, Nxx, tile_i0: int = 32, planes: int = 2) -> np.ndarray:
+    """Placement of the two-substep kernel's blocks that a solver plans for a grid of Nxx
+    interior points on a GPU of ncu CUs (entry L: the block workgroup L computes, BC blocks
+    first); empty when the kernel keeps its own order. Host computation only."""
+    buf = (C.c_ushort * 1024)()
+    n = lib().nrpyell_block_order_plan(int(ncu), int(Nxx[0]), int(Nxx[1]), int(Nxx[2]), int(tile_i0), int(planes),
+                                       buf, 1024)
+    if n < 0:
+        check(n, "block_order_plan")
+    return np.array(buf[:n], dtype=np.int64)
 
 
 class DeviceArray:

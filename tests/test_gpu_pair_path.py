@@ -79,6 +79,7 @@ def test_pair_block_order_places_every_block_once():
         ntile0, ntile1, ng = 128 // 32, 128 // 8, 16 // 2
         nb, ni = (128 // 16) * 16, ntile0 * ntile1 * ng
         assert len(o) == nb + ni and np.array_equal(np.sort(o), np.arange(nb + ni))
+        assert np.array_equal(o, runtime.block_order_plan(256, (128, 128, 16), 32, 2))  # MI355X: 256 CUs
         assert np.array_equal(o[:nb], np.arange(nb))
         L = np.arange(nb, nb + ni)
         assert np.array_equal((o[L] - nb) // (ni // 8), L % 8)
