@@ -872,10 +872,9 @@ struct nrpyell_solver {
   // from auxevol: the only reads of those arrays. Raises `bad` (cleared first) when the wave
   // speed varies along xx2.
   // also_clear: another flag to zero with the wave-speed check's (the drop-in step's done flag)
+  // (the flags are zeroed by a kernel, not a memset: this runs inside the drop-in step's graph)
   void refresh_tables(hipStream_t st, int *also_clear = nullptr) {
-    launch_set_int(bad, 0, st, also_clear);  // a kernel, not a memset: this runs inside the drop-in step's graph
-    launch_coef(g, &gd->rfmstruct, coef64, nullptr, st);
-    launch_fold(g, gd->gridfuncs.auxevol_gfs, coef64, w2d, kc64, kc32, src64, bad, st);
+    launch_tables(g, &gd->rfmstruct, gd->gridfuncs.auxevol_gfs, w2d, coef64, kc64, kc32, src64, bad, also_clear, st);
     HIPCHK(hipGetLastError());
   }
   // The fused kernels keep only the interior phi planes current; this fills the ghost planes

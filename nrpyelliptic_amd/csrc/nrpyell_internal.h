@@ -162,19 +162,19 @@ struct SubstepArgs {
 
 
 // kernel launchers (nrpyell_kernels.hip)
-// *p = v (and *p2 = v when p2 is given): a kernel, so that it can sit in a captured graph
-void launch_set_int(int *p, int v, hipStream_t st, int *p2 = nullptr);
 void launch_rhs_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in, double *out,
                       double eta, hipStream_t st);
 void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *in,
                            double *out, hipStream_t st);
-void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st);
+// the solver's tables from rfm and auxevol: metric coefficients (coef64), their c^2-folded record
+// with the L2 weight (kc64, kc32), the per-point source (src); *bad raised where the wave speed
+// varies along xx2; *bad and *also_clear (when given) zeroed first, by the same launch
+void launch_tables(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *w2d, double *coef64,
+                   double *kc64, float *kc32, double *src, int *bad, int *also_clear, hipStream_t st);
 template <typename T>
 void launch_fill_ghost_planes(const GridDims &g, T *y, int ngf, hipStream_t st);
 // dst = src with dst's ghost phi planes filled from their sources (copy + fill_ghost_planes)
 void launch_settle_fill(const GridDims &g, const double *src, double *dst, int ngf, hipStream_t st);
-void launch_fold(const GridDims &g, const double *aux, const double *coef, const double *w2d, double *kc64,
-                 float *kc32, double *src, int *bad, hipStream_t st);
 template <typename T>
 void launch_substep(const SubstepArgs<T> &a, int stage, hipStream_t st);
 // two RK4 substeps (first_stage 1: stages 1+2; 3: stages 3+4) in one launch (nrpyell_substep2.hip)

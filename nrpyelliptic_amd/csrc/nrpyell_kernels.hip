@@ -1,16 +1,17 @@
-// nrpyell_kernels.hip -- gfx950 kernels of the hyperbolic-relaxation hot path.
-//
-// Kernels (all one thread per grid point, i0 across lanes so every load is coalesced):
-//   rhs_exact_kernel        rhs_eval.c:7-117 with the reference FMA tree (drop-in rhs_eval)
-//   residual_exact_kernel   residual_all_points.c:7-112 (drop-in residual_all_points)
-//   substep_kernel<T,S>     one RK4 substep fused: stencil RHS + radiation BC on the outer
-//                           ghost layers + stage update + parity-image writes of the ghost
-//                           zones (MoL_step_forward_in_time.c:11-176 with
-//                           apply_bcs_outerradiation_and_inner.c:369-469 folded in)
-//   residual_l2_kernel<T>   residual + masked, volume-weighted sum of squares, per-block
-//                           partials reduced in a fixed order (L2_norm_of_gf.c:6-43)
-//   radiation_bc_kernel, inner_gather_kernel   drop-in apply_bcs_* entry points
-//   setup kernels           rfm factors, auxevol (psi_bg, AA^2), wavespeed, coefficient tables
+// nrpyell_kernels.hip -- gfx950 kernels of the hyperbolic-relaxation hot path other than the fused
+// RK4 substeps (nrpyell_substep.hip, nrpyell_substep2.hip). One thread per grid point or list
+// entry, i0 across lanes so that loads coalesce:
+//   rhs_exact_kernel, residual_exact_kernel   rhs_eval.c:7-117, residual_all_points.c:7-112 with
+//                           the reference FMA tree (drop-in entry points; 64 x 4 tiles, x0/x1
+//                           stencil arms staged in LDS)
+//   coef_fold_kernel, fold_source_kernel      the solver's tables from rfm and auxevol
+//   residual_l2_partial_kernel, l2_partial_kernel, l2_finalize_kernel   residual + masked,
+//                           volume-weighted sum of squares, partials reduced in a fixed order
+//                           (L2_norm_of_gf.c:6-43)
+//   radiation_rows_kernel, radiation_list_kernel, sommerfeld_list_kernel, inner_gather_kernel
+//                           drop-in apply_bcs_* entry points and the thorn's Sommerfeld BC
+//   rfm_kernel, auxevol_kernel, wavespeed_kernel   set-up producers
+//   fill_ghost_planes_kernel, settle_fill_kernel, convert, fill, shift_v   state handling
 //
 // Built with -ffp-contract=off: every fused multiply-add below is an explicit fma().
 #include <hip/hip_runtime.h>
@@ -173,10 +174,18 @@ void launch_residual_exact(const GridDims &g, const rfm_struct *rfm, const doubl
 //   B   = f3 (f3 (f0' d0f2 + f0'' f2) - f0' f2 f3')/(f0'^3 f2^3) - d0f2 f3^2/(f0'^2 f2^3)
 //         - f3^2/(f0 f0' f2^2)
 // ---------------------------------------------------------------------------------------
-template <typename T>
-__global__ void coef_kernel(GridDims g, RfmPtrs r, T *coef) {
+// coef_fold_kernel writes K00 .. K0 (coef, read by the residual check) and the same five times
+// c^2 plus the L2 weight / c^4 (kc64, kc32: the fused kernels' per-(i0,i1) record, c the wave
+// speed on the plane i2 = NG), and zeroes the flags it is given (thread 0 of block (0, 0)):
+// the wave-speed check's, which fold_source_kernel then raises, and the caller's.
+__global__ void coef_fold_kernel(GridDims g, RfmPtrs r, const double *aux, const double *w2d, double *coef,
+                                 double *kc64, float *kc32, int *flag0, int *flag1) {
   const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
   const int i1 = blockIdx.y;
+  if (i0 == 0 && i1 == 0) {
+    *flag0 = 0;
+    if (flag1) *flag1 = 0;
+  }
   if (i0 >= g.G0) return;
   const long k2 = i0 + (long)g.G0 * i1, n2 = (long)g.G0 * g.G1;
   const double f0 = r.f0[i0], f0p = r.f0_D0[i0], f0pp = r.f0_DD00[i0];
@@ -186,19 +195,43 @@ __global__ void coef_kernel(GridDims g, RfmPtrs r, T *coef) {
   const double a0 = 1.0 / (f2 * f2), a2 = (f3 * f3) / (f0p * f0p), a3 = a0 / f2;
   const double B = f3 * a3 * (f3 * (f0p * f2_0 + f0pp * f2) - f0p * f2 * f3p) / (f0p * f0p * f0p) -
                    a2 * a3 * f2_0 - a0 * (f3 * f3) / (f0p * f0);
-  coef[0 * n2 + k2] = (T)(a0 * a2 * (g.invdx0 * g.invdx0));
-  coef[1 * n2 + k2] = (T)(a0 * (g.invdx1 * g.invdx1));
-  coef[2 * n2 + k2] = (T)((1.0 / (f1 * f1)) / (f0 * f0) * (g.invdx2 * g.invdx2));
-  coef[3 * n2 + k2] = (T)(a0 * f1p / f1 * g.invdx1);
-  coef[4 * n2 + k2] = (T)(-B * g.invdx0);
+  const double K[5] = {a0 * a2 * (g.invdx0 * g.invdx0), a0 * (g.invdx1 * g.invdx1),
+                       (1.0 / (f1 * f1)) / (f0 * f0) * (g.invdx2 * g.invdx2), a0 * f1p / f1 * g.invdx1,
+                       -B * g.invdx0};
+  const double c = aux[WAVESPEEDGF * g.ntot + idx3(g, i0, i1, NG)], c2 = c * c;
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    coef[k * n2 + k2] = K[k];
+    const double v = c2 * K[k];
+    kc64[6 * k2 + k] = v;
+    kc32[6 * k2 + k] = (float)v;
+  }
+  const double w = c2 > 0.0 ? w2d[k2] / (c2 * c2) : 0.0;
+  kc64[6 * k2 + 5] = w;
+  kc32[6 * k2 + 5] = (float)w;
 }
 
-void launch_coef(const GridDims &g, const rfm_struct *rfm, double *coef64, float *coef32, hipStream_t st) {
+// (psi, c^2 AA^2 / 8) interleaved per point (one 16-byte load in the fused kernels); raises *bad
+// where the wave speed at an interior point differs from its value on the plane i2 = NG.
+__global__ void fold_source_kernel(GridDims g, const double *aux, double *src, int *bad) {
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x, nt = g.ntot;
+  if (p >= nt) return;
+  const int plane = g.G0 * g.G1;
+  const int i2 = (int)(p / plane), rem = (int)(p - (long)i2 * plane), i1 = rem / g.G0, i0 = rem - i1 * g.G0;
+  const double c = aux[WAVESPEEDGF * nt + p];
+  src[2 * p] = aux[PSI_BACKGROUNDGF * nt + p];
+  src[2 * p + 1] = 0.125 * (c * c) * aux[ADD_TIMES_AUUGF * nt + p];
+  const bool inside = i0 >= NG && i0 < g.N0 + NG && i1 >= NG && i1 < g.N1 + NG && i2 >= NG && i2 < g.N2 + NG;
+  if (inside && c != aux[WAVESPEEDGF * nt + (p - (long)(i2 - NG) * plane)]) *bad = 1;
+}
+
+void launch_tables(const GridDims &g, const rfm_struct *rfm, const double *aux, const double *w2d, double *coef64,
+                   double *kc64, float *kc32, double *src, int *bad, int *also_clear, hipStream_t st) {
   RfmPtrs r{rfm->f0_of_xx0, rfm->f0_of_xx0__D0, rfm->f0_of_xx0__DD00, rfm->f1_of_xx1, rfm->f1_of_xx1__D1,
             rfm->f2_of_xx0_xx1, rfm->f2_of_xx0_xx1__D0, rfm->f3_of_xx0, rfm->f3_of_xx0__D0};
-  dim3 blk(64), grd((g.G0 + 63) / 64, g.G1);
-  if (coef64) hipLaunchKernelGGL(coef_kernel<double>, grd, blk, 0, st, g, r, coef64);
-  if (coef32) hipLaunchKernelGGL(coef_kernel<float>, grd, blk, 0, st, g, r, coef32);
+  hipLaunchKernelGGL(coef_fold_kernel, dim3((g.G0 + 63) / 64, g.G1), dim3(64), 0, st, g, r, aux, w2d, coef64, kc64,
+                     kc32, bad, also_clear);
+  hipLaunchKernelGGL(fold_source_kernel, dim3((unsigned)((g.ntot + 255) / 256)), dim3(256), 0, st, g, aux, src, bad);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -404,17 +437,6 @@ void launch_l2_finalize(const double *partial, long npart, double volume, Solver
 // Drop-in boundary kernels.
 // ---------------------------------------------------------------------------------------
 // apply_bcs_inner_only.c:18-26: gfs[dst] = parity[0] * gfs[src] for both evolved gfs.
-// *p = v on the stream: a kernel where a memset would do, because this is recorded into graphs
-// (the drop-in MoL step's), and memset nodes are not used in them.
-__global__ void set_int_kernel(int *p, int *p2, int v) {
-  if (threadIdx.x == 0) *p = v;
-  if (threadIdx.x == 1 && p2) *p2 = v;
-}
-
-void launch_set_int(int *p, int v, hipStream_t st, int *p2) {
-  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, st, p, p2, v);
-}
-
 __global__ void inner_gather_kernel(long nt, int n, const int *dst, const int *src, const int8_t *par0, double *gfs) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
@@ -672,37 +694,6 @@ void launch_convert_f2d(long n, const float *in, double *out, hipStream_t st) {
 // differs from that of its (i0,i1) column on plane NG raises *bad.
 // kc[6 k2 + j]: j < 5 the folded coefficients, j = 5 the folded weight, per (i0,i1) in one
 // 48-byte record (three 16-byte loads in the FP64 kernel).
-__global__ void fold_coef_kernel(GridDims g, const double *aux, const double *coef, const double *w2d, double *kc64,
-                                 float *kc32) {
-  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, i1 = blockIdx.y;
-  if (i0 >= g.G0) return;
-  const long k2 = i0 + (long)g.G0 * i1, n2 = (long)g.G0 * g.G1;
-  const double c = aux[WAVESPEEDGF * g.ntot + idx3(g, i0, i1, NG)], c2 = c * c;
-  for (int k = 0; k < 5; k++) {
-    const double v = c2 * coef[k * n2 + k2];
-    kc64[6 * k2 + k] = v;
-    kc32[6 * k2 + k] = (float)v;
-  }
-  const double w = c2 > 0.0 ? w2d[k2] / (c2 * c2) : 0.0;
-  kc64[6 * k2 + 5] = w;
-  kc32[6 * k2 + 5] = (float)w;
-}
-__global__ void fold_source_kernel(GridDims g, const double *aux, double *src, int *bad) {
-  const int i0 = blockIdx.x * blockDim.x + threadIdx.x, i1 = blockIdx.y, i2 = blockIdx.z;
-  if (i0 >= g.G0) return;
-  const long p = idx3(g, i0, i1, i2), nt = g.ntot;
-  const double c = aux[WAVESPEEDGF * nt + p];
-  src[2 * p] = aux[PSI_BACKGROUNDGF * nt + p];                    // interleaved per point:
-  src[2 * p + 1] = 0.125 * (c * c) * aux[ADD_TIMES_AUUGF * nt + p];  // one 16-byte load
-  const bool inside = i0 >= NG && i0 < g.N0 + NG && i1 >= NG && i1 < g.N1 + NG && i2 >= NG && i2 < g.N2 + NG;
-  if (inside && c != aux[WAVESPEEDGF * nt + idx3(g, i0, i1, NG)]) *bad = 1;
-}
-void launch_fold(const GridDims &g, const double *aux, const double *coef, const double *w2d, double *kc64,
-                 float *kc32, double *src, int *bad, hipStream_t st) {
-  hipLaunchKernelGGL(fold_coef_kernel, dim3((g.G0 + 63) / 64, g.G1), dim3(64), 0, st, g, aux, coef, w2d, kc64, kc32);
-  hipLaunchKernelGGL(fold_source_kernel, dim3((g.G0 + 63) / 64, g.G1, g.G2), dim3(64), 0, st, g, aux, src, bad);
-}
-
 // Ghost phi planes (i2 < NG, i2 >= N2+NG) of the first ngf gridfunctions of y from the
 // interior planes: every point takes its source's value (ghost_source; parity +1 for u and v).
 // Sources lie in interior planes, so no thread reads what another writes.
