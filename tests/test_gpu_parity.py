@@ -121,6 +121,27 @@ def test_l2_norm(pair):
     assert abs(P.l2(gf) - O.l2(gf)) <= 1e-13
 
 
+def test_l2_norm_back_to_back_calls(pair):
+    """L2_norm_of_gf's result is written by the device into host memory and read after the
+    stream synchronises: back-to-back calls on different device gridfunctions and radii (no
+    upload, so no other synchronisation, in between) each return their own value."""
+    import ctypes as C
+    from nrpyelliptic_amd import runtime
+    P, O = pair
+    rng = np.random.default_rng(11)
+    gfs = [rng.uniform(-1, 1, P.ntot) * (k + 1) for k in range(4)]
+    dev = [runtime.DeviceArray(P.ntot).upload(gf) for gf in gfs]
+    L = runtime.lib()
+    try:
+        for radius in (100.0, 3.0):
+            got = [L.L2_norm_of_gf(C.byref(P.gd.params), 0, radius, P.gd.xx, d.as_real_p()) for d in dev]
+            for gf, v in zip(gfs, got):
+                assert abs(v - O.l2(gf, radius)) <= 1e-13 * max(1.0, abs(v)), (radius, v)
+    finally:
+        for d in dev:
+            d.free()
+
+
 @pytest.mark.parametrize("nsteps", [1, 20])
 def test_mol_step_matches_oracle(pair, nsteps):
     """MoL_step_forward_in_time (the fused kernels: two launches of two substeps where the grid
