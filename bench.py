@@ -326,6 +326,28 @@ def nn_seeded_paths(P, n_held_out=256):
             "calibration_seconds": tune_s, "bench_problem": bench, "held_out_batched": held}
 
 
+def larger_grids(sizes=((256, 256, 32), (512, 256, 32)), steps=200, reps=3):
+    """The production FP64 path (two-substep kernel, residual every step, graph replay) on grids
+    larger than the bench problem (one seed-3 configuration each): microseconds per RK4 step and
+    the rate by the same 92 B/pt-per-substep accounting as the roofline object."""
+    from nrpyelliptic_amd import config, runtime
+    out = {}
+    for Nxx in sizes:
+        P = runtime.Problem(config.random_batch(1, seed=3, grids=False, Nxx=Nxx)[0])
+        try:
+            P.zero_guess()
+            S = P.solver(steps_per_graph=20)
+            best = min(1e3 * S.timed_steps(steps)[0] / steps for _ in range(reps))
+            spl = S.substeps_per_launch
+            S.close()
+        finally:
+            P.close()
+        gbs = 4 * BYTES_PER_GP_SUBSTEP * Nxx[0] * Nxx[1] * Nxx[2] / (best * 1e-6) / 1e9
+        out["x".join(map(str, Nxx))] = {"us_per_step": round(best, 2), "substeps_per_launch": spl,
+                                        "accounting_GBs": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 3)}
+    return out
+
+
 def mixed_precision_q4():
     """The FP32 -> FP64 solver on the unequal-mass workload (q = 4, D = 10, seed-7 spins,
     128x128x16) beside the FP64 solver: steps, FP32 steps and seconds to log10 L2 <= -10 from the
@@ -508,6 +530,7 @@ def main():
     admbase = admbase_throughput(P)
     batched = batched_solve()
     mixed_q4 = mixed_precision_q4()
+    larger = larger_grids()
 
     cpu = None if args.no_cpu_baseline else run_cpu_baseline_subprocess()
     if cpu is not None and isinstance(cpu.get("admbase"), dict):
@@ -540,6 +563,7 @@ def main():
         "et_consumer_admbase": admbase,
         "batched_64_seed42": batched,
         "mixed_precision_q4_seed7": mixed_q4,
+        "larger_grids": larger,
         "cpu_baseline": cpu,
         "time_to_residual": {"target_log10": -10.0, "steps": r.steps, "seconds": r.seconds,
                              "log10_l2": r.log10_l2, "converged": bool(r.converged),
