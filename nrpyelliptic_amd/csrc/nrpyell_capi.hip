@@ -1534,6 +1534,9 @@ struct StepCacheEntry {
   std::vector<double> grid;
 };
 static std::map<const griddata_struct *, StepCacheEntry> g_step_cache;
+// Guards g_step_cache: MoL_step_forward_in_time holds it for its whole call, the functions that
+// free griddata memory while they evict. (Not g_mu: the step creates solvers, which take g_mu.)
+static std::mutex g_step_mu;
 
 static std::vector<double> step_grid_key(const griddata_struct *gd) {
   const paramstruct &p = gd->params;
@@ -1546,6 +1549,7 @@ static std::vector<double> step_grid_key(const griddata_struct *gd) {
 
 template <typename Pred>
 static void step_cache_evict(Pred pred) {
+  std::lock_guard<std::mutex> lk(g_step_mu);
   for (auto it = g_step_cache.begin(); it != g_step_cache.end();) {
     if (pred(it->second)) {
       nrpyell_solver_destroy(it->second.s);
@@ -1566,6 +1570,7 @@ static void step_cache_forget_buffer(const void *p) {
 
 extern "C" void MoL_step_forward_in_time(griddata_struct *griddata, const REAL dt) {
   ref_call("MoL_step_forward_in_time", [&]() {
+    std::lock_guard<std::mutex> lk(g_step_mu);
     nrpyell_solver *s = nullptr;
     const std::vector<double> key = step_grid_key(griddata);
     const MoL_gridfunctions_struct &gf = griddata->gridfuncs;

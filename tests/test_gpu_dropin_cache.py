@@ -88,3 +88,42 @@ def test_problem_replaced_on_the_same_grid():
             assert rel_maxnorm(got, want) <= 1e-12, k
         finally:
             Q.close()
+
+
+def test_steps_from_two_threads():
+    """MoL_step_forward_in_time from two host threads at once, on two griddata of different
+    grids (each call blocks until its step is done; the solver cache is shared): each state
+    equals the same steps run from one thread."""
+    import threading
+    cfgs = [config.shipped_free_parameters((16, 16, 6)), config.shipped_free_parameters((32, 16, 8))]
+    probs = [runtime.Problem(c) for c in cfgs]
+    try:
+        y0 = [np.random.default_rng(20 + k).uniform(-1e-2, 1e-2, 2 * P.ntot) for k, P in enumerate(probs)]
+        want = []
+        for P, y in zip(probs, y0):
+            P.set("y_n_gfs", y)
+            for _ in range(12):
+                P.mol_step()
+            want.append(P.get("y_n_gfs"))
+        for P, y in zip(probs, y0):
+            P.set("y_n_gfs", y)
+        errors = []
+
+        def run(P):
+            try:
+                for _ in range(12):
+                    P.mol_step()
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=run, args=(P,)) for P in probs]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join(timeout=60)
+        assert not errors and not any(t.is_alive() for t in threads)
+        for P, w in zip(probs, want):
+            assert np.array_equal(P.get("y_n_gfs"), w)
+    finally:
+        for P in probs:
+            P.close()
