@@ -101,17 +101,35 @@ __global__ void __launch_bounds__(256) admbase_kernel(AdmArgs a) {
   if (ok) {
     // sum in the order k (xx2), j (xx1), i (xx0 contiguous rows)
     double acc = 0.0;
-    for (int k = 0; k <= ORDER; k++) {
-      double a1 = 0.0;
-      for (int j = 0; j <= ORDER; j++) {
-        const double *row =
-            a.u + (long)f[0] + (long)a.dims[0] * ((long)(f[1] + j) + (long)a.dims[1] * (long)(f[2] + k));
-        double a0 = 0.0;
+    if constexpr (ORDER >= 5) {
+      // one xx2 plane of loads in flight at a time: with the loads of all (ORDER+1)^3 values
+      // hoisted the kernel needs 256 VGPRs and spills (order 6: 252 of them)
+#pragma unroll 1
+      for (int k = 0; k <= ORDER; k++) {
+        double a1 = 0.0;
+        for (int j = 0; j <= ORDER; j++) {
+          const double *row =
+              a.u + (long)f[0] + (long)a.dims[0] * ((long)(f[1] + j) + (long)a.dims[1] * (long)(f[2] + k));
+          double a0 = 0.0;
 #pragma unroll
-        for (int ii = 0; ii <= ORDER; ii++) a0 += w[0][ii] * row[ii];
-        a1 += w[1][j] * a0;
+          for (int ii = 0; ii <= ORDER; ii++) a0 += w[0][ii] * row[ii];
+          a1 += w[1][j] * a0;
+        }
+        acc += w[2][k] * a1;
       }
-      acc += w[2][k] * a1;
+    } else {
+      for (int k = 0; k <= ORDER; k++) {
+        double a1 = 0.0;
+        for (int j = 0; j <= ORDER; j++) {
+          const double *row =
+              a.u + (long)f[0] + (long)a.dims[0] * ((long)(f[1] + j) + (long)a.dims[1] * (long)(f[2] + k));
+          double a0 = 0.0;
+#pragma unroll
+          for (int ii = 0; ii <= ORDER; ii++) a0 += w[0][ii] * row[ii];
+          a1 += w[1][j] * a0;
+        }
+        acc += w[2][k] * a1;
+      }
     }
     const double uu = acc;
     // conformally_flat_BBH_compute_ADM_Cartesian_quantities_from_uu.c:36-103 and the lapse
