@@ -481,8 +481,9 @@ __device__ inline void radiation_point(const GridDims &g, const RadPoint &P, con
 }
 
 // One (which_gz, dirn) list, one thread per (point, gridfunction).
-__global__ void radiation_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs,
-                                      double c_u, double c_v, double finf_u, double finf_v) {
+__global__ void __launch_bounds__(64)
+    radiation_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs, double c_u,
+        double c_v, double finf_u, double finf_v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= 2 * n) return;
   const int gf = k / n;
@@ -502,8 +503,9 @@ void launch_radiation_list(const GridDims &g, int n, const RadPoint *pts, const 
 // recurrence rhs_p = A_p + (r_{p-1}/r_p)^3 (rhs_{p-1} - A_{p-1}), A = -c (df/dr + (f - finf)/r),
 // telescopes to rhs_p = A_p + (r_0/r_p)^3 (rhs_0 - A_0) with index 0 the last interior point, so
 // every (row, gridfunction, layer) is independent: one thread each.
-__global__ void radiation_rows_kernel(GridDims g, int n, RadLayers L, const double *gfs, double *rhs, double c_u,
-                                      double c_v, double finf_u, double finf_v) {
+__global__ void __launch_bounds__(64)
+    radiation_rows_kernel(GridDims g, int n, RadLayers L, const double *gfs, double *rhs, double c_u, double c_v,
+        double finf_u, double finf_v) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= 2 * NG * n) return;
   const int row = k % n, gf = (k / n) % 2, gz = k / (2 * n);
@@ -558,8 +560,9 @@ __device__ inline double sommerfeld_dr(const GridDims &g, const RadPoint &P, int
   return P.geo[w][1] * d0 + P.geo[w][2] * d1;
 }
 
-__global__ void sommerfeld_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs,
-                                       SommerfeldParams sp) {
+__global__ void __launch_bounds__(64)
+    sommerfeld_list_kernel(GridDims g, int n, const RadPoint *pts, const double *gfs, double *rhs,
+        SommerfeldParams sp) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= 2 * n) return;
   const int gf = k / n;
