@@ -20,11 +20,14 @@ SMALL = [config.shipped_free_parameters((16, 16, 6)), config.unequal_mass_q4((32
          config.random_batch(1, seed=5, grids=False, Nxx=(16, 40, 8))[0],
          config.random_batch(1, seed=9, grids=False, Nxx=(33, 16, 6))[0],
          config.random_batch(1, seed=42, grids=False, Nxx=(96, 128, 16))[0],
-         config.random_batch(1, seed=7, grids=False, Nxx=(160, 128, 16))[0]]
+         config.random_batch(1, seed=7, grids=False, Nxx=(160, 128, 16))[0],
+         config.random_batch(1, seed=13, grids=False, Nxx=(70, 18, 8))[0]]
 # 33: the tile holding the last interior i0 holds only that column (the one-sided BC stencil
-# reaches into the previous tile); 160x128x16: the largest grid of the batched workloads
+# reaches into the previous tile); 160x128x16: the largest grid of the batched workloads;
+# 70x18x8: Nxx1 divides neither the drop-in kernels' 4-row tiles, the fused kernels' 8-row tiles
+# nor the 16-row BC blocks
 IDS = ["shipped_16x16x6", "q4_32x32x8", "rand_24x16x6", "rand_16x40x8", "rand_33x16x6", "rand_96x128x16",
-       "max_160x128x16"]
+       "max_160x128x16", "rand_70x18x8"]
 
 
 def interior_mask(Ng):
@@ -92,6 +95,23 @@ def test_residual_bit_exact(pair):
     got, want = P.residual(st), O.residual(st)
     m = interior_mask(P.Ng)
     assert np.array_equal(got[m], want[m])
+
+
+def test_rhs_and_residual_bit_exact_on_a_large_ragged_grid():
+    """rhs_eval and residual_all_points on 200x130x16 (2 x 64 + 72 columns, 32 x 4 + 2 rows of the
+    drop-in kernels' tiles): bit-exact against the oracle at every interior point."""
+    cfg = config.random_batch(1, seed=17, grids=False, Nxx=(200, 130, 16))[0]
+    P, O = runtime.Problem(cfg), oracle.OracleProblem(cfg)
+    try:
+        use_oracle_inputs(P, O)
+        st = np.random.default_rng(7).uniform(-0.1, 0.1, 2 * P.ntot)
+        m = interior_mask(P.Ng)
+        got, want = P.rhs(st), O.rhs(st)
+        assert np.array_equal(got[np.concatenate([m, m])], want[np.concatenate([m, m])])
+        got, want = P.residual(st), O.residual(st)
+        assert np.array_equal(got[m], want[m])
+    finally:
+        P.close()
 
 
 def test_radiation_and_inner_bc(pair):
