@@ -448,6 +448,9 @@ def main():
                     help="only the warmup and the timed steps (the region the profiles cover)")
     ap.add_argument("--admbase-only", action="store_true",
                     help="only the solve and the Einstein Toolkit consumer path (ADMBase on 128^3 points)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only the 64-configuration seed-42 batched solve (the region the batched kernel "
+                         "profile covers)")
     ap.add_argument("--dropin-only", action="store_true",
                     help="only the reference-surface entry points, 200 calls each (the region the drop-in "
                          "kernel profile covers)")
@@ -456,6 +459,9 @@ def main():
         pmc_probe()
         return
 
+    if args.batched_only:
+        print(json.dumps({"batched_64_seed42": batched_solve()}), flush=True)
+        return
     if args.cpu_baseline_only:
         a = args.cpu_baseline_only
         print(json.dumps(cpu_baseline(a[0], a[1], a[2] if len(a) > 2 else 5, admbase=not args.no_admbase)))
