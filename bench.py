@@ -446,6 +446,9 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true")
     ap.add_argument("--timed-only", action="store_true",
                     help="only the warmup and the timed steps (the region the profiles cover)")
+    ap.add_argument("--one-substep", action="store_true",
+                    help="with --timed-only: time the one-substep-per-launch path instead of the production "
+                         "two-substep path (the region the one-substep kernel profile covers)")
     ap.add_argument("--admbase-only", action="store_true",
                     help="only the solve and the Einstein Toolkit consumer path (ADMBase on 128^3 points)")
     ap.add_argument("--batched-only", action="store_true",
@@ -477,7 +480,7 @@ def main():
     cfg = config.equal_mass_boosted()
     P = runtime.Problem(cfg)
     npts = cfg.Nxx[0] * cfg.Nxx[1] * cfg.Nxx[2]
-    S = P.solver(steps_per_graph=100)
+    S = P.solver(steps_per_graph=100, substeps_per_launch=1 if args.timed_only and args.one_substep else 0)
 
     if args.dropin_only:
         P.set("y_n_gfs", np.zeros(2 * P.ntot))
@@ -525,7 +528,7 @@ def main():
                 "seconds": r_et.seconds, "log10_l2": r_et.log10_l2, "converged": bool(r_et.converged)}
     # per-launch kernel durations (back-to-back launches, HIP events on the solver stream): the
     # two-substep kernels of the production path, and the one-substep kernels with the RK4 step
-    # they make (the drop-in MoL_step_forward_in_time, batched and FP32 paths use those)
+    # they make (the path of the Sommerfeld BC and of grids the two-substep kernel does not take)
     stage_us = {"stages12": round(S.time_substep(5, 200), 3), "stages34": round(S.time_substep(6, 200), 3)}
     S1 = P.solver(steps_per_graph=100, substeps_per_launch=1)
     stage_us.update({f"one_substep_stage{k}": round(S1.time_substep(k if k > 1 else 0, 200), 3) for k in (1, 2, 3, 4)})
